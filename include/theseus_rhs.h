@@ -1,0 +1,182 @@
+/*
+ * theseus_rhs.h -- C-ABI of the MI355X DGSEM right-hand-side engine.
+ *
+ * This is the drop-in boundary for the reference's RHS evaluation
+ * (chess-uiuc/Theseus):
+ *
+ *   reference                                   | here
+ *   --------------------------------------------+------------------------------------
+ *   MakeRHSOperator(runTime, vfes, ...)         | theseus_rhs_create(desc, &handle)
+ *     include/SimFactory.hpp:22-31,             |
+ *     include/SimFactory_impl.hpp:63-488        |
+ *   RHSOperator::Finalize(t)                    | (done inside theseus_rhs_create)
+ *     include/RHSOperator_impl.hpp:273-314      |
+ *   RHSOperator<P>::Mult(u, dudt)               | theseus_rhs_eval(h, u, dudt, stream)
+ *     include/RHSOperator_impl.hpp:719-793      |
+ *   RHSOperator<P>::EstimateStability(u)        | theseus_rhs_estimate_stability(h, u, r)
+ *     include/RHSOperator_impl.hpp:10-270       |
+ *   mfem::RK3SSPSolver::Step(x, t, dt) driving  | theseus_rhs_step_ssprk3(h, u, t, dt, s)
+ *     Mult (src/Simulation.cpp:265-268,1255)    |
+ *   RHSOperatorBase::~RHSOperatorBase           | theseus_rhs_destroy(h)
+ *
+ * Data layout of every state vector (u, dudt): the reference's L-vector,
+ * "byNODES" (include/Simulation.hpp:68-69):
+ *     u[q * (ne * ndof) + e * ndof + node]
+ * with q the equation (rho, rho*u_d..., rho*E), e the element and node the
+ * lexicographic GLL node index i + Np*(j + Np*k) within the element.
+ *
+ * All device pointers are plain HIP device pointers (hipMalloc'd or any other
+ * device allocation); no torch or MFEM types cross this boundary.  Every call
+ * returns 0 on success or a negative THESEUS_ERR_* code; theseus_last_error()
+ * returns a thread-local message for the last failure.  A handle is not
+ * reentrant: calls on one handle must not overlap (the reference's Mult is
+ * likewise const but not thread-safe, include/RHSOperator.hpp:129-130).
+ */
+#ifndef THESEUS_RHS_H
+#define THESEUS_RHS_H
+
+#if defined(__GNUC__) || defined(__clang__)
+#define THESEUS_API __attribute__((visibility("default")))
+#else
+#define THESEUS_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define THESEUS_OK 0
+#define THESEUS_ERR_INVALID_ARGUMENT (-1)
+#define THESEUS_ERR_UNSUPPORTED (-2)
+#define THESEUS_ERR_DEVICE (-3)
+#define THESEUS_ERR_NONFINITE (-4)
+
+/* ---- flow model (SimFactory_impl.hpp:119-123 "flow_model") ------------- */
+#define THESEUS_FLOW_EULER 0
+#define THESEUS_FLOW_NAVIER_STOKES 1
+
+/* ---- two-point volume fluxes (A3) -------------------------------------- */
+#define THESEUS_VOLUME_CENTRAL 0       /* RoeFlux.hpp:18-45, HLLFlux.hpp:19-51, LaxFriedrichsFlux.hpp:20-52 */
+#define THESEUS_VOLUME_CHANDRASHEKAR 1 /* ChandrashekarFlux.hpp:20-89 */
+#define THESEUS_VOLUME_RANOCHA 2       /* RanochaFlux.hpp:20-79 */
+#define THESEUS_VOLUME_PIROZZOLI 3     /* PirozzoliFlux.hpp:19-71 */
+
+/* ---- surface (face) fluxes (A5) ---------------------------------------- */
+#define THESEUS_SURFACE_CHANDRASHEKAR 0 /* EC + Scalar_dissipation, ChandrashekarFlux.hpp:91-155 */
+#define THESEUS_SURFACE_RANOCHA 1       /* EC + Roe_dissipation, RanochaFlux.hpp:81-134 */
+#define THESEUS_SURFACE_PIROZZOLI 2     /* KEP + Roe_dissipation, PirozzoliFlux.hpp:73-120 */
+#define THESEUS_SURFACE_ROE 3           /* RoeFlux.hpp:50-157 */
+#define THESEUS_SURFACE_HLL 4           /* HLLFlux.hpp:53-116 */
+#define THESEUS_SURFACE_LLF 5           /* LaxFriedrichsFlux.hpp:54-77 */
+
+/* ---- boundary condition types (bc_cache_utilities.hpp:11-23) ------------ */
+#define THESEUS_BC_SUPERSONIC_OUTFLOW 2 /* F*(q,q,n), bc_kernels.hpp:236-238 */
+
+/* ---- precision modes ---------------------------------------------------- */
+#define THESEUS_PRECISION_FP64 0
+#define THESEUS_PRECISION_MIXED 1 /* FP32 fluxes, FP64 stage accumulation */
+
+/*
+ * Case descriptor.  Host pointers; copied into device memory by create, so
+ * the caller may free them afterwards.
+ *
+ * Geometry is affine per element (J and the adjugate of dx/dxi on the
+ * reference cube [0,1]^dim), which is what the reference's cache holds for
+ * every node of an affine element (dgsem_cache_utilities.hpp:469-514).
+ *
+ * Connectivity: face_neighbor[e*2*dim + 2*d + s] for local face (d, s),
+ * d = reference direction, s = 0 (xi_d = 0 side) or 1 (xi_d = 1 side):
+ *   >= 0 : the neighbouring element, whose opposite face (d, 1-s) is
+ *          matched node-for-node (same in-face lexicographic order);
+ *   <  0 : a boundary face with bc index -(value+1) into bc_type[].
+ */
+typedef struct theseus_case_desc {
+  int dim;           /* 2 or 3 */
+  int order;         /* polynomial order p (Np = p+1 GLL nodes per direction) */
+  int num_elements;  /* ne */
+  int flow_model;    /* THESEUS_FLOW_* */
+  int volume_flux;   /* THESEUS_VOLUME_* */
+  int surface_flux;  /* THESEUS_SURFACE_* */
+  int precision;     /* THESEUS_PRECISION_* */
+  int blending;      /* 1: Persson-Peraire indicator + subcell FV blending (SUBCELL_FV_BLENDING) */
+
+  /* gas and transport constants (Physics.hpp:13-43) */
+  double gamma;
+  double Pr;
+  double R_gas;
+  double mu;
+  double mu_bulk;    /* reference default 2/3 */
+
+  /* blending constants (RHSOperator.hpp:33-35,53,64) */
+  double alpha_max;  /* reference default 0.5 */
+  double alpha_min;  /* reference default 0.001 */
+
+  const double *elem_jac;      /* [ne] */
+  const double *elem_adj;      /* [ne][dim][dim], row = reference direction */
+  const int *face_neighbor;    /* [ne][2*dim] */
+
+  int num_bc;
+  const int *bc_type;          /* [num_bc] THESEUS_BC_* */
+} theseus_case_desc;
+
+typedef struct theseus_rhs theseus_rhs; /* opaque */
+
+/* Stability rates (StabilityEstimate.hpp:40-55): total = max(adv, surf) + diff */
+typedef struct theseus_stability {
+  double advective_rate;
+  double surface_rate;
+  double diffusive_rate;
+} theseus_stability;
+
+/* Replaces MakeRHSOperator + RHSOperator::Finalize (SimFactory_impl.hpp:63-488,
+ * RHSOperator_impl.hpp:273-314).  Allocates the device-resident cache. */
+THESEUS_API int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out);
+
+/* Replaces RHSOperatorBase::~RHSOperatorBase. */
+THESEUS_API int theseus_rhs_destroy(theseus_rhs *h);
+
+/* Number of scalar dofs per equation (ne * Np^dim) and equations (dim+2). */
+THESEUS_API int theseus_rhs_sizes(const theseus_rhs *h, long long *ndof_total, int *num_equations);
+
+/* Replaces RHSOperator<P>::Mult(u, dudt) (RHSOperator_impl.hpp:719-793):
+ * dudt = L(u) with u, dudt FP64 L-vectors in device memory.  Enqueued on
+ * `stream` (a hipStream_t; NULL = default stream), asynchronous. */
+THESEUS_API int theseus_rhs_eval(theseus_rhs *h, const double *u, double *dudt, void *stream);
+
+/* Replaces RHSOperator<P>::EstimateStability(u) (RHSOperator_impl.hpp:10-270)
+ * followed by the host max-reduction; the reduction runs on the device and
+ * only three doubles come back.  Synchronises `stream`. */
+THESEUS_API int theseus_rhs_estimate_stability(theseus_rhs *h, const double *u,
+                                   theseus_stability *out, void *stream);
+
+/* One Shu-Osher SSP-RK3 step of u in place (what mfem::RK3SSPSolver::Step
+ * does around Mult, src/Simulation.cpp:1255), stage times t, t+dt, t+dt/2.
+ * Uses handle-owned stage buffers; asynchronous on `stream`. */
+THESEUS_API int theseus_rhs_step_ssprk3(theseus_rhs *h, double *u, double t, double dt, void *stream);
+
+/* Copies the per-element blending coefficient alpha of the last eval/step
+ * (ne doubles, the reference's operator_cache.alpha,
+ * RHSOperator_impl.hpp:411-437) into the device buffer `alpha`; fails with
+ * THESEUS_ERR_INVALID_ARGUMENT when blending is off.  Asynchronous. */
+THESEUS_API int theseus_rhs_get_alpha(const theseus_rhs *h, double *alpha, void *stream);
+
+/* ---- instrumentation (no counterpart in the reference) -------------------
+ * Per-kernel HIP-event timing for measurement tools: when enabled, every
+ * kernel the handle enqueues is bracketed by hipEvents on its own stream, up
+ * to `capacity` launches per kernel kind.  Kinds: 0 = the fused
+ * volume+face kernel (Euler) / Navier-Stokes pass 2, 1 = Navier-Stokes
+ * pass 1 (BR1 gradient + viscous flux).  enable(h, 0) disables and clears. */
+THESEUS_API int theseus_rhs_enable_kernel_timing(theseus_rhs *h, int capacity);
+/* Synchronises the recorded events; returns the mean duration in ms and the
+ * number of timed launches of one kernel kind. */
+THESEUS_API int theseus_rhs_kernel_timing(theseus_rhs *h, int kind, double *mean_ms, int *count);
+
+/* Thread-local message of the last failing call. */
+THESEUS_API const char *theseus_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* THESEUS_RHS_H */

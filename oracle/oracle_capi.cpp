@@ -1,0 +1,145 @@
+// TEST INFRASTRUCTURE ONLY.  extern "C" shim over the CPU FP64 restatement
+// (oracle_physics.hpp, oracle_operator.hpp) so that pytest (ctypes) and the
+// bench's cpu_baseline leg can call it.  The product never loads this library.
+#include <cstdio>
+#include <exception>
+#include <string>
+
+#include "oracle_operator.hpp"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using namespace oracle;
+
+static thread_local std::string g_err;
+
+extern "C" {
+
+const char *oracle_last_error() { return g_err.c_str(); }
+
+int oracle_num_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void oracle_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+double oracle_log_mean(double x, double y, double eps) { return ComputeLogMean(x, y, eps); }
+
+void oracle_volume_flux(int id, int dim, double gamma, const double *q1, const double *q2, const double *m1,
+                        const double *m2, double *F) {
+  Gas g(dim, gamma, 0.72, 287.05, 0.02, 2.0 / 3.0);
+  VolumeFlux(id, g, q1, q2, m1, m2, F);
+}
+
+void oracle_face_flux(int id, int dim, double gamma, const double *q1, const double *q2, const double *n,
+                      double *F) {
+  Gas g(dim, gamma, 0.72, 287.05, 0.02, 2.0 / 3.0);
+  FaceFlux(id, g, q1, q2, n, F);
+}
+
+void oracle_physical_flux(int dim, double gamma, const double *q, double *F /* [MAXEQ*MAXDIM] */) {
+  Gas g(dim, gamma, 0.72, 287.05, 0.02, 2.0 / 3.0);
+  double f[MAXEQ][MAXDIM] = {{0}};
+  ComputeInviscidFluxKernel(g, q, f);
+  for (int e = 0; e < MAXEQ; ++e)
+    for (int d = 0; d < MAXDIM; ++d) F[e * MAXDIM + d] = f[e][d];
+}
+
+void oracle_entropy_state(int dim, double gamma, const double *q, double *E) {
+  Gas g(dim, gamma, 0.72, 287.05, 0.02, 2.0 / 3.0);
+  g.entropy_state(q, E);
+}
+
+void oracle_grad_entropy_to_grad_prim(int dim, double gamma, const double *q, const double *dE, double *dP) {
+  Gas g(dim, gamma, 0.72, 287.05, 0.02, 2.0 / 3.0);
+  g.grad_entropy_to_grad_prim(q, dE, dP);
+}
+
+void oracle_viscous_flux(int dim, double gamma, double Pr, double R, double mu, double mu_bulk, const double *q,
+                         const double *dx, const double *dy, const double *dz, double *F /* [MAXEQ*MAXDIM] */) {
+  Gas g(dim, gamma, Pr, R, mu, mu_bulk);
+  double f[MAXEQ][MAXDIM] = {{0}};
+  ComputeViscousFluxKernel(g, q, dx, dy, dz, f);
+  for (int e = 0; e < MAXEQ; ++e)
+    for (int d = 0; d < MAXDIM; ++d) F[e * MAXDIM + d] = f[e][d];
+}
+
+// x, w: Np; D, Dhat, Dhat2: Np*Np stored [i*Np + l]
+void oracle_basis(int p, double *x, double *w, double *D, double *Dhat, double *Dhat2) {
+  Basis b(p);
+  for (int i = 0; i < b.Np; ++i) {
+    x[i] = b.x[i];
+    w[i] = b.w[i];
+  }
+  for (int k = 0; k < b.Np * b.Np; ++k) {
+    D[k] = b.D[k];
+    Dhat[k] = b.Dhat[k];
+    Dhat2[k] = b.Dhat2[k];
+  }
+}
+
+void *oracle_case_create(const theseus_case_desc *desc) {
+  try {
+    return new Case(*desc);
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void oracle_case_destroy(void *h) { delete static_cast<Case *>(h); }
+
+// dudt = L(U); alpha (may be NULL) receives the per-element blending coefficient.
+int oracle_rhs(void *h, const double *U, double *dudt, double *alpha) {
+  try {
+    const Case &c = *static_cast<Case *>(h);
+    std::vector<double> a;
+    RHS(c, U, dudt, &a);
+    if (alpha && !a.empty())
+      for (int e = 0; e < c.ne; ++e) alpha[e] = a[e];
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int oracle_blending(void *h, const double *U, double *alpha, double *eta) {
+  const Case &c = *static_cast<Case *>(h);
+  BlendingCoefficient(c, U, alpha, eta);
+  return 0;
+}
+
+int oracle_estimate_stability(void *h, const double *U, double *out3) {
+  try {
+    EstimateStability(*static_cast<Case *>(h), U, out3);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int oracle_ssprk3_step(void *h, double *U, double dt) {
+  try {
+    SSPRK3Step(*static_cast<Case *>(h), U, dt);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU FP64 oracle
+(liboracle.so, built from oracle_capi.cpp by oracle/Makefile).
+
+Only tests/, the smoke check and the bench's cpu_baseline leg may use it, and
+only as the checker."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        raise FileNotFoundError(f"{LIB} not built (make -C oracle)")
+    lib = C.CDLL(str(LIB))
+    lib.oracle_last_error.restype = C.c_char_p
+    lib.oracle_num_threads.restype = C.c_int
+    lib.oracle_set_num_threads.argtypes = [C.c_int]
+    lib.oracle_log_mean.argtypes = [C.c_double, C.c_double, C.c_double]
+    lib.oracle_log_mean.restype = C.c_double
+    lib.oracle_volume_flux.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_face_flux.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+    lib.oracle_physical_flux.argtypes = [C.c_int, C.c_double, _dp, _dp]
+    lib.oracle_entropy_state.argtypes = [C.c_int, C.c_double, _dp, _dp]
+    lib.oracle_grad_entropy_to_grad_prim.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp]
+    lib.oracle_viscous_flux.argtypes = [C.c_int] + [C.c_double] * 5 + [_dp] * 5
+    lib.oracle_basis.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_case_create.argtypes = [C.c_void_p]
+    lib.oracle_case_create.restype = C.c_void_p
+    lib.oracle_case_destroy.argtypes = [C.c_void_p]
+    lib.oracle_rhs.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    lib.oracle_rhs.restype = C.c_int
+    lib.oracle_blending.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    lib.oracle_estimate_stability.argtypes = [C.c_void_p, _dp, _dp]
+    lib.oracle_estimate_stability.restype = C.c_int
+    lib.oracle_ssprk3_step.argtypes = [C.c_void_p, _dp, C.c_double]
+    lib.oracle_ssprk3_step.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def arr(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def log_mean(x, y, eps=1e-4):
+    return load().oracle_log_mean(float(x), float(y), float(eps))
+
+
+def volume_flux(fid, dim, gamma, q1, q2, m1, m2):
+    q1, q2, m1, m2 = arr(q1), arr(q2), arr(m1), arr(m2)
+    F = np.zeros(dim + 2)
+    load().oracle_volume_flux(fid, dim, gamma, _p(q1), _p(q2), _p(m1), _p(m2), _p(F))
+    return F
+
+
+def face_flux(fid, dim, gamma, q1, q2, n):
+    q1, q2, n = arr(q1), arr(q2), arr(n)
+    F = np.zeros(dim + 2)
+    load().oracle_face_flux(fid, dim, gamma, _p(q1), _p(q2), _p(n), _p(F))
+    return F
+
+
+def physical_flux(dim, gamma, q):
+    q = arr(q)
+    F = np.zeros(15)
+    load().oracle_physical_flux(dim, gamma, _p(q), _p(F))
+    return F.reshape(5, 3)[: dim + 2, :dim]
+
+
+def entropy_state(dim, gamma, q):
+    q = arr(q)
+    E = np.zeros(dim + 2)
+    load().oracle_entropy_state(dim, gamma, _p(q), _p(E))
+    return E
+
+
+def grad_entropy_to_grad_prim(dim, gamma, q, dE):
+    q, dE = arr(q), arr(dE)
+    dP = np.zeros(dim + 2)
+    load().oracle_grad_entropy_to_grad_prim(dim, gamma, _p(q), _p(dE), _p(dP))
+    return dP
+
+
+def viscous_flux(dim, gamma, Pr, R, mu, mu_bulk, q, dx, dy, dz):
+    q, dx, dy, dz = arr(q), arr(dx), arr(dy), arr(dz)
+    F = np.zeros(15)
+    load().oracle_viscous_flux(dim, gamma, Pr, R, mu, mu_bulk, _p(q), _p(dx), _p(dy), _p(dz), _p(F))
+    return F.reshape(5, 3)[: dim + 2, :dim]
+
+
+def basis(p):
+    Np = p + 1
+    x, w = np.zeros(Np), np.zeros(Np)
+    D, Dh, Dh2 = np.zeros(Np * Np), np.zeros(Np * Np), np.zeros(Np * Np)
+    load().oracle_basis(p, _p(x), _p(w), _p(D), _p(Dh), _p(Dh2))
+    return x, w, D.reshape(Np, Np), Dh.reshape(Np, Np), Dh2.reshape(Np, Np)
+
+
+class OracleCase:
+    """The oracle's operator for one case descriptor (theseus_case_desc)."""
+
+    def __init__(self, desc):
+        lib = load()
+        self._lib = lib
+        self._h = lib.oracle_case_create(C.byref(desc))
+        if not self._h:
+            raise RuntimeError(lib.oracle_last_error().decode())
+        self.ne = desc.num_elements
+
+    def rhs(self, U):
+        U = arr(U)
+        out = np.zeros_like(U)
+        alpha = np.zeros(self.ne)
+        rc = self._lib.oracle_rhs(self._h, _p(U), _p(out), _p(alpha))
+        if rc:
+            raise RuntimeError(self._lib.oracle_last_error().decode())
+        self.alpha = alpha
+        return out
+
+    def blending(self, U):
+        U = arr(U)
+        alpha, eta = np.zeros(self.ne), np.zeros(self.ne)
+        self._lib.oracle_blending(self._h, _p(U), _p(alpha), _p(eta))
+        return alpha, eta
+
+    def stability(self, U):
+        U = arr(U)
+        out = np.zeros(3)
+        rc = self._lib.oracle_estimate_stability(self._h, _p(U), _p(out))
+        if rc:
+            raise RuntimeError(self._lib.oracle_last_error().decode())
+        return out
+
+    def step(self, U, dt):
+        U = arr(U).copy()
+        rc = self._lib.oracle_ssprk3_step(self._h, _p(U), float(dt))
+        if rc:
+            raise RuntimeError(self._lib.oracle_last_error().decode())
+        return U
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.oracle_case_destroy(self._h)
+        except Exception:
+            pass

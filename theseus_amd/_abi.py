@@ -1,0 +1,127 @@
+"""ctypes view of the C-ABI in include/theseus_rhs.h.
+
+Loading is explicit and loud: ``load()`` raises if libtheseus_rhs.so has not
+been built; nothing in this package falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libtheseus_rhs.so"
+
+# error codes
+OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_UNSUPPORTED = -2
+ERR_DEVICE = -3
+
+FLOW_EULER = 0
+FLOW_NAVIER_STOKES = 1
+
+VOLUME_CENTRAL = 0
+VOLUME_CHANDRASHEKAR = 1
+VOLUME_RANOCHA = 2
+VOLUME_PIROZZOLI = 3
+
+SURFACE_CHANDRASHEKAR = 0
+SURFACE_RANOCHA = 1
+SURFACE_PIROZZOLI = 2
+SURFACE_ROE = 3
+SURFACE_HLL = 4
+SURFACE_LLF = 5
+
+BC_SUPERSONIC_OUTFLOW = 2
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+EXPORTED_SYMBOLS = (
+    "theseus_rhs_create",
+    "theseus_rhs_destroy",
+    "theseus_rhs_sizes",
+    "theseus_rhs_eval",
+    "theseus_rhs_estimate_stability",
+    "theseus_rhs_step_ssprk3",
+    "theseus_rhs_get_alpha",
+    "theseus_rhs_enable_kernel_timing",
+    "theseus_rhs_kernel_timing",
+    "theseus_last_error",
+)
+
+
+class CaseDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("order", C.c_int),
+        ("num_elements", C.c_int),
+        ("flow_model", C.c_int),
+        ("volume_flux", C.c_int),
+        ("surface_flux", C.c_int),
+        ("precision", C.c_int),
+        ("blending", C.c_int),
+        ("gamma", C.c_double),
+        ("Pr", C.c_double),
+        ("R_gas", C.c_double),
+        ("mu", C.c_double),
+        ("mu_bulk", C.c_double),
+        ("alpha_max", C.c_double),
+        ("alpha_min", C.c_double),
+        ("elem_jac", C.POINTER(C.c_double)),
+        ("elem_adj", C.POINTER(C.c_double)),
+        ("face_neighbor", C.POINTER(C.c_int)),
+        ("num_bc", C.c_int),
+        ("bc_type", C.POINTER(C.c_int)),
+    ]
+
+
+class Stability(C.Structure):
+    _fields_ = [("advective_rate", C.c_double), ("surface_rate", C.c_double), ("diffusive_rate", C.c_double)]
+
+
+_lib = None
+
+
+def load(path: Path | str | None = None) -> C.CDLL:
+    """Load libtheseus_rhs.so (raises FileNotFoundError when it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(f"{p} is not built; run `python -m theseus_amd.build` (or __graft_entry__.build())")
+    lib = C.CDLL(str(p))
+    lib.theseus_rhs_create.argtypes = [C.POINTER(CaseDesc), C.POINTER(C.c_void_p)]
+    lib.theseus_rhs_create.restype = C.c_int
+    lib.theseus_rhs_destroy.argtypes = [C.c_void_p]
+    lib.theseus_rhs_destroy.restype = C.c_int
+    lib.theseus_rhs_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
+    lib.theseus_rhs_sizes.restype = C.c_int
+    lib.theseus_rhs_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.theseus_rhs_eval.restype = C.c_int
+    lib.theseus_rhs_estimate_stability.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Stability), C.c_void_p]
+    lib.theseus_rhs_estimate_stability.restype = C.c_int
+    lib.theseus_rhs_step_ssprk3.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p]
+    lib.theseus_rhs_step_ssprk3.restype = C.c_int
+    lib.theseus_rhs_get_alpha.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.theseus_rhs_get_alpha.restype = C.c_int
+    lib.theseus_rhs_enable_kernel_timing.argtypes = [C.c_void_p, C.c_int]
+    lib.theseus_rhs_enable_kernel_timing.restype = C.c_int
+    lib.theseus_rhs_kernel_timing.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.theseus_rhs_kernel_timing.restype = C.c_int
+    lib.theseus_last_error.argtypes = []
+    lib.theseus_last_error.restype = C.c_char_p
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class TheseusError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"theseus error {code}: {msg}")
+        self.code = code
+
+
+def check(lib: C.CDLL, rc: int) -> None:
+    if rc != OK:
+        raise TheseusError(rc, lib.theseus_last_error().decode(errors="replace"))

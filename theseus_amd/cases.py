@@ -1,0 +1,128 @@
+"""Initial conditions and the workload definitions the engine is measured on.
+
+ICs produce the byNODES L-vector (numpy float64, shape (neq*ndof_total,))
+by nodal interpolation at the GLL nodes, as the reference's
+``ProjectCoefficient`` does for a DG GLL space (src/Simulation.cpp:1076-1109).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _abi as A
+from .config import RunTime
+from .mesh import BoxMesh, box_mesh, gll_nodes
+
+
+def _pack(rho, mom, E):
+    return np.concatenate([rho[None], np.asarray(mom), E[None]], axis=0).reshape(-1)
+
+
+def taylor_green(mesh: BoxMesh, order: int, gamma: float = 1.4, Ma: float = 0.1) -> np.ndarray:
+    """TaylorGreenVortexIC (StateInit.hpp:139-162)."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    x, y, z = X[0], X[1], X[2]
+    p0 = 1.0 / (gamma * Ma * Ma)
+    den = np.ones_like(x)
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(x)
+    p = p0 + 1.0 / 16.0 * (np.cos(2.0 * x) + np.cos(2.0 * y)) * (np.cos(2.0 * z) + 2)
+    E = p / (gamma - 1.0) + 0.5 * den * (u * u + v * v + w * w)
+    return _pack(den, [den * u, den * v, den * w], E)
+
+
+def shu_vortex(mesh: BoxMesh, order: int, gamma: float = 1.4, strength: float = 5.0,
+               center=(0.0, 0.0), u_inf=(1.0, 1.0)) -> np.ndarray:
+    """Textbook isentropic vortex (Shu): rho_inf = p_inf = 1, u_inf = (1,1)."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    dx, dy = X[0] - center[0], X[1] - center[1]
+    r2 = dx * dx + dy * dy
+    b = strength
+    du = b / (2 * np.pi) * np.exp(0.5 * (1 - r2))
+    T = 1.0 - (gamma - 1) * b * b / (8 * gamma * np.pi**2) * np.exp(1 - r2)
+    rho = T ** (1.0 / (gamma - 1))
+    p = rho**gamma
+    u = u_inf[0] - du * dy
+    v = u_inf[1] + du * dx
+    E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v)
+    return _pack(rho, [rho * u, rho * v], E)
+
+
+def reference_isentropic_vortex(mesh: BoxMesh, order: int, radius=0.2, Minf=0.5, beta=0.2,
+                                R_gas=287.05, gamma=1.4) -> np.ndarray:
+    """IsentropicVortexIC (StateInit.hpp:392-434), the reference's own vortex."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    x, y = X[0], X[1]
+    vel_inf, den_inf = 1.0, 1.0
+    pres_inf = (den_inf / gamma) * (vel_inf / Minf) * (vel_inf / Minf)
+    temp_inf = pres_inf / (den_inf * R_gas)
+    r2rad = (x * x + y * y) / (radius * radius)
+    shrinv1 = 1.0 / (gamma - 1.0)
+    velX = vel_inf * (1 - beta * y / radius * np.exp(-0.5 * r2rad))
+    velY = vel_inf * beta * x / radius * np.exp(-0.5 * r2rad)
+    vel2 = velX * velX + velY * velY
+    specific_heat = R_gas * gamma * shrinv1
+    temp = temp_inf - 0.5 * (vel_inf * beta) ** 2 / specific_heat * np.exp(-r2rad)
+    den = den_inf * (temp / temp_inf) ** shrinv1
+    pres = den * R_gas * temp
+    energy = shrinv1 * pres / den + 0.5 * vel2
+    return _pack(den, [den * velX, den * velY], den * energy)
+
+
+def four_quadrant_riemann(mesh: BoxMesh, order: int, gamma: float = 1.4, split=(0.5, 0.5)) -> np.ndarray:
+    """2-D four-quadrant Riemann problem (configuration with quadrant states
+    NE (1.5,0,0,1.5), NW (0.5323,1.206,0,0.3), SW (0.138,1.206,1.206,0.029),
+    SE (0.5323,0,1.206,0.3))."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    east = X[0] >= split[0]
+    north = X[1] >= split[1]
+    states = {
+        (True, True): (1.5, 0.0, 0.0, 1.5),
+        (False, True): (0.5323, 1.206, 0.0, 0.3),
+        (False, False): (0.138, 1.206, 1.206, 0.029),
+        (True, False): (0.5323, 0.0, 1.206, 0.3),
+    }
+    rho = np.empty_like(X[0])
+    u = np.empty_like(X[0])
+    v = np.empty_like(X[0])
+    p = np.empty_like(X[0])
+    for (ee, nn), (r, uu, vv, pp) in states.items():
+        m = (east == ee) & (north == nn)
+        rho[m], u[m], v[m], p[m] = r, uu, vv, pp
+    E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v)
+    return _pack(rho, [rho * u, rho * v], E)
+
+
+# ---------------------------------------------------------------------------
+# Workloads (sizes as the engine is benchmarked on)
+# ---------------------------------------------------------------------------
+
+def c1_vortex2d(n: int = 16, order: int = 3):
+    """2-D Euler isentropic vortex, periodic [-5,5]^2, Chandrashekar volume + Roe face."""
+    mesh = box_mesh(2, n, -5.0, 5.0, periodic=True)
+    rt = RunTime(dim=2, order=order, flow_model=A.FLOW_EULER, volume_flux=A.VOLUME_CHANDRASHEKAR,
+                 surface_flux=A.SURFACE_ROE, cfl=0.5, blending=True)
+    return rt, mesh, shu_vortex(mesh, order, rt.gamma)
+
+
+def c2_tgv3d(n: int = 32, order: int = 3, Ma: float = 0.1, Re: float = 1600.0, Pr: float = 0.71):
+    """3-D compressible Navier-Stokes Taylor-Green vortex, periodic [0,2pi]^3,
+    Ranocha volume + Lax-Friedrichs face, BR1, constant viscosity mu = 1/Re
+    (rho0 = 1, U0 = 1, L = 1)."""
+    mesh = box_mesh(3, n, 0.0, 2 * np.pi, periodic=True)
+    rt = RunTime(dim=3, order=order, flow_model=A.FLOW_NAVIER_STOKES, volume_flux=A.VOLUME_RANOCHA,
+                 surface_flux=A.SURFACE_LLF, Pr=Pr, mu=1.0 / Re, blending=True)
+    return rt, mesh, taylor_green(mesh, order, rt.gamma, Ma)
+
+
+def c3_riemann2d(n: int = 128, order: int = 4):
+    """2-D four-quadrant Riemann problem on [0,1]^2, transmissive boundaries,
+    central volume + HLL face (reference pairing 'hll'), blending on."""
+    mesh = box_mesh(2, n, 0.0, 1.0, periodic=False, bc=A.BC_SUPERSONIC_OUTFLOW)
+    rt = RunTime(dim=2, order=order, flow_model=A.FLOW_EULER, volume_flux=A.VOLUME_CENTRAL,
+                 surface_flux=A.SURFACE_HLL, blending=True)
+    return rt, mesh, four_quadrant_riemann(mesh, order, rt.gamma)

@@ -1,0 +1,17 @@
+// Kernel instantiations: reference 'llf': central volume + local Lax-Friedrichs face.
+#include "registry.hpp"
+
+namespace thx {
+namespace {
+template <int DIM>
+struct Orders {
+  FluxRegistrar<DIM, 0, 5, 1> r1;
+  FluxRegistrar<DIM, 0, 5, 2> r2;
+  FluxRegistrar<DIM, 0, 5, 3> r3;
+  FluxRegistrar<DIM, 0, 5, 4> r4;
+  FluxRegistrar<DIM, 0, 5, 5> r5;
+};
+Orders<2> orders2;
+Orders<3> orders3;
+}  // namespace
+}  // namespace thx

@@ -1,0 +1,116 @@
+"""Affine box meshes: the geometry the reference's MFEM setup would produce
+for a Cartesian box of quads/hexes (dgsem_cache_utilities.hpp:469-639),
+restated in closed form.
+
+Elements are numbered lexicographically (x fastest).  The reference
+coordinate of every element is [0,1]^dim (MFEM's reference cube), so for an
+element of size h the Jacobian determinant is prod(h) and the adjugate of
+dx/dxi = diag(h) is diag(prod(h)/h_d).  Local faces are ordered
+(-x, +x, -y, +y, -z, +z); a periodic direction wraps, a non-periodic one
+gets boundary code -(bc+1).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class BoxMesh:
+    dim: int
+    n: tuple
+    lower: tuple
+    upper: tuple
+    periodic: tuple
+    elem_jac: np.ndarray = field(repr=False)
+    elem_adj: np.ndarray = field(repr=False)
+    face_neighbor: np.ndarray = field(repr=False)
+    bc_type: np.ndarray = field(repr=False)
+
+    @property
+    def num_elements(self) -> int:
+        return int(np.prod(self.n))
+
+    @property
+    def h(self) -> np.ndarray:
+        return (np.asarray(self.upper, float) - np.asarray(self.lower, float)) / np.asarray(self.n, float)
+
+    def element_index(self, idx) -> int:
+        e = 0
+        for d in reversed(range(self.dim)):
+            e = e * self.n[d] + idx[d]
+        return e
+
+    def node_coordinates(self, x01: np.ndarray) -> np.ndarray:
+        """Physical coordinates of every GLL node, shape (dim, ne*ndof) in the
+        L-vector dof order e*ndof + (i + Np*(j + Np*k))."""
+        Np = len(x01)
+        dim = self.dim
+        h = self.h
+        lo = np.asarray(self.lower, float)
+        ne = self.num_elements
+        ndof = Np**dim
+        coords = np.empty((dim, ne, ndof))
+        eidx = np.indices(self.n[::-1]).reshape(dim, -1)[::-1]  # element multi-index, x fastest
+        nidx = np.indices((Np,) * dim).reshape(dim, -1)[::-1]    # node multi-index, i fastest
+        for d in range(dim):
+            coords[d] = lo[d] + h[d] * (eidx[d][:, None] + x01[nidx[d]][None, :])
+        return coords.reshape(dim, ne * ndof)
+
+
+def box_mesh(dim: int, n, lower, upper, periodic=True, bc: int = 2) -> BoxMesh:
+    """Cartesian box of n[0] x n[1] (x n[2]) affine elements.
+
+    periodic: bool or per-direction tuple.  Non-periodic faces get boundary
+    index 0 whose type is `bc` (default: supersonic outflow = transmissive).
+    """
+    n = tuple(int(v) for v in (n if np.iterable(n) else [n] * dim))
+    lower = tuple(float(v) for v in (lower if np.iterable(lower) else [lower] * dim))
+    upper = tuple(float(v) for v in (upper if np.iterable(upper) else [upper] * dim))
+    per = tuple(bool(v) for v in (periodic if np.iterable(periodic) else [periodic] * dim))
+    assert len(n) == dim and len(lower) == dim and len(upper) == dim and len(per) == dim
+    ne = int(np.prod(n))
+    h = (np.asarray(upper) - np.asarray(lower)) / np.asarray(n)
+    J = float(np.prod(h))
+    jac = np.full(ne, J)
+    adj = np.zeros((ne, dim, dim))
+    for d in range(dim):
+        adj[:, d, d] = J / h[d]
+    idx = np.indices(n[::-1]).reshape(dim, -1)[::-1]  # (dim, ne), x fastest
+    strides = np.cumprod((1,) + n[:-1])
+    nbr = np.empty((ne, 2 * dim), dtype=np.int32)
+    any_bnd = False
+    for d in range(dim):
+        for s, step in ((0, -1), (1, +1)):
+            j = idx[d] + step
+            if per[d]:
+                jj = j % n[d]
+                nbr[:, 2 * d + s] = np.arange(ne) + (jj - idx[d]) * strides[d]
+            else:
+                inside = (j >= 0) & (j < n[d])
+                val = np.where(inside, np.arange(ne) + step * strides[d], -1)
+                any_bnd |= bool((~inside).any())
+                nbr[:, 2 * d + s] = val
+    bc_type = np.array([bc] if any_bnd else [], dtype=np.int32)
+    return BoxMesh(dim, n, lower, upper, per, jac, adj.reshape(ne, dim * dim).copy(), nbr, bc_type)
+
+
+def gll_nodes(p: int):
+    """GLL nodes and weights on [0,1] (float64), as the engine builds them."""
+    N = p
+    x = np.zeros(p + 1)
+    w = np.zeros(p + 1)
+    from numpy.polynomial import legendre as L
+
+    c = np.zeros(N + 1)
+    c[N] = 1
+    dc = L.legder(c)
+    inner = np.sort(L.legroots(dc)) if N > 1 else np.array([])
+    z = np.concatenate([[-1.0], inner, [1.0]])
+    PN = L.legval(z, c)
+    wz = 2.0 / (N * (N + 1) * PN**2)
+    x[:] = (1 + z) / 2
+    w[:] = wz / 2
+    x[0], x[-1] = 0.0, 1.0
+    return x, w

@@ -1,0 +1,183 @@
+"""Host-side mirror of the reference's RHS operator interface.
+
+The reference exposes ``RHSOperatorBase`` (an ``mfem::TimeDependentOperator``)
+with ``Mult(u, dudt)``, ``EstimateStability(u)`` and the concrete
+``EulerOperator`` / ``NSOperator`` built by ``MakeRHSOperator(runTime, ...)``
+(include/RHSOperator.hpp:19-188, include/SimFactory.hpp:22-31).  These
+classes keep those names and argument meanings; vectors are FP64 device
+tensors in the reference's byNODES L-vector layout, and every call goes
+through the C-ABI of libtheseus_rhs.so.  There is no CPU fallback: the
+constructor raises when the library is missing or the tensors are not on a
+GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi as A
+from .config import RunTime
+from .mesh import BoxMesh
+
+
+@dataclass
+class StabilityEstimate:
+    """StabilityEstimate.hpp:40-55"""
+
+    advective_rate: float = 0.0
+    surface_rate: float = 0.0
+    diffusive_rate: float = 0.0
+
+    def TotalRate(self) -> float:
+        return max(self.advective_rate, self.surface_rate) + self.diffusive_rate
+
+    def CFL(self, step: float) -> float:
+        return step * self.TotalRate()
+
+
+def _stream_ptr(stream):
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def build_descriptor(rt: RunTime, mesh: BoxMesh):
+    """Fill a theseus_case_desc; returns (desc, keepalive arrays)."""
+    if rt.dim != mesh.dim:
+        raise ValueError(f"runTime dim {rt.dim} does not match mesh dim {mesh.dim}")
+    jac = np.ascontiguousarray(mesh.elem_jac, dtype=np.float64)
+    adj = np.ascontiguousarray(mesh.elem_adj, dtype=np.float64)
+    nbr = np.ascontiguousarray(mesh.face_neighbor, dtype=np.int32)
+    bct = np.ascontiguousarray(mesh.bc_type, dtype=np.int32)
+    d = A.CaseDesc()
+    d.dim = rt.dim
+    d.order = rt.order
+    d.num_elements = mesh.num_elements
+    d.flow_model = rt.flow_model
+    d.volume_flux = rt.volume_flux
+    d.surface_flux = rt.surface_flux
+    d.precision = rt.precision
+    d.blending = 1 if rt.blending else 0
+    d.gamma = rt.gamma
+    d.Pr = rt.Pr
+    d.R_gas = rt.R_gas
+    d.mu = rt.mu
+    d.mu_bulk = rt.mu_bulk
+    d.alpha_max = rt.alpha_max
+    d.alpha_min = rt.alpha_min
+    d.elem_jac = jac.ctypes.data_as(C.POINTER(C.c_double))
+    d.elem_adj = adj.ctypes.data_as(C.POINTER(C.c_double))
+    d.face_neighbor = nbr.ctypes.data_as(C.POINTER(C.c_int))
+    d.num_bc = int(bct.size)
+    d.bc_type = bct.ctypes.data_as(C.POINTER(C.c_int)) if bct.size else C.POINTER(C.c_int)()
+    return d, (jac, adj, nbr, bct)
+
+
+class RHSOperator:
+    """RHSOperator<PhysicsT> (include/RHSOperator.hpp:119-188) over the GPU engine."""
+
+    def __init__(self, runtime: RunTime, mesh: BoxMesh):
+        self._lib = A.load()
+        self.runtime = runtime
+        self.mesh = mesh
+        desc, keep = build_descriptor(runtime, mesh)
+        h = C.c_void_p()
+        A.check(self._lib, self._lib.theseus_rhs_create(C.byref(desc), C.byref(h)))
+        self._h = h
+        n = C.c_longlong()
+        q = C.c_int()
+        A.check(self._lib, self._lib.theseus_rhs_sizes(h, C.byref(n), C.byref(q)))
+        self.ndof_total = int(n.value)
+        self.num_equations = int(q.value)
+        self.num_elements = mesh.num_elements
+        self.dim = runtime.dim
+        self.order = runtime.order
+
+    # -- reference interface -------------------------------------------------
+    @property
+    def Height(self) -> int:
+        return self.ndof_total * self.num_equations
+
+    def _check(self, t, name):
+        import torch
+
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError(f"{name} must be a CUDA/HIP torch tensor (the engine has no CPU path)")
+        if t.dtype != torch.float64:
+            raise TypeError(f"{name} must be float64")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+        if t.numel() != self.Height:
+            raise ValueError(f"{name} has {t.numel()} entries, expected {self.Height}")
+
+    def Mult(self, u, dudt, stream=None) -> None:
+        """dudt = L(u) (RHSOperator_impl.hpp:719-793)."""
+        self._check(u, "u")
+        self._check(dudt, "dudt")
+        A.check(self._lib, self._lib.theseus_rhs_eval(self._h, C.c_void_p(u.data_ptr()),
+                                                      C.c_void_p(dudt.data_ptr()), _stream_ptr(stream)))
+
+    def EstimateStability(self, u, stream=None) -> StabilityEstimate:
+        """RHSOperator_impl.hpp:10-270 with the max-reduction on the device."""
+        self._check(u, "u")
+        s = A.Stability()
+        A.check(self._lib, self._lib.theseus_rhs_estimate_stability(self._h, C.c_void_p(u.data_ptr()),
+                                                                    C.byref(s), _stream_ptr(stream)))
+        return StabilityEstimate(s.advective_rate, s.surface_rate, s.diffusive_rate)
+
+    def Step(self, u, t: float, dt: float, stream=None) -> float:
+        """One SSP-RK3 step in place (mfem::RK3SSPSolver::Step); returns t + dt."""
+        self._check(u, "u")
+        A.check(self._lib, self._lib.theseus_rhs_step_ssprk3(self._h, C.c_void_p(u.data_ptr()), float(t),
+                                                             float(dt), _stream_ptr(stream)))
+        return t + dt
+
+    def Alpha(self, stream=None):
+        """Per-element blending coefficient of the last evaluation (device tensor)."""
+        import torch
+
+        out = torch.empty(self.num_elements, dtype=torch.float64, device="cuda")
+        A.check(self._lib, self._lib.theseus_rhs_get_alpha(self._h, C.c_void_p(out.data_ptr()),
+                                                           _stream_ptr(stream)))
+        return out
+
+    # -- instrumentation -------------------------------------------------------
+    def enable_kernel_timing(self, capacity: int) -> None:
+        A.check(self._lib, self._lib.theseus_rhs_enable_kernel_timing(self._h, int(capacity)))
+
+    def kernel_timing(self, kind: int):
+        """(mean ms, count) of kernel kind 0 (volume+face / NS pass 2) or 1 (NS pass 1)."""
+        ms = C.c_double()
+        n = C.c_int()
+        A.check(self._lib, self._lib.theseus_rhs_kernel_timing(self._h, int(kind), C.byref(ms), C.byref(n)))
+        return float(ms.value), int(n.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.theseus_rhs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class EulerOperator(RHSOperator):
+    """EulerOperator<PhysicsT> (include/EulerOperator.hpp:12-49)."""
+
+
+class NSOperator(RHSOperator):
+    """NSOperator<PhysicsT> (include/NSOperator.hpp:12-71), BR1 viscous terms."""
+
+
+def MakeRHSOperator(runtime: dict | RunTime, mesh: BoxMesh) -> RHSOperator:
+    """SimFactory_impl.hpp:63-488: pick the operator from the runTime strings."""
+    rt = runtime if isinstance(runtime, RunTime) else RunTime.from_json(runtime)
+    cls = NSOperator if rt.flow_model == A.FLOW_NAVIER_STOKES else EulerOperator
+    return cls(rt, mesh)
