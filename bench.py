@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark: DOF*RHS-evals/s of the DGSEM engine on the 3-D compressible
+Navier-Stokes Taylor-Green vortex, 32^3 hexes, p=3 (10,485,760 DOF), BR1
+viscous terms, Ranocha volume + Lax-Friedrichs face flux, FP64.
+
+A "step" is one SSP-RK3 time step (3 RHS evaluations with the stage
+updates fused into the RHS kernels) over the whole mesh, inputs resident in
+HBM.  ``value`` = DOF * 3 * steps / elapsed.  One JSON line on stdout.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(mesh_ne, order, dim, kind, stage_modes):
+    """Compulsory bytes per launch of one kernel kind, averaged over the
+    launches of a step.  Per node: conservative state (dim+2 doubles), the
+    stored viscous flux (dim*(dim+1)/2 + dim doubles), the output state and,
+    for SSP-RK3 stages 2 and 3, the step's base state."""
+    ndof = (order + 1) ** dim
+    nodes = mesh_ne * ndof
+    neq = dim + 2
+    nv = dim * (dim + 1) // 2 + dim
+    if kind == 1:  # NS pass 1: read U, write Fv
+        per_node = 8 * (neq + nv)
+        return nodes * per_node, per_node / neq
+    per = []
+    for mode in stage_modes:  # pass 2: read U, Fv, write out, (+ read X)
+        per.append(8 * (neq + nv + neq + (neq if mode == 2 else 0)))
+    per_node = sum(per) / len(per)
+    return nodes * per_node, per_node / neq
+
+
+def cpu_baseline(rt, mesh, U0, seconds):
+    """The CPU FP64 oracle (a restatement of the reference's RHS path) on
+    the host cores, SSP-RK3 steps on the same grid, bounded to ~`seconds`."""
+    from oracle import pyoracle
+    from theseus_amd.operator import build_descriptor
+
+    if not pyoracle.LIB.exists():
+        pyoracle.build()
+    lib = pyoracle.load()
+    desc, keep = build_descriptor(rt, mesh)
+    case = pyoracle.OracleCase(desc)
+    rates = case.stability(U0)
+    dt = 0.5 / (max(rates[0], rates[1]) + rates[2])
+    U = U0.copy()
+    t0 = time.perf_counter()
+    U = case.step(U, dt)
+    one = time.perf_counter() - t0
+    nsteps = max(1, int(seconds / max(one, 1e-9)) - 1)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        U = case.step(U, dt)
+    el = time.perf_counter() - t0
+    dof = U0.size
+    return {
+        "value": dof * 3 * nsteps / el,
+        "unit": "DOF*RHS-evals/s",
+        "cores": int(lib.oracle_num_threads()),
+        "kind": "port",
+        "sample": f"full {mesh.n[0]}^3 p={rt.order} grid, {nsteps} SSP-RK3 steps ({3 * nsteps} RHS evals) "
+                  f"after 1 untimed step, {el:.1f} s, OpenMP over elements",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=32, help="elements per direction")
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design (see DESIGN.md), "
+                                   "not a measured path", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from theseus_amd.cases import c2_tgv3d
+    from theseus_amd.operator import MakeRHSOperator
+
+    rt, mesh, U0 = c2_tgv3d(n=args.n, order=args.order)
+    op = MakeRHSOperator(rt, mesh)
+    dof = U0.size
+    u = torch.from_numpy(U0).cuda()
+    stab = op.EstimateStability(u)
+    dt = args.cfl / stab.TotalRate()
+    log(f"workload: TGV {args.n}^3 p={args.order} NS, DOF={dof}, dt={dt:.4e} (CFL {args.cfl})")
+
+    t = 0.0
+    for _ in range(args.warmup):
+        t = op.Step(u, t, dt)
+    torch.cuda.synchronize()
+    op.enable_kernel_timing(3 * args.steps)
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        t = op.Step(u, t, dt)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    if not torch.isfinite(u).all():
+        raise RuntimeError("non-finite state after the timed steps")
+    value = dof * 3 * args.steps / (ms * 1e-3)
+
+    k2_ms, k2_n = op.kernel_timing(0)
+    k1_ms, k1_n = op.kernel_timing(1)
+    modes = [1, 2, 2]
+    b2, b2_dof = algorithmic_bytes(mesh.num_elements, args.order, 3, 0, modes)
+    b1, b1_dof = algorithmic_bytes(mesh.num_elements, args.order, 3, 1, modes)
+    kernels = {
+        "ns_pass2_volume_face": {"mean_ms": k2_ms, "launches": k2_n, "alg_bytes": b2, "alg_B_per_dof": b2_dof},
+        "ns_pass1_br1_gradient": {"mean_ms": k1_ms, "launches": k1_n, "alg_bytes": b1, "alg_B_per_dof": b1_dof},
+    }
+    dom_name = max(kernels, key=lambda k: kernels[k]["mean_ms"])
+    dom = kernels[dom_name]
+    achieved = dom["alg_bytes"] / (dom["mean_ms"] * 1e-3) / 1e9
+    for k, v in kernels.items():
+        gbps = v["alg_bytes"] / (v["mean_ms"] * 1e-3) / 1e9 if v["mean_ms"] > 0 else 0.0
+        log(f"kernel {k}: {v['mean_ms']*1e3:.1f} us x {v['launches']}, {gbps:.0f} GB/s algorithmic "
+            f"({v['alg_B_per_dof']:.2f} B/DOF)")
+    log(f"step: {ms/args.steps:.3f} ms, {value:.4e} DOF*RHS/s")
+
+    traffic = None
+    tr_file = ROOT / "profiles" / "traffic_tgv32_p3_fp64.txt"
+    if tr_file.exists():
+        for line in tr_file.read_text().splitlines():
+            if line.startswith(f"{dom_name}_hbm_bytes_per_launch:"):
+                traffic = float(line.split(":", 1)[1])
+
+    out = {
+        "metric": "DOF*RHS-evals/s (3-D TGV p=3, FP64)",
+        "value": value,
+        "unit": "DOF*RHS-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green initial condition)",
+        "config": {
+            "workload": f"3-D compressible Navier-Stokes Taylor-Green vortex, {args.n}^3 hexes, p={args.order}, "
+                        "BR1, Ranocha volume + Lax-Friedrichs face, SSP-RK3, Ma 0.1, Re 1600, Pr 0.71",
+            "elements": mesh.num_elements,
+            "order": args.order,
+            "dof": dof,
+            "rhs_evals_per_step": 3,
+            "dt": dt,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS,
+            "traffic": traffic,
+        },
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(rt, mesh, U0, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
