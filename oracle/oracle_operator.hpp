@@ -167,6 +167,45 @@ struct Case {
   // Persson-Peraire cache (dgsem_cache_utilities.hpp:867-943)
   std::vector<double> modal, keep_M1, keep_M2;
   double adv_scale, diff_scale, surf_scale;
+  // Interior faces of direction d (by low-side element) split into colours
+  // so that no two faces of one colour share an element: faces of one colour
+  // can be processed in parallel.  The colour is the parity of the low-side
+  // element's position along its chain of d-neighbours; a closed chain of odd
+  // length puts its last face in a third colour.
+  std::vector<std::vector<int>> face_cols[3];
+
+  void build_face_colours() {
+    for (int dd = 0; dd < 3; ++dd) face_cols[dd].assign(3, {});
+    for (int dd = 0; dd < dim; ++dd) {
+      std::vector<int> pos(ne, -1);
+      for (int e = 0; e < ne; ++e) {
+        if (pos[e] >= 0) continue;
+        int start = e, guard = 0;
+        while (true) {  // walk back to the start of the chain (or once around a ring)
+          const int prev = nbr[(size_t)start * 2 * dim + 2 * dd];
+          if (prev < 0 || prev == e || guard++ > ne) break;
+          start = prev;
+        }
+        std::vector<int> chain;
+        int cur = start;
+        while (cur >= 0 && pos[cur] < 0) {
+          pos[cur] = (int)chain.size();
+          chain.push_back(cur);
+          cur = nbr[(size_t)cur * 2 * dim + 2 * dd + 1];
+        }
+        const bool ring = cur == start;
+        const int len = (int)chain.size();
+        for (int k = 0; k < len; ++k) {
+          const int el = chain[k];
+          if (nbr[(size_t)el * 2 * dim + 2 * dd + 1] < 0) continue;
+          int col = k & 1;
+          if (ring && (len & 1) && k == len - 1) col = 2;
+          if (ring && len == 1) col = 0;
+          face_cols[dd][col].push_back(el);
+        }
+      }
+    }
+  }
 
   explicit Case(const theseus_case_desc &d) : basis(d.order) {
     dim = d.dim;
@@ -195,6 +234,7 @@ struct Case {
     diff_scale = ReferenceBR1DiffusionSpectralScale(p);
     surf_scale = adv_scale / (1.0 / basis.w[0]);
     build_persson();
+    build_face_colours();
   }
 
   // Modal transform: V(i,k) = prod_j L_{deg(k,j)}(x_j(i)) (ModalBasis.cpp:47-140),
@@ -422,20 +462,15 @@ inline void BlendingCoefficient(const Case &c, const double *U, double *alpha_ou
 // ---------------------------------------------------------------------------
 // Face traversal helpers
 // ---------------------------------------------------------------------------
-// Interior faces, visited direction by direction; within a direction the
-// faces are split into colours so that no two faces of one colour share an
-// element, and each colour runs in parallel (OpenMP).  Every dof therefore
+// Interior faces, visited direction by direction and colour by colour (see
+// Case::face_cols); each colour runs in parallel.  Every dof therefore
 // receives its face contributions in a fixed order (x faces, then y, then z)
-// independent of the thread count.  The colour of a face is the parity of the
-// low-side element's position along the chain of d-neighbours; a closed
-// chain of odd length puts its last face in a third colour.
-inline const std::vector<std::vector<int>> &FaceColours(const Case &c, int d);
+// whatever the thread count.
 template <typename F>
 inline void ForEachInteriorFace(const Case &c, F &&f) {
   for (int d = 0; d < c.dim; ++d) {
-    const auto &cols = FaceColours(c, d);
     for (int col = 0; col < 3; ++col) {
-      const std::vector<int> &list = cols[col];
+      const std::vector<int> &list = c.face_cols[d][col];
       const long n = (long)list.size();
 #pragma omp parallel for schedule(static)
       for (long k = 0; k < n; ++k) {
@@ -444,45 +479,6 @@ inline void ForEachInteriorFace(const Case &c, F &&f) {
       }
     }
   }
-}
-inline const std::vector<std::vector<int>> &FaceColours(const Case &c, int d) {
-  static thread_local const Case *cached_case = nullptr;
-  static thread_local std::vector<std::vector<int>> cols[3];
-  if (cached_case != &c) {
-    for (int dd = 0; dd < 3; ++dd) cols[dd].assign(3, {});
-    for (int dd = 0; dd < c.dim; ++dd) {
-      std::vector<int> pos(c.ne, -1);
-      for (int e = 0; e < c.ne; ++e) {
-        if (pos[e] >= 0) continue;
-        // walk back to the start of the chain (or around a closed ring)
-        int start = e, guard = 0;
-        while (true) {
-          const int prev = c.nbr[(size_t)start * 2 * c.dim + 2 * dd];
-          if (prev < 0 || prev == e || guard++ > c.ne) break;
-          start = prev;
-        }
-        std::vector<int> chain;
-        int cur = start;
-        while (cur >= 0 && pos[cur] < 0) {
-          pos[cur] = (int)chain.size();
-          chain.push_back(cur);
-          cur = c.nbr[(size_t)cur * 2 * c.dim + 2 * dd + 1];
-        }
-        const bool ring = cur == start;
-        const int len = (int)chain.size();
-        for (int k = 0; k < len; ++k) {
-          const int el = chain[k];
-          if (c.nbr[(size_t)el * 2 * c.dim + 2 * dd + 1] < 0) continue;
-          int col = k & 1;
-          if (ring && (len & 1) && k == len - 1) col = 2;
-          if (ring && len == 1) col = 0;
-          cols[dd][col].push_back(el);
-        }
-      }
-    }
-    cached_case = &c;
-  }
-  return cols[d];
 }
 template <typename F>
 inline void ForEachInteriorFaceSerial(const Case &c, F &&f) {

@@ -1,0 +1,139 @@
+"""CPU-side checks of the drop-in boundary: the library loads, exports every
+entry point include/theseus_rhs.h declares, and rejects invalid case
+descriptors before touching a GPU; the host mirror parses the reference's
+runTime strings like the reference factory."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from theseus_amd import _abi as A
+from theseus_amd.config import RunTime, is_viscous, numerical_flux_name
+from theseus_amd.mesh import box_mesh
+from theseus_amd.operator import build_descriptor
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "theseus_rhs.h").read_text()
+    return sorted(set(re.findall(r"THESEUS_API\s+[\w\s\*]+?\b(theseus_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = A.load()
+    decl = declared_symbols()
+    assert decl and set(decl) == set(A.EXPORTED_SYMBOLS)
+    out = subprocess.run(["nm", "-D", "--defined-only", str(A.LIB_PATH)], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (theseus_\w+)", out))
+    assert exported == set(decl), (exported ^ set(decl))
+    for name in decl:
+        getattr(lib, name)
+
+
+def _desc(**over):
+    rt = RunTime(dim=2, order=3, flow_model=A.FLOW_EULER, volume_flux=A.VOLUME_RANOCHA, surface_flux=A.SURFACE_LLF)
+    mesh = box_mesh(2, (3, 2), 0.0, 1.0)
+    d, keep = build_descriptor(rt, mesh)
+    for k, v in over.items():
+        setattr(d, k, v)
+    return d, keep
+
+
+@pytest.mark.parametrize("field,value,code", [
+    ("dim", 4, A.ERR_INVALID_ARGUMENT),
+    ("order", 0, A.ERR_INVALID_ARGUMENT),
+    ("num_elements", 0, A.ERR_INVALID_ARGUMENT),
+    ("flow_model", 7, A.ERR_INVALID_ARGUMENT),
+    ("gamma", 1.0, A.ERR_INVALID_ARGUMENT),
+    ("precision", A.PRECISION_MIXED, A.ERR_UNSUPPORTED),
+    ("volume_flux", 9, A.ERR_UNSUPPORTED),
+])
+def test_create_rejects_invalid_descriptors(field, value, code):
+    lib = A.load()
+    d, keep = _desc(**{field: value})
+    h = C.c_void_p()
+    rc = lib.theseus_rhs_create(C.byref(d), C.byref(h))
+    assert rc == code and not h.value
+    assert lib.theseus_last_error().decode()
+
+
+def test_create_rejects_inconsistent_connectivity():
+    lib = A.load()
+    rt = RunTime(dim=2, order=3, flow_model=A.FLOW_EULER, volume_flux=A.VOLUME_RANOCHA, surface_flux=A.SURFACE_LLF)
+    mesh = box_mesh(2, (3, 2), 0.0, 1.0)
+    mesh.face_neighbor[0, 1] = 2  # element 0's +x neighbour should be 1
+    d, keep = build_descriptor(rt, mesh)
+    h = C.c_void_p()
+    assert lib.theseus_rhs_create(C.byref(d), C.byref(h)) == A.ERR_INVALID_ARGUMENT
+    assert "symmetric" in lib.theseus_last_error().decode()
+
+
+def test_navier_stokes_boundaries_are_rejected():
+    lib = A.load()
+    rt = RunTime(dim=2, order=3, flow_model=A.FLOW_NAVIER_STOKES, volume_flux=A.VOLUME_RANOCHA,
+                 surface_flux=A.SURFACE_LLF)
+    mesh = box_mesh(2, (3, 2), 0.0, 1.0, periodic=False)
+    d, keep = build_descriptor(rt, mesh)
+    h = C.c_void_p()
+    assert lib.theseus_rhs_create(C.byref(d), C.byref(h)) == A.ERR_UNSUPPORTED
+
+
+def test_null_arguments():
+    lib = A.load()
+    assert lib.theseus_rhs_create(None, None) == A.ERR_INVALID_ARGUMENT
+    assert lib.theseus_rhs_eval(None, None, None, None) == A.ERR_INVALID_ARGUMENT
+    assert lib.theseus_rhs_destroy(None) == A.OK
+
+
+@pytest.mark.parametrize("s,fam", [("", "chandrashekar"), ("Chandrashekar", "chandrashekar"), ("chan", "chandrashekar"),
+                                   ("RANOCHA", "ranocha"), ("piflux", "pirozzoli"), ("pi", "pirozzoli"),
+                                   ("hll", "hll"), ("LaxFriedrichs", "llf"), ("lfr", "llf"), ("lax_x", "llf"),
+                                   ("Roe", "roe")])
+def test_numerical_flux_aliases(s, fam):
+    """SimFactory_impl.hpp:78-118"""
+    assert numerical_flux_name(s) == fam
+
+
+def test_numerical_flux_errors():
+    with pytest.raises(ValueError):
+        numerical_flux_name("upwind")
+    with pytest.raises(NotImplementedError):
+        numerical_flux_name("lteec")
+
+
+def test_flow_model_strings():
+    for s in ("viscous", "Navier-Stokes", "cns", "ns"):
+        assert is_viscous(s)
+    for s in ("euler", "", None, "inviscid"):
+        assert not is_viscous(s)
+
+
+def test_runtime_json_defaults_and_overrides():
+    rt = RunTime.from_json({"order": 3, "dim": 3, "numerical_flux": "Ranocha", "flow_model": "navier-stokes"})
+    assert (rt.volume_flux, rt.surface_flux) == (A.VOLUME_RANOCHA, A.SURFACE_RANOCHA)
+    assert rt.flow_model == A.FLOW_NAVIER_STOKES
+    assert (rt.gamma, rt.Pr, rt.R_gas, rt.mu, rt.alpha_max) == (1.4, 0.72, 287.05, 0.02, 0.5)
+    rt = RunTime.from_json({"order": 3, "dim": 3, "numerical_flux": "ranocha", "surface_flux": "llf", "mu": 1e-3,
+                            "Pr": 0.71, "cfl": 0.4, "variable_dt": False, "dt": 1e-4})
+    assert rt.surface_flux == A.SURFACE_LLF and rt.mu == 1e-3 and rt.dt == 1e-4 and not rt.variable_dt
+    with pytest.raises(NotImplementedError):
+        RunTime.from_json({"order": 3, "dim": 2, "gas_model": "lte"})
+
+
+def test_box_mesh_connectivity_and_geometry():
+    m = box_mesh(3, (4, 3, 2), (0, 0, 0), (2.0, 1.5, 1.0), periodic=(True, False, True))
+    ne = m.num_elements
+    nb = m.face_neighbor
+    for e in range(ne):
+        for f in range(6):
+            o = nb[e, f]
+            if o >= 0:
+                assert nb[o, f ^ 1] == e
+    assert (nb[:, 2:4] < 0).any() and (nb[:, [0, 1, 4, 5]] >= 0).all()
+    h = np.array([0.5, 0.5, 0.5])
+    np.testing.assert_allclose(m.elem_jac, np.prod(h))
+    np.testing.assert_allclose(m.elem_adj.reshape(ne, 3, 3)[0], np.diag(np.prod(h) / h))

@@ -123,3 +123,31 @@ def test_ssprk3_100_steps_parity(oracle, flow, pairing, dim, order, n):
     got = u.cpu().numpy()
     err = rel_l2(got, ref)
     assert err <= 1e-10, (err, per_field_rel(got, ref, dim + 2))
+
+
+def test_engine_reproduces_reference_golden_vortex():
+    """The HIP engine on the reference's golden isentropic-vortex case
+    (500 SSP-RK3 steps, Chandrashekar, blending on) against the reference's
+    own output fields (tests/golden/isentropic_vortex_cycle500.npz)."""
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+
+    from .golden_vortex import DT, NSTEPS, golden_permutation, primitive_fields, vortex_case
+
+    _say("case golden vortex 500 steps")
+    rt, mesh, U0 = vortex_case()
+    d, perm = golden_permutation(mesh)
+    op = MakeRHSOperator(rt, mesh)
+    u = torch.from_numpy(U0).cuda()
+    t = 0.0
+    for _ in range(NSTEPS):
+        t = op.Step(u, t, DT)
+    torch.cuda.synchronize()
+    f = primitive_fields(u.cpu().numpy())
+    for k in ("rho", "u", "v", "p"):
+        g = d[k][perm]
+        rel = np.linalg.norm(f[k] - g) / np.linalg.norm(g)
+        assert rel < 1e-12, (k, rel)
+    alpha = op.Alpha().cpu().numpy()
+    np.testing.assert_array_equal(alpha, d["alpha"][perm].reshape(-1, 16)[:, 0])

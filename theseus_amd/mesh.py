@@ -59,6 +59,63 @@ class BoxMesh:
         return coords.reshape(dim, ne * ndof)
 
 
+@dataclass
+class TensorMesh(BoxMesh):
+    """Cartesian tensor-product mesh with per-direction break points
+    (element sizes may vary per direction and index)."""
+
+    breaks: tuple = ()
+
+    @property
+    def h(self):
+        raise AttributeError("TensorMesh has per-element sizes; use breaks")
+
+    def node_coordinates(self, x01: np.ndarray) -> np.ndarray:
+        Np = len(x01)
+        dim = self.dim
+        ne = self.num_elements
+        ndof = Np**dim
+        coords = np.empty((dim, ne, ndof))
+        eidx = np.indices(self.n[::-1]).reshape(dim, -1)[::-1]
+        nidx = np.indices((Np,) * dim).reshape(dim, -1)[::-1]
+        for d in range(dim):
+            b = np.asarray(self.breaks[d], float)
+            x0 = b[eidx[d]]
+            hh = b[eidx[d] + 1] - b[eidx[d]]
+            coords[d] = x0[:, None] + hh[:, None] * x01[nidx[d]][None, :]
+        return coords.reshape(dim, ne * ndof)
+
+
+def tensor_mesh(breaks, periodic=True, bc: int = 2) -> TensorMesh:
+    """Affine tensor-product mesh; breaks[d] are the element boundaries along d."""
+    dim = len(breaks)
+    breaks = tuple(np.asarray(b, dtype=np.float64) for b in breaks)
+    n = tuple(len(b) - 1 for b in breaks)
+    base = box_mesh(dim, n, [b[0] for b in breaks], [b[-1] for b in breaks], periodic=periodic, bc=bc)
+    ne = base.num_elements
+    eidx = np.indices(n[::-1]).reshape(dim, -1)[::-1]
+    h = np.stack([np.diff(breaks[d])[eidx[d]] for d in range(dim)])  # (dim, ne)
+    jac = np.prod(h, axis=0)
+    adj = np.zeros((ne, dim, dim))
+    for d in range(dim):
+        adj[:, d, d] = jac / h[d]
+    return TensorMesh(dim, n, base.lower, base.upper, base.periodic, jac, adj.reshape(ne, dim * dim).copy(),
+                      base.face_neighbor, base.bc_type, breaks=breaks)
+
+
+def refine_breaks(base, levels: int) -> np.ndarray:
+    """Uniform refinement of each base interval by midpoint bisection,
+    `levels` times (what uniform mesh refinement does to an affine segment)."""
+    b = np.asarray(base, dtype=np.float64)
+    for _ in range(levels):
+        mid = 0.5 * (b[:-1] + b[1:])
+        out = np.empty(2 * len(b) - 1)
+        out[0::2] = b
+        out[1::2] = mid
+        b = out
+    return b
+
+
 def box_mesh(dim: int, n, lower, upper, periodic=True, bc: int = 2) -> BoxMesh:
     """Cartesian box of n[0] x n[1] (x n[2]) affine elements.
 
