@@ -90,7 +90,20 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 10 times (scales the HBM byte counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": src.numel() * 8, "copy_bytes_written_per_launch": src.numel() * 8}))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

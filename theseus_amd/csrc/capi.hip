@@ -4,7 +4,7 @@
 // reference assembles in RHSOperator::Finalize (RHSOperator_impl.hpp:273-314,
 // dgsem_cache_utilities.hpp:18-215,867-943): GLL basis, derivative
 // operators, per-element affine geometry, face connectivity and the
-// Persson-Peraire modal transform.  Evaluation enqueues one kernel for Euler
+// Persson-Peraire modal transform (1-D inverse Legendre Vandermonde).  Evaluation enqueues one kernel for Euler
 // and two for Navier-Stokes; the SSP-RK3 step fuses each stage's vector
 // update into the RHS kernel's epilogue.
 #include <hip/hip_runtime.h>
@@ -117,90 +117,52 @@ static void derivative_ops(int p, const std::vector<double> &x, const std::vecto
   for (int i = 0; i < Np; ++i) b.w[i] = w[i];
 }
 
-// Persson-Peraire modal transform (src/ModalBasis.cpp:47-140,
-// dgsem_cache_utilities.hpp:867-943): Legendre P_n(2x-1) Vandermonde on the
-// lexicographic nodes, inverted by Gauss-Jordan with partial pivoting.
-static void modal_ops(int dim, int p, const std::vector<double> &x, std::vector<double> &modal,
-                      std::vector<double> &k1, std::vector<double> &k2) {
-  const int Np = p + 1;
-  const int n = dim == 2 ? Np * Np : Np * Np * Np;
-  std::vector<int> deg(n * 3, 0);
-  {
-    int m = 0;
-    if (dim == 2) {
-      for (int i = 0; i < Np; i++)
-        for (int j = 0; j < Np; j++, m++) {
-          deg[m * 3] = i;
-          deg[m * 3 + 1] = j;
-        }
-    } else {
-      for (int i = 0; i < Np; i++)
-        for (int j = 0; j < Np; j++)
-          for (int k = 0; k < Np; k++, m++) {
-            deg[m * 3] = i;
-            deg[m * 3 + 1] = j;
-            deg[m * 3 + 2] = k;
-          }
+// Inverse of the 1-D Legendre Vandermonde V(i,a) = P_a(2 x_i - 1) on the
+// GLL nodes (Gauss-Jordan with partial pivoting in long double).  The
+// reference inverts the full tensor-product Vandermonde of the element
+// (src/ModalBasis.cpp:47-140, MFEM Poly_1D::CalcLegendre on [0,1]); that
+// matrix is the Kronecker product of this one.
+static void legendre_vandermonde_inverse(int p, const std::vector<double> &x, double *Vinv) {
+  const int n = p + 1;
+  std::vector<long double> V(n * n), I(n * n, 0.0L);
+  for (int i = 0; i < n; ++i) {
+    const long double z = 2.0L * x[i] - 1.0L;
+    long double P0 = 1.0L, P1 = z;
+    V[i * n + 0] = 1.0L;
+    if (n > 1) V[i * n + 1] = z;
+    for (int a = 1; a + 1 < n; ++a) {
+      const long double P2 = ((2 * a + 1) * z * P1 - a * P0) / (a + 1);
+      V[i * n + a + 1] = P2;
+      P0 = P1;
+      P1 = P2;
     }
+    I[i * n + i] = 1.0L;
   }
-  auto legendre01 = [&](double xx, std::vector<double> &u) {
-    u.assign(Np, 0.0);
-    u[0] = 1.0;
-    if (p == 0) return;
-    const double z = 2.0 * xx - 1.0;
-    u[1] = z;
-    for (int nn = 1; nn < p; nn++) u[nn + 1] = ((2 * nn + 1) * z * u[nn] - nn * u[nn - 1]) / (nn + 1);
-  };
-  std::vector<long double> V((size_t)n * n), I((size_t)n * n, 0.0L);
-  std::vector<double> L;
-  for (int node = 0; node < n; ++node) {
-    const int id[3] = {node % Np, (node / Np) % Np, node / (Np * Np)};
-    std::vector<double> row(n, 1.0);
-    for (int d = 0; d < dim; ++d) {
-      legendre01(x[id[d]], L);
-      for (int k = 0; k < n; ++k) row[k] = (d == 0) ? L[deg[k * 3 + d]] : row[k] * L[deg[k * 3 + d]];
-    }
-    for (int k = 0; k < n; ++k) V[(size_t)node * n + k] = row[k];
-  }
-  for (int i = 0; i < n; ++i) I[(size_t)i * n + i] = 1.0L;
   for (int c = 0; c < n; ++c) {
     int piv = c;
     for (int r = c + 1; r < n; ++r)
-      if (fabsl(V[(size_t)r * n + c]) > fabsl(V[(size_t)piv * n + c])) piv = r;
+      if (fabsl(V[r * n + c]) > fabsl(V[piv * n + c])) piv = r;
     if (piv != c)
       for (int k = 0; k < n; ++k) {
-        std::swap(V[(size_t)c * n + k], V[(size_t)piv * n + k]);
-        std::swap(I[(size_t)c * n + k], I[(size_t)piv * n + k]);
+        std::swap(V[c * n + k], V[piv * n + k]);
+        std::swap(I[c * n + k], I[piv * n + k]);
       }
-    const long double inv = 1.0L / V[(size_t)c * n + c];
+    const long double inv = 1.0L / V[c * n + c];
     for (int k = 0; k < n; ++k) {
-      V[(size_t)c * n + k] *= inv;
-      I[(size_t)c * n + k] *= inv;
+      V[c * n + k] *= inv;
+      I[c * n + k] *= inv;
     }
     for (int r = 0; r < n; ++r)
       if (r != c) {
-        const long double f = V[(size_t)r * n + c];
-        if (f == 0.0L) continue;
+        const long double f = V[r * n + c];
         for (int k = 0; k < n; ++k) {
-          V[(size_t)r * n + k] -= f * V[(size_t)c * n + k];
-          I[(size_t)r * n + k] -= f * I[(size_t)c * n + k];
+          V[r * n + k] -= f * V[c * n + k];
+          I[r * n + k] -= f * I[c * n + k];
         }
       }
   }
-  modal.assign((size_t)n * n, 0.0);
-  for (int q = 0; q < n; ++q)
-    for (int m = 0; m < n; ++m) modal[(size_t)q * n + m] = (double)I[(size_t)m * n + q];
-  k1.assign(n, 0.0);
-  k2.assign(n, 0.0);
-  for (int m = 0; m < n; ++m) {
-    bool keep1 = true, keep2 = true;
-    for (int d = 0; d < dim; ++d) {
-      if (deg[m * 3 + d] > p - 2) keep2 = false;
-      if (deg[m * 3 + d] > p - 1) keep1 = false;
-    }
-    k1[m] = keep1 ? 1.0 : 0.0;
-    k2[m] = keep2 ? 1.0 : 0.0;
-  }
+  // I = V^{-1}: mode a = sum_i I[a][i] u_i
+  for (int k = 0; k < n * n; ++k) Vinv[k] = (double)I[k];
 }
 
 // StabilityEstimate.hpp:61-91
@@ -254,7 +216,7 @@ struct theseus_rhs {
   BasisArgs basis;
   GasC<double> gas;
   double adv_scale, diff_scale, surf_scale;
-  double *d_J = nullptr, *d_adj = nullptr, *d_modal = nullptr, *d_keep1 = nullptr, *d_keep2 = nullptr;
+  double *d_J = nullptr, *d_adj = nullptr;
   int *d_nbr = nullptr;
   double *d_alpha = nullptr;
   double *d_Fv = nullptr;       // NS viscous flux components
@@ -282,9 +244,6 @@ struct theseus_rhs {
     a.alpha_min = d.alpha_min;
     a.sharp = 9.21024;                                                 // RHSOperator.hpp:33
     a.thresh = 0.5 * std::pow(10.0, -1.8 * std::pow(d.order, 0.25));  // RHSOperator.hpp:53
-    a.modal = d_modal;
-    a.keep1 = d_keep1;
-    a.keep2 = d_keep2;
     a.J = d_J;
     a.adj = d_adj;
     a.nbr = d_nbr;
@@ -397,8 +356,7 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
   std::vector<double> x, w;
   gll01(d.order, x, w);
   derivative_ops(d.order, x, w, h->basis);
-  std::vector<double> modal, k1, k2;
-  modal_ops(dim, d.order, x, modal, k1, k2);
+  legendre_vandermonde_inverse(d.order, x, h->basis.Vinv);
   // PhysicsConstants (Physics.hpp:33-38)
   h->gas.gamma = d.gamma;
   h->gas.gammaM1 = d.gamma - 1.0;
@@ -416,8 +374,7 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
   const size_t M = (size_t)h->Ntot * h->neq;
   if ((rc = upload(&h->d_J, d.elem_jac, ne)) || (rc = upload(&h->d_adj, d.elem_adj, (size_t)ne * dim * dim)) ||
       (rc = upload(&h->d_nbr, d.face_neighbor, (size_t)ne * 2 * dim)) ||
-      (rc = upload(&h->d_modal, modal.data(), modal.size())) || (rc = upload(&h->d_keep1, k1.data(), k1.size())) ||
-      (rc = upload(&h->d_keep2, k2.data(), k2.size())) || (rc = upload<double>(&h->d_alpha, nullptr, ne)) ||
+      (rc = upload<double>(&h->d_alpha, nullptr, ne)) ||
       (rc = upload<unsigned long long>(&h->d_rates, nullptr, 4)) ||
       (rc = upload<double>(&h->d_stage[0], nullptr, M)) || (rc = upload<double>(&h->d_stage[1], nullptr, M))) {
     theseus_rhs_destroy(h);
@@ -482,9 +439,6 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_J);
   hipFree(h->d_adj);
   hipFree(h->d_nbr);
-  hipFree(h->d_modal);
-  hipFree(h->d_keep1);
-  hipFree(h->d_keep2);
   hipFree(h->d_alpha);
   hipFree(h->d_Fv);
   hipFree(h->d_stage[0]);

@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "fastmath.hpp"
+
 namespace thx {
 
 template <typename R>
@@ -23,8 +25,9 @@ struct GasC {
 template <typename R, int DIM>
 struct Pt {
   R q[DIM + 2];  // rho, m_0..m_{DIM-1}, rhoE
-  R v[DIM];
-  R p;
+  R v[DIM];      // velocity m/rho
+  R p;           // pressure
+  R ir;          // 1/rho
 };
 
 template <typename R>
@@ -32,7 +35,7 @@ __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 template <>
 __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
 template <typename R>
-__device__ __forceinline__ R dlog(R x) { return log(x); }
+__device__ __forceinline__ R dlog(R x) { return ln_pos(x); }
 template <>
 __device__ __forceinline__ float dlog<float>(float x) { return logf(x); }
 template <typename R>
@@ -57,8 +60,43 @@ struct LogMeanEps<float> {
   static constexpr double value = 1e-2;
 };
 
-// Logarithmic mean (y-x)/ln(y/x) with the reference's series branch
-// (theseus_kernels.hpp:106-123).
+// Logarithmic mean LM(x,y) = (y-x)/ln(y/x) and its reciprocal, following
+// the reference's two branches (theseus_kernels.hpp:106-123): with
+// u = ((y-x)/(y+x))^2 (the reference's (xi^2-2xi+1)/(xi^2+2xi+1), xi=y/x),
+//   u <  eps : LM = (x+y) 52.5/(105 + 35u + 21u^2 + 15u^3)
+//              = ((x+y)/2) / P(u),  P(u) = 1 + u/3 + u^2/5 + u^3/7,
+//   u >= eps : LM = (y-x)/ln(y/x).
+// In the series branch one division (1/(x+y)) yields both LM and 1/LM:
+// 1/LM = (2/(x+y)) P(u) exactly as the reference's rational form, and
+// LM = ((x+y)/2) Q(u) with Q = 1 - u/3 - 4u^2/45 - 44u^3/945, the series of
+// 1/P, whose truncation (O(u^4) < 1e-16 for u < 1e-4) is below FP64
+// rounding.  The log branch is the reference's formula.
+template <typename R>
+struct LogMean {
+  R lm, inv;
+};
+
+template <typename R, bool NEED_LM, bool NEED_INV>
+__device__ __forceinline__ LogMean<R> log_mean2(R x, R y) {
+  LogMean<R> r;
+  const R s = x + y;
+  const R d = y - x;
+  if (d * d < R(LogMeanEps<R>::value) * (s * s)) {
+    const R rs = R(1) / s;
+    const R t = d * rs;
+    const R u = t * t;
+    if constexpr (NEED_LM) r.lm = (R(0.5) * s) * (R(1) - u * (R(1.0 / 3.0) + u * (R(4.0 / 45.0) + u * R(44.0 / 945.0))));
+    if constexpr (NEED_INV) r.inv = (R(2) * rs) * (R(1) + u * (R(1.0 / 3.0) + u * (R(0.2) + u * R(1.0 / 7.0))));
+  } else {
+    const R l = dlog(y / x);
+    if constexpr (NEED_LM) r.lm = d / l;
+    if constexpr (NEED_INV) r.inv = l / d;
+  }
+  return r;
+}
+
+// Logarithmic mean in the reference's exact formulation (kept for the
+// face fluxes, where it costs little).
 template <typename R>
 __device__ __forceinline__ R log_mean(R x, R y) {
   const R xi = y / x;
@@ -70,26 +108,27 @@ __device__ __forceinline__ R log_mean(R x, R y) {
   return (y - x) / dlog(xi);
 }
 
-// Derive velocity and pressure from the conservative vector (same operation
-// order as the reference's accessors).
+// Derive velocity and pressure from the conservative vector, as the
+// reference's accessors do (GasState.hpp:169-172, EOS.hpp:75-138), with the
+// divisions by rho replaced by one reciprocal.
 template <typename R, int DIM>
 __device__ __forceinline__ void derive(const GasC<R> &g, Pt<R, DIM> &s) {
-  const R rho = s.q[0];
+  s.ir = R(1) / s.q[0];
   R m2 = R(0);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     const R m = s.q[1 + d];
     m2 += m * m;
   }
-  const R ke = R(0.5) * m2 / rho;
+  const R ke = R(0.5) * m2 * s.ir;
   s.p = g.gammaM1 * (s.q[DIM + 1] - ke);
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) s.v[d] = s.q[1 + d] / rho;
+  for (int d = 0; d < DIM; ++d) s.v[d] = s.q[1 + d] * s.ir;
 }
 
 template <typename R, int DIM>
 __device__ __forceinline__ R sound_speed(const GasC<R> &g, const Pt<R, DIM> &s) {
-  return dsqrt(g.gamma * s.p / s.q[0]);
+  return dsqrt(g.gamma * s.p * s.ir);
 }
 
 // ---------------------------------------------------------------------------
@@ -101,7 +140,8 @@ template <typename R, int DIM>
 __device__ __forceinline__ void vol_chandrashekar(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b,
                                                   const R *met, R *F) {
   const R rho1 = a.q[0], rho2 = b.q[0];
-  const R rho_ln = log_mean(rho1, rho2);
+  const LogMean<R> lr = log_mean2<R, true, true>(rho1, rho2);
+  const R rho_ln = lr.lm;
   R mom_hat[DIM];
   R h_hat = R(0), vn = R(0);
 #pragma unroll
@@ -114,10 +154,10 @@ __device__ __forceinline__ void vol_chandrashekar(const GasC<R> &g, const Pt<R, 
   }
   const R beta1 = R(0.5) * rho1 / a.p;
   const R beta2 = R(0.5) * rho2 / b.p;
-  const R beta_ln = log_mean(beta1, beta2);
+  const R inv_beta_ln = log_mean2<R, false, true>(beta1, beta2).inv;
   const R p_hat = R(0.5) * (rho1 + rho2) / (beta1 + beta2);
-  const R gm1_av_inv = R(2) / (g.gamma + g.gamma - R(2));
-  h_hat += R(0.5) / beta_ln * gm1_av_inv + p_hat / rho_ln;
+  const R gm1_av_inv = g.gammaM1Inverse;  // 2/(gamma1 + gamma2 - 2) with one gamma
+  h_hat += R(0.5) * inv_beta_ln * gm1_av_inv + p_hat * lr.inv;
   F[0] = rho_ln * vn;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) F[1 + d] = vn * mom_hat[d] + p_hat * met[d];
@@ -129,7 +169,7 @@ template <typename R, int DIM>
 __device__ __forceinline__ void vol_ranocha(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b, const R *met,
                                             R *F) {
   const R rho1 = a.q[0], rho2 = b.q[0];
-  const R rho_ln = log_mean(rho1, rho2);
+  const R rho_ln = log_mean2<R, true, false>(rho1, rho2).lm;
   const R p1 = a.p, p2 = b.p;
   const R pbar = R(0.5) * (p1 + p2);
   R vbar[DIM], ppm[DIM];
@@ -142,8 +182,8 @@ __device__ __forceinline__ void vol_ranocha(const GasC<R> &g, const Pt<R, DIM> &
     ke_flux += R(0.5) * (v1 * v2);
     ppm[d] = R(0.5) * (p1 * v2 + p2 * v1);
   }
-  const R inv_beta_ln = (p1 * p2) / log_mean(rho1 * p2, rho2 * p1);
-  const R inv_gm1 = R(1) / (g.gamma - R(1));
+  const R inv_beta_ln = (p1 * p2) * log_mean2<R, false, true>(rho1 * p2, rho2 * p1).inv;
+  const R inv_gm1 = g.gammaM1Inverse;
   const R ie_flux = inv_beta_ln * inv_gm1;
   R fe = R(0);
 #pragma unroll
@@ -179,9 +219,9 @@ __device__ __forceinline__ void vol_pirozzoli(const GasC<R> &g, const Pt<R, DIM>
     m2a += a.q[1 + d] * a.q[1 + d];
     m2b += b.q[1 + d] * b.q[1 + d];
   }
-  const R e1 = (a.q[DIM + 1] - R(0.5) * m2a / rho1) / rho1;
-  const R e2 = (b.q[DIM + 1] - R(0.5) * m2b / rho2) / rho2;
-  const R h_mean = R(0.5) * (e1 + p1 / rho1 + e2 + p2 / rho2);
+  const R e1 = (a.q[DIM + 1] - R(0.5) * m2a * a.ir) * a.ir;
+  const R e2 = (b.q[DIM + 1] - R(0.5) * m2b * b.ir) * b.ir;
+  const R h_mean = R(0.5) * (e1 + p1 * a.ir + e2 + p2 * b.ir);
   F[0] = rho_mean * vn;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) F[1 + d] = rho_mean * vn * vbar[d] + pbar * met[d];
@@ -193,7 +233,7 @@ __device__ __forceinline__ void vol_pirozzoli(const GasC<R> &g, const Pt<R, DIM>
 template <typename R, int DIM>
 __device__ __forceinline__ void vol_central(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b, const R *met,
                                             R *F) {
-  const R sv1 = R(1) / a.q[0], sv2 = R(1) / b.q[0];
+  const R sv1 = a.ir, sv2 = b.ir;
   const R H1 = (a.q[DIM + 1] + a.p) * sv1, H2 = (b.q[DIM + 1] + b.p) * sv2;
   R f[DIM + 2];
 #pragma unroll
@@ -224,7 +264,7 @@ __device__ __forceinline__ void vol_central(const GasC<R> &g, const Pt<R, DIM> &
 // Physical normal flux of one state (NavierStokesFlux.hpp:274-332).
 template <typename R, int DIM>
 __device__ __forceinline__ void normal_flux(const Pt<R, DIM> &s, const R *n, R *fn) {
-  const R inv_rho = R(1) / s.q[0];
+  const R inv_rho = s.ir;
   const R H = (s.q[DIM + 1] + s.p) * inv_rho;
 #pragma unroll
   for (int e = 0; e < DIM + 2; ++e) fn[e] = R(0);
@@ -279,9 +319,9 @@ template <typename R, int DIM>
 __device__ __forceinline__ void roe_dissipation(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b,
                                                 const R *nor, R *diss) {
   const R rho1 = a.q[0], rho2 = b.q[0];
-  const R H1 = (a.q[DIM + 1] + a.p) / rho1;
-  const R H2 = (b.q[DIM + 1] + b.p) / rho2;
-  const R ratio = dsqrt(rho2 / rho1);
+  const R H1 = (a.q[DIM + 1] + a.p) * a.ir;
+  const R H2 = (b.q[DIM + 1] + b.p) * b.ir;
+  const R ratio = dsqrt(rho2 * a.ir);
   const R ca = R(1) / (ratio + R(1));
   const R cb = ca * ratio;
   const R roe_rho = ratio * rho1;
@@ -412,8 +452,8 @@ __device__ __forceinline__ void face_roe(const GasC<R> &g, const Pt<R, DIM> &L, 
   const R rootL = dsqrt(rhoL), rootR = dsqrt(rhoR);
   const R root_sum_inv = R(1) / (rootL + rootR);
   const R rho_roe = rootL * rootR;
-  const R HL = (L.q[DIM + 1] + pL) / rhoL;
-  const R HR = (Rs.q[DIM + 1] + pR) / rhoR;
+  const R HL = (L.q[DIM + 1] + pL) * L.ir;
+  const R HR = (Rs.q[DIM + 1] + pR) * Rs.ir;
   const R H = (rootL * HL + rootR * HR) * root_sum_inv;
   R u[DIM], du_t[DIM], nhat[DIM], du[DIM];
   R u2 = R(0), un = R(0), dun = R(0);
@@ -532,43 +572,59 @@ __device__ __forceinline__ void surface_flux(const GasC<R> &g, const Pt<R, DIM> 
 // BR1 helpers (EOS.hpp:159-263, NavierStokesFlux.hpp:389-519)
 // ---------------------------------------------------------------------------
 
-// Entropy variables (EOS.hpp:206-234)
+// Node quantities shared by the entropy variables, the gradient conversion
+// and the viscous flux (EOS.hpp:75-263): 1/rho, velocity, kinetic and
+// internal energy density, pressure.
 template <typename R, int DIM>
-__device__ __forceinline__ void entropy_state(const GasC<R> &g, const R *q, R *E) {
-  const R rho = q[0];
+struct NodeThermo {
+  R rho, ir, u[DIM], ke, ie, p;
+};
+
+template <typename R, int DIM>
+__device__ __forceinline__ NodeThermo<R, DIM> node_thermo(const GasC<R> &g, const R *q) {
+  NodeThermo<R, DIM> t;
+  t.rho = q[0];
+  t.ir = R(1) / q[0];
   R m2 = R(0);
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) m2 += q[1 + d] * q[1 + d];
-  const R ke = R(0.5) * m2 / rho;
-  const R p = g.gammaM1 * (q[DIM + 1] - ke);
-  const R s = dlog(p) - g.gamma * dlog(rho);
-  const R beta = rho / p;
-  const R v2o2 = ke / rho;
-  E[0] = (g.gamma - s) / (g.gamma - R(1)) - beta * v2o2;
+  for (int d = 0; d < DIM; ++d) {
+    m2 += q[1 + d] * q[1 + d];
+    t.u[d] = q[1 + d] * t.ir;
+  }
+  t.ke = R(0.5) * m2 * t.ir;
+  t.ie = q[DIM + 1] - t.ke;
+  t.p = g.gammaM1 * t.ie;
+  return t;
+}
+
+// Entropy variables (EOS.hpp:206-234):
+// ((gamma - s)/(gamma-1) - beta |u|^2/2, beta u, -beta), s = ln p - gamma ln rho, beta = rho/p.
+template <typename R, int DIM>
+__device__ __forceinline__ void entropy_state(const GasC<R> &g, const R *q, R *E) {
+  const NodeThermo<R, DIM> t = node_thermo<R, DIM>(g, q);
+  const R s = dlog(t.p) - g.gamma * dlog(t.rho);
+  const R beta = t.rho / t.p;
+  const R v2o2 = t.ke * t.ir;
+  E[0] = (g.gamma - s) * g.gammaM1Inverse - beta * v2o2;
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) E[1 + d] = beta * (q[1 + d] / rho);
+  for (int d = 0; d < DIM; ++d) E[1 + d] = beta * t.u[d];
   E[DIM + 1] = -beta;
 }
 
 // EOS.hpp:236-263 : gradient of entropy variables -> gradient of (rho, u, p)
 template <typename R, int DIM>
-__device__ __forceinline__ void grad_entropy_to_grad_prim(const GasC<R> &g, const R *q, const R *dE, R *dP) {
-  const R rho = q[0];
-  R m2 = R(0);
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) m2 += q[1 + d] * q[1 + d];
-  const R ke = R(0.5) * m2 / rho;
-  const R ie = q[DIM + 1] - ke;
-  const R p = g.gammaM1 * ie;
+__device__ __forceinline__ void grad_entropy_to_grad_prim(const NodeThermo<R, DIM> &t, const R *q, const R *dE,
+                                                          R *dP) {
+  const R p_over_rho = t.p * t.ir;
   R drho = R(0);
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    dP[1 + d] = p / rho * (dE[1 + d] + (q[1 + d] / rho) * dE[DIM + 1]);
+    dP[1 + d] = p_over_rho * (dE[1 + d] + t.u[d] * dE[DIM + 1]);
     drho += q[1 + d] * dP[1 + d];
   }
-  drho = rho * dE[0] - dE[DIM + 1] * (ke - ie) + rho * drho / p;
+  drho = t.rho * dE[0] - dE[DIM + 1] * (t.ke - t.ie) + t.rho * drho / t.p;
   dP[0] = drho;
-  dP[DIM + 1] = p / rho * (dP[0] + p * dE[DIM + 1]);
+  dP[DIM + 1] = p_over_rho * (dP[0] + t.p * dE[DIM + 1]);
 }
 
 // Number of stored viscous-flux components per node: the symmetric stress
@@ -589,25 +645,19 @@ struct VisComp {
 // Physical viscous flux, NavierStokesFlux.hpp:389-519 (Cartesian); writes the
 // unique components in VisComp order.
 template <typename R, int DIM>
-__device__ __forceinline__ void viscous_flux(const GasC<R> &g, const R *q, const R (*dP)[DIM + 2], R *out) {
+__device__ __forceinline__ void viscous_flux(const GasC<R> &g, const NodeThermo<R, DIM> &t, const R (*dP)[DIM + 2],
+                                             R *out) {
   const R mu = g.mu;
   const R kappa = g.mu * g.cp * g.PrInverse;
   const R mu_bulk = g.mu_bulk;
-  const R rho = q[0];
-  R m2 = R(0);
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) m2 += q[1 + d] * q[1 + d];
-  const R p = g.gammaM1 * (q[DIM + 1] - R(0.5) * m2 / rho);
-  // grad_temperature (EOS.hpp:159-173)
-  const R pressor = p / rho;
-  const R cv = g.cp / g.gamma;
-  const R fac = g.gammaM1Inverse / (cv * rho);
-  R gt[DIM], vel[DIM];
+  // grad_temperature (EOS.hpp:159-173): fac = (1/(gamma-1)) / (cv rho), cv = cp/gamma
+  const R pressor = t.p * t.ir;
+  const R fac = g.gammaM1Inverse * g.gamma / g.cp * t.ir;
+  R gt[DIM];
   R div_vel = R(0);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     gt[d] = fac * (dP[d][DIM + 1] - pressor * dP[d][0]);
-    vel[d] = q[1 + d] / rho;
     div_vel += dP[d][1 + d];
   }
   R tau[DIM][DIM];
@@ -626,7 +676,7 @@ __device__ __forceinline__ void viscous_flux(const GasC<R> &g, const R *q, const
   for (int dir = 0; dir < DIM; ++dir) {
     R ef = kappa * gt[dir];
 #pragma unroll
-    for (int mom = 0; mom < DIM; ++mom) ef += vel[mom] * tau[mom][dir];
+    for (int mom = 0; mom < DIM; ++mom) ef += t.u[mom] * tau[mom][dir];
     out[VisComp<DIM>::en(dir)] = ef;
   }
 }

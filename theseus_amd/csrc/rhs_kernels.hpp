@@ -45,6 +45,7 @@ struct BasisArgs {
   double Dhat[MAXNP * MAXNP];   // D + (1/w0) e0 e0^T - (1/wN) eN eN^T
   double Dhat2[MAXNP * MAXNP];  // 2D with the same corner corrections
   double w[MAXNP];              // GLL weights on [0,1]
+  double Vinv[MAXNP * MAXNP];   // inverse 1-D Legendre Vandermonde: mode a = sum_i Vinv[a*Np+i] u_i
 };
 
 struct RHSArgs {
@@ -60,9 +61,6 @@ struct RHSArgs {
   int ne;
   int blending;
   double alpha_max, alpha_min, sharp, thresh;
-  const double *modal;  // [q*NDOF + m] = Vinv(m, q)
-  const double *keep1;
-  const double *keep2;
   const double *J;      // [ne]
   const double *adj;    // [ne][DIM][DIM]
   const int *nbr;       // [ne][2*DIM]
@@ -101,40 +99,69 @@ __device__ __forceinline__ int logical_block() {
   return b;
 }
 
-// Persson-Peraire indicator for one element (RHSOperator_impl.hpp:398-527):
-// rho*p per node, modal energy ratios, sigmoid.  Reductions follow the
-// reference's power-of-two tree so eta is reproduced bit for bit (the mode
-// sums use non-contracted mul/add like the host reference build).
+// Persson-Peraire indicator of one element (RHSOperator_impl.hpp:355-527):
+// modes of the indicator field pressure*density, eta = max(1 - |M1|^2/|m|^2,
+// 1 - |M2|^2/|M1|^2) with M1 (M2) the modes of degree <= p-1 (p-2) in every
+// direction, alpha = sigmoid(9.21 (eta - theta)/theta) cut at alpha_min and
+// alpha_max.  The reference applies the inverse of the full Legendre
+// Vandermonde (ModalBasis.cpp:115-140); that matrix is the Kronecker
+// product of the 1-D inverse, applied here as one 1-D sweep per direction.
+// `buf` holds at least 2*ND + 3*NPOW2 doubles of the element's scratch.
 template <int DIM, int P>
-__device__ __forceinline__ double blending_alpha(const RHSArgs &a, double (*sred)[Shape<DIM, P>::NPOW2],
-                                                 const double *ind, int node, bool lane_ok, bool valid) {
+__device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, const double *sVi,
+                                                 const double *sp_el, const double *srho_el, int node, int i, int j,
+                                                 int k, bool lane_ok, bool valid) {
   using S = Shape<DIM, P>;
-  constexpr int ND = S::NDOF, NP2 = S::NPOW2;
-  double mode = 0.0;
-  if (valid) {
-    for (int q = 0; q < ND; ++q) mode = __dadd_rn(mode, __dmul_rn(a.modal[q * ND + node], ind[q]));
-  }
-  const double mode2 = __dmul_rn(mode, mode);
+  constexpr int Np = S::Np, ND = S::NDOF, NP2 = S::NPOW2;
+  double *A = buf, *B = buf + ND;
+  if (lane_ok) A[node] = valid ? sp_el[node] * srho_el[node] : 0.0;
+  __syncthreads();
+  // sweep along x: B(a,j,k) = sum_l Vinv(a,l) A(l,j,k)
   if (lane_ok) {
-    sred[0][node] = valid ? mode2 : 0.0;
-    sred[1][node] = valid ? __dmul_rn(a.keep1[node], mode2) : 0.0;
-    sred[2][node] = valid ? __dmul_rn(a.keep2[node], mode2) : 0.0;
-    for (int t = node + ND; t < NP2; t += ND) {  // zero padding up to the power of two
-      sred[0][t] = 0.0;
-      sred[1][t] = 0.0;
-      sred[2][t] = 0.0;
-    }
+    double acc = 0.0;
+    const int base = node - i;
+#pragma unroll
+    for (int l = 0; l < Np; ++l) acc += sVi[i * Np + l] * A[base + l];
+    B[node] = acc;
+  }
+  __syncthreads();
+  if (lane_ok) {
+    double acc = 0.0;
+    const int base = node - j * Np;
+#pragma unroll
+    for (int l = 0; l < Np; ++l) acc += sVi[j * Np + l] * B[base + l * Np];
+    A[node] = acc;
+  }
+  __syncthreads();
+  double mode = A[lane_ok ? node : 0];
+  if constexpr (DIM == 3) {
+    double acc = 0.0;
+    const int base = node - k * Np * Np;
+#pragma unroll
+    for (int l = 0; l < Np; ++l) acc += sVi[k * Np + l] * A[lane_ok ? base + l * Np * Np : 0];
+    mode = acc;
+  }
+  __syncthreads();  // the sweeps' buffers are reused for the reduction below
+  double *red = buf;  // 3 rows of NP2
+  const double mode2 = mode * mode;
+  const bool keep1 = i <= P - 1 && j <= P - 1 && k <= P - 1;
+  const bool keep2 = i <= P - 2 && j <= P - 2 && k <= P - 2;
+  if (lane_ok) {
+    red[node] = valid ? mode2 : 0.0;
+    red[NP2 + node] = valid && keep1 ? mode2 : 0.0;
+    red[2 * NP2 + node] = valid && keep2 ? mode2 : 0.0;
+    for (int t = node + ND; t < NP2; t += ND) red[t] = red[NP2 + t] = red[2 * NP2 + t] = 0.0;
   }
   __syncthreads();
   for (int stride = NP2 / 2; stride > 0; stride /= 2) {
     if (lane_ok && node < stride) {
-      sred[0][node] = __dadd_rn(sred[0][node], sred[0][node + stride]);
-      sred[1][node] = __dadd_rn(sred[1][node], sred[1][node + stride]);
-      sred[2][node] = __dadd_rn(sred[2][node], sred[2][node + stride]);
+      red[node] += red[node + stride];
+      red[NP2 + node] += red[NP2 + node + stride];
+      red[2 * NP2 + node] += red[2 * NP2 + node + stride];
     }
     __syncthreads();
   }
-  const double mm = sred[0][0], m1m1 = sred[1][0], m2m2 = sred[2][0];
+  const double mm = red[0], m1m1 = red[NP2], m2m2 = red[2 * NP2];
   const double eps = 1.0e-30;
   double val = 0.0;
   if (mm > eps) {
@@ -152,24 +179,67 @@ __device__ __forceinline__ double blending_alpha(const RHSArgs &a, double (*sred
 // ---------------------------------------------------------------------------
 // Fused volume + face kernel (Euler; Navier-Stokes pass 2 when VISC)
 // ---------------------------------------------------------------------------
+
+// Index of the pair (a, b), a < b, among the Np(Np-1)/2 pairs of one GLL line.
+template <int Np>
+__device__ __forceinline__ int pair_index(int a, int b) {
+  return a * Np - (a * (a + 1)) / 2 + (b - a - 1);
+}
+template <int Np>
+__device__ __forceinline__ void pair_nodes(int pr, int &a, int &b) {
+  int aa = 0, rem = pr;
+#pragma unroll
+  for (int t = 0; t < Np - 1; ++t) {
+    const int cnt = Np - 1 - t;
+    if (aa == t && rem >= cnt) {
+      rem -= cnt;
+      aa = t + 1;
+    }
+  }
+  a = aa;
+  b = aa + 1 + rem;
+}
+
+template <int DIM>
+__device__ __forceinline__ int vis_index(int q, int dd) {
+  return q == DIM + 1 ? VisComp<DIM>::en(dd) : VisComp<DIM>::tau(q - 1, dd);
+}
+
+template <typename R, int DIM, int ND, int NQ>
+__device__ __forceinline__ void load_pt(const double (*sq)[ND], const double (*sv)[ND], const double *sp,
+                                        const double *sir, int n, Pt<R, DIM> &o) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) o.q[q] = sq[q][n];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) o.v[d] = sv[d][n];
+  o.p = sp[n];
+  o.ir = sir[n];
+}
+
 template <int DIM, int P, int VOL, int SURF, bool VISC>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   using S = Shape<DIM, P>;
-  constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB;
+  constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NV = VisComp<DIM>::N;
   constexpr int NFACE = 2 * DIM * NF;
-  constexpr int NI = (Np - 1) * (ND / Np);  // subcell interfaces per direction
+  constexpr int NLINE = ND / Np;              // GLL lines per direction
+  constexpr int NPL = Np * (Np - 1) / 2;      // node pairs per line
+  constexpr int NPAIR = NLINE * NPL;          // pairs per direction
+  constexpr int NI = (Np - 1) * NLINE;        // subcell interfaces per direction
+  constexpr int NS0 = NPAIR > NFACE ? NPAIR : NFACE;
+  constexpr int NS1 = NS0 > NI ? NS0 : NI;
+  constexpr int NIND = 2 * ND + 3 * S::NPOW2;  // indicator sweeps + reduction
+  constexpr int NSCR = (NIND > NS1 * NQ) ? (NIND + NQ - 1) / NQ : NS1;  // scratch rows
 
   __shared__ double sq[EPB][NQ][ND];
   __shared__ double sv[EPB][DIM][ND];
   __shared__ double sp[EPB][ND];
-  __shared__ double sface[EPB][2 * DIM][NF][NQ];
+  __shared__ double sir[EPB][ND];
+  __shared__ double sFv[VISC ? EPB : 1][VISC ? NV : 1][ND];
+  __shared__ double sscr[EPB][NSCR][NQ];  // pair fluxes / face terms / FV fluxes / indicator sums
   __shared__ double sD2[Np * Np];
   __shared__ double sDh[Np * Np];
-  __shared__ double sFv[VISC ? EPB : 1][VISC ? NV : 1][ND];
-  __shared__ double sred[EPB][3][S::NPOW2];
-  __shared__ double sfv[EPB][NI][NQ];
-  __shared__ double sind[EPB][ND];
+  __shared__ double sVi[Np * Np];
   __shared__ int sblend;
 
   const GasC<double> &g = a.gas;
@@ -178,28 +248,34 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   const int node = tid - el * ND;
   const bool lane_ok = el < EPB;
   const int elc = lane_ok ? el : 0;
-  const int e = logical_block() * EPB + el;
+  const int ebase_blk = logical_block() * EPB;
+  const int e = ebase_blk + el;
   const bool valid = lane_ok && e < a.ne;
   const long long Ntot = a.Ntot;
 
-  for (int k = tid; k < Np * Np; k += S::TPB) {
+  for (int k = tid; k < Np * Np; k += TPB) {
     sD2[k] = a.basis.Dhat2[k];
     sDh[k] = a.basis.Dhat[k];
+    sVi[k] = a.basis.Vinv[k];
   }
   if (tid == 0) sblend = 0;
 
-  Pt<double, DIM> me;
+  // ---- stage the element's state (coalesced: nodes are contiguous per equation)
   const size_t ebase = (size_t)(valid ? e : 0) * ND + node;
+  double qown[NQ];
   if (valid) {
+    Pt<double, DIM> me;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       me.q[q] = a.U[q * Ntot + ebase];
+      qown[q] = me.q[q];
       sq[el][q][node] = me.q[q];
     }
     derive(g, me);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) sv[el][d][node] = me.v[d];
     sp[el][node] = me.p;
+    sir[el][node] = me.ir;
     if constexpr (VISC) {
 #pragma unroll
       for (int c = 0; c < NV; ++c) sFv[el][c][node] = a.Fv[c * Ntot + ebase];
@@ -212,53 +288,61 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   const int k = DIM > 2 ? node / (Np * Np) : 0;
   const int idx[3] = {i, j, k};
   constexpr int stride[3] = {1, Np, Np * Np};
-  const double *adj_e = a.adj + (size_t)(valid ? e : 0) * DIM * DIM;
   const double Je = valid ? a.J[e] : 1.0;
 
-  // ---- volume: split-form flux differencing (DGSEMIntegrator.hpp:124-218)
+  // ---- blending coefficient first: it only needs the staged state
+  double alpha = 0.0;
+  if (a.blending) {
+    alpha = blending_alpha<DIM, P>(a, &sscr[elc][0][0], sVi, &sp[elc][0], &sq[elc][0][0], node, i, j, k, lane_ok,
+                                   valid);
+    if (valid && node == 0) a.alpha[e] = alpha;
+    if (valid && alpha > 1e-16) sblend = 1;
+    __syncthreads();
+  }
+
+  // ---- volume: split-form flux differencing, each pair evaluated once
+  //      (DGSEMIntegrator.hpp:124-218: dU_i = -(1/J) sum_dir sum_{m != i} Dhat2(i,m) F#(u_lo,u_hi))
   double rate[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) rate[q] = 0.0;
-  if (valid) {
-#pragma unroll
-    for (int dir = 0; dir < DIM; ++dir) {
-      double met[DIM];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) met[d] = adj_e[dir * DIM + d];
-      const int mi = idx[dir];
-      const int base = node - mi * stride[dir];
-#pragma unroll
-      for (int o = 1; o < Np; ++o) {
-        int m = mi + o;
-        if (m >= Np) m -= Np;
-        const int pn = base + m * stride[dir];
-        Pt<double, DIM> pt;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) pt.q[q] = sq[el][q][pn];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) pt.v[d] = sv[el][d][pn];
-        pt.p = sp[el][pn];
-        const bool lower = m < mi;
+#pragma unroll 1
+  for (int dir = 0; dir < DIM; ++dir) {
+    const int sdir = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
+    for (int w = tid; w < EPB * NPAIR; w += TPB) {
+      const int pe = w / NPAIR;
+      const int r = w - pe * NPAIR;
+      if (ebase_blk + pe < a.ne) {
+        const int line = r / NPL;
+        int pa, pb;
+        pair_nodes<Np>(r - line * NPL, pa, pb);
+        const int base = face_node<DIM, Np>(dir, 0, line);
         Pt<double, DIM> lo, hi;
+        load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], base + pa * sdir, lo);
+        load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], base + pb * sdir, hi);
+        const double *adj_p = a.adj + (size_t)(ebase_blk + pe) * DIM * DIM + dir * DIM;
+        double met[DIM];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          lo.q[q] = lower ? pt.q[q] : me.q[q];
-          hi.q[q] = lower ? me.q[q] : pt.q[q];
-        }
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          lo.v[d] = lower ? pt.v[d] : me.v[d];
-          hi.v[d] = lower ? me.v[d] : pt.v[d];
-        }
-        lo.p = lower ? pt.p : me.p;
-        hi.p = lower ? me.p : pt.p;
+        for (int d = 0; d < DIM; ++d) met[d] = adj_p[d];
         double F[NQ];
         volume_flux<VOL, double, DIM>(g, lo, hi, met, F);
-        const double coef = sD2[mi * Np + m];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) rate[q] += coef * F[q];
+        for (int q = 0; q < NQ; ++q) sscr[pe][r][q] = F[q];
       }
     }
+    __syncthreads();
+    if (valid) {
+      const int mi = dir == 0 ? i : (dir == 1 ? j : k);
+      const int line = face_index<DIM, Np>(dir, i, j, k);
+#pragma unroll
+      for (int m = 0; m < Np; ++m) {
+        if (m == mi) continue;
+        const int pr = m < mi ? pair_index<Np>(m, mi) : pair_index<Np>(mi, m);
+        const double coef = sD2[mi * Np + m];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr[el][line * NPL + pr][q];
+      }
+    }
+    __syncthreads();
   }
   double du[NQ];
   {
@@ -267,159 +351,79 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
     for (int q = 0; q < NQ; ++q) du[q] = scale * rate[q];
   }
 
-  // ---- blending: indicator and subcell FV (RHSOperator_impl.hpp:355-437,
-  //      EulerOperator_impl.hpp:120-157, DGSEMIntegrator.hpp:338-483)
-  if (a.blending) {
-    if (valid) sind[el][node] = me.p * me.q[0];  // indicator field pressure*density
-    __syncthreads();
-    const double alpha = blending_alpha<DIM, P>(a, sred[elc], sind[elc], node, lane_ok, valid);
-    if (valid && node == 0) a.alpha[e] = alpha;
-    if (valid && alpha > 1e-16) sblend = 1;
-    __syncthreads();
-    if (sblend) {
-      double dfv[NQ];
+  // ---- subcell FV residual and blend (EulerOperator_impl.hpp:120-157, DGSEMIntegrator.hpp:338-483)
+  if (a.blending && sblend) {
+    double dfv[NQ];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) dfv[q] = 0.0;
+    for (int q = 0; q < NQ; ++q) dfv[q] = 0.0;
 #pragma unroll
-      for (int dir = 0; dir < DIM; ++dir) {
-        if (valid) {
+    for (int dir = 0; dir < DIM; ++dir) {
+      for (int w = tid; w < EPB * NI; w += TPB) {
+        const int pe = w / NI;
+        const int ii = w - pe * NI;
+        if (ebase_blk + pe < a.ne) {
+          const int L = ii / (Np - 1), pos = ii - L * (Np - 1);
+          const int ln = face_node<DIM, Np>(dir, 0, L) + pos * stride[dir];
+          Pt<double, DIM> sl, sr;
+          load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], ln, sl);
+          load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], ln + stride[dir], sr);
+          const double *adj_p = a.adj + (size_t)(ebase_blk + pe) * DIM * DIM + dir * DIM;
           double nor[DIM];
 #pragma unroll
-          for (int d = 0; d < DIM; ++d) nor[d] = adj_e[dir * DIM + d];
-          for (int ii = node; ii < NI; ii += ND) {
-            const int L = ii / (Np - 1), pos = ii - L * (Np - 1);
-            const int ln = face_node<DIM, Np>(dir, 0, L) + pos * stride[dir];
-            const int rn = ln + stride[dir];
-            Pt<double, DIM> sl, sr;
+          for (int d = 0; d < DIM; ++d) nor[d] = adj_p[d];
+          double F[NQ];
+          surface_flux<SURF, double, DIM>(g, sl, sr, nor, F);
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              sl.q[q] = sq[el][q][ln];
-              sr.q[q] = sq[el][q][rn];
-            }
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-              sl.v[d] = sv[el][d][ln];
-              sr.v[d] = sv[el][d][rn];
-            }
-            sl.p = sp[el][ln];
-            sr.p = sp[el][rn];
-            double F[NQ];
-            surface_flux<SURF, double, DIM>(g, sl, sr, nor, F);
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) sfv[el][ii][q] = F[q];
-          }
+          for (int q = 0; q < NQ; ++q) sscr[pe][ii][q] = F[q];
         }
-        __syncthreads();
-        if (valid) {
-          const int L = face_index<DIM, Np>(dir, i, j, k);
-          const int pos = idx[dir];
-          const double den = Je * a.basis.w[pos];
+      }
+      __syncthreads();
+      if (valid) {
+        const int L = face_index<DIM, Np>(dir, i, j, k);
+        const int pos = dir == 0 ? i : (dir == 1 ? j : k);
+        const double den = Je * a.basis.w[pos];
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const double fm = pos > 0 ? sfv[el][L * (Np - 1) + pos - 1][q] : 0.0;
-            const double fp = pos < Np - 1 ? sfv[el][L * (Np - 1) + pos][q] : 0.0;
-            const double val = (fm - fp) / den;
-            dfv[q] = dir == 0 ? val : dfv[q] + val;
-          }
+        for (int q = 0; q < NQ; ++q) {
+          const double fm = pos > 0 ? sscr[el][L * (Np - 1) + pos - 1][q] : 0.0;
+          const double fp = pos < Np - 1 ? sscr[el][L * (Np - 1) + pos][q] : 0.0;
+          const double val = (fm - fp) / den;
+          dfv[q] = dir == 0 ? val : dfv[q] + val;
         }
-        __syncthreads();
       }
-      if (valid && alpha > 1e-16) {
-        const double a_dg = 1.0 - alpha;
+      __syncthreads();
+    }
+    if (valid && alpha > 1e-16) {
+      const double a_dg = 1.0 - alpha;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) du[q] = a_dg * du[q] + alpha * dfv[q];
-      }
+      for (int q = 0; q < NQ; ++q) du[q] = a_dg * du[q] + alpha * dfv[q];
     }
   }
 
-  // ---- faces (EulerOperator_impl.hpp:211-392; NSOperator_impl.hpp:579-713)
-  const double w0 = a.basis.w[0];
-  if (valid) {
-    const double wown = 1.0 / (w0 * Je);
-    for (int fi = node; fi < NFACE; fi += ND) {
-      const int f = fi / NF;
-      const int t = fi - f * NF;
-      const int d = f >> 1, s = f & 1;
-      const int on = face_node<DIM, Np>(d, s, t);
-      Pt<double, DIM> own;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) own.q[q] = sq[el][q][on];
-#pragma unroll
-      for (int dd = 0; dd < DIM; ++dd) own.v[dd] = sv[el][dd][on];
-      own.p = sp[el][on];
-      const int nb = a.nbr[(size_t)e * 2 * DIM + f];
-      double F[NQ], contrib_scale;
-      if (nb >= 0) {
-        const int nn = face_node<DIM, Np>(d, 1 - s, t);
-        const size_t nbase = (size_t)nb * ND + nn;
-        Pt<double, DIM> oth;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) oth.q[q] = a.U[q * Ntot + nbase];
-        derive(g, oth);
-        double n[DIM];
-        const double *adj_L = s ? adj_e : a.adj + (size_t)nb * DIM * DIM;
-#pragma unroll
-        for (int dd = 0; dd < DIM; ++dd) n[dd] = adj_L[d * DIM + dd];
-        if (s) surface_flux<SURF, double, DIM>(g, own, oth, n, F);
-        else surface_flux<SURF, double, DIM>(g, oth, own, n, F);
-        if constexpr (VISC) {
-          // F* - n . 0.5 (Fv_L + Fv_R)   (DGSEMIntegrator.hpp:261-316)
-          double vo[NV], vn[NV];
-#pragma unroll
-          for (int c = 0; c < NV; ++c) {
-            vo[c] = sFv[el][c][on];
-            vn[c] = a.Fv[c * Ntot + nbase];
-          }
-          const double *vL = s ? vo : vn;
-          const double *vR = s ? vn : vo;
-#pragma unroll
-          for (int q = 1; q < NQ; ++q)
-#pragma unroll
-            for (int dd = 0; dd < DIM; ++dd) {
-              const int c = (q == NQ - 1) ? VisComp<DIM>::en(dd) : VisComp<DIM>::tau(q - 1, dd);
-              const double avg = 0.5 * (vL[c] + vR[c]);
-              F[q] -= n[dd] * avg;
-            }
-        }
-        contrib_scale = s ? -wown : wown;
-      } else {
-        // boundary: supersonic outflow F*(q,q,n_out) (bc_kernels.hpp:236-238)
-        double n[DIM];
-#pragma unroll
-        for (int dd = 0; dd < DIM; ++dd) n[dd] = s ? adj_e[d * DIM + dd] : -adj_e[d * DIM + dd];
-        surface_flux<SURF, double, DIM>(g, own, own, n, F);
-        contrib_scale = -wown;
-      }
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) sface[el][f][t][q] = contrib_scale * F[q];
-    }
-  }
-
-  // ---- viscous volume divergence (DGSEMIntegrator.hpp:486-622)
+  // ---- viscous volume divergence: (1/J) sum_dir sum_l Dhat(i,l) (adj_dir . Fv)(l)
+  //      (DGSEMIntegrator.hpp:486-622)
   if constexpr (VISC) {
     if (valid) {
+      const double *adj_e = a.adj + (size_t)e * DIM * DIM;
       double dv[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) dv[q] = 0.0;
-#pragma unroll
+#pragma unroll 1
       for (int dir = 0; dir < DIM; ++dir) {
         double row[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) row[d] = adj_e[dir * DIM + d];
-        const int mi = idx[dir];
-        const int base = node - mi * stride[dir];
+        const int mi = dir == 0 ? i : (dir == 1 ? j : k);
+        const int sdir = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
+        const int base = node - mi * sdir;
 #pragma unroll
         for (int l = 0; l < Np; ++l) {
-          const int sn = base + l * stride[dir];
+          const int sn = base + l * sdir;
           const double coef = sDh[mi * Np + l];
 #pragma unroll
           for (int q = 1; q < NQ; ++q) {
-            double fr = 0.0;
+            double fr = row[0] * sFv[el][vis_index<DIM>(q, 0)][sn];
 #pragma unroll
-            for (int ph = 0; ph < DIM; ++ph) {
-              const int c = (q == NQ - 1) ? VisComp<DIM>::en(ph) : VisComp<DIM>::tau(q - 1, ph);
-              fr += row[ph] * sFv[el][c][sn];
-            }
+            for (int ph = 1; ph < DIM; ++ph) fr += row[ph] * sFv[el][vis_index<DIM>(q, ph)][sn];
             dv[q] += coef * fr;
           }
         }
@@ -428,6 +432,63 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
 #pragma unroll
       for (int q = 1; q < NQ; ++q) du[q] += sc * dv[q];
     }
+  }
+
+  // ---- faces: Riemann flux at every point of the element's own faces
+  //      (EulerOperator_impl.hpp:211-392; NSOperator_impl.hpp:579-713)
+  const double w0 = a.basis.w[0];
+  for (int w = tid; w < EPB * NFACE; w += TPB) {
+    const int pe = w / NFACE;
+    const int fi = w - pe * NFACE;
+    const int ee = ebase_blk + pe;
+    if (ee >= a.ne) continue;
+    const int f = fi / NF;
+    const int t = fi - f * NF;
+    const int d = f >> 1, s = f & 1;
+    const int on = face_node<DIM, Np>(d, s, t);
+    Pt<double, DIM> own;
+    load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], on, own);
+    const double *adj_o = a.adj + (size_t)ee * DIM * DIM;
+    const double wown = 1.0 / (w0 * a.J[ee]);
+    const int nb = a.nbr[(size_t)ee * 2 * DIM + f];
+    double F[NQ], cs;
+    if (nb >= 0) {
+      const int nn = face_node<DIM, Np>(d, 1 - s, t);
+      const size_t nbase = (size_t)nb * ND + nn;
+      Pt<double, DIM> oth;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) oth.q[q] = a.U[q * Ntot + nbase];
+      derive(g, oth);
+      const double *adj_L = s ? adj_o : a.adj + (size_t)nb * DIM * DIM;
+      double n[DIM];
+#pragma unroll
+      for (int dd = 0; dd < DIM; ++dd) n[dd] = adj_L[d * DIM + dd];
+      if (s) surface_flux<SURF, double, DIM>(g, own, oth, n, F);
+      else surface_flux<SURF, double, DIM>(g, oth, own, n, F);
+      if constexpr (VISC) {
+        // F* - n . 0.5 (Fv_L + Fv_R)   (DGSEMIntegrator.hpp:261-316)
+#pragma unroll
+        for (int q = 1; q < NQ; ++q)
+#pragma unroll
+          for (int dd = 0; dd < DIM; ++dd) {
+            const int c = vis_index<DIM>(q, dd);
+            const double vo = sFv[pe][c][on];
+            const double vn = a.Fv[c * Ntot + nbase];
+            const double avg = s ? 0.5 * (vo + vn) : 0.5 * (vn + vo);
+            F[q] -= n[dd] * avg;
+          }
+      }
+      cs = s ? -wown : wown;
+    } else {
+      // boundary: supersonic outflow F*(q,q,n_out) (bc_kernels.hpp:236-238)
+      double n[DIM];
+#pragma unroll
+      for (int dd = 0; dd < DIM; ++dd) n[dd] = s ? adj_o[d * DIM + dd] : -adj_o[d * DIM + dd];
+      surface_flux<SURF, double, DIM>(g, own, own, n, F);
+      cs = -wown;
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sscr[pe][fi][q] = cs * F[q];
   }
   __syncthreads();
 
@@ -441,11 +502,11 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
       const int t = face_index<DIM, Np>(d, i, j, k);
       if (idx[d] == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) fsum[q] += sface[el][2 * d][t][q];
+        for (int q = 0; q < NQ; ++q) fsum[q] += sscr[el][(2 * d) * NF + t][q];
       }
       if (idx[d] == Np - 1) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) fsum[q] += sface[el][2 * d + 1][t][q];
+        for (int q = 0; q < NQ; ++q) fsum[q] += sscr[el][(2 * d + 1) * NF + t][q];
       }
     }
 #pragma unroll
@@ -453,11 +514,8 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
       const double r = du[q] + fsum[q];
       double o;
       if (a.mode == 0) o = r;
-      else if (a.mode == 1) o = me.q[q] + a.dt * r;
-      else {
-        const double tmp = me.q[q] + a.dt * r;
-        o = a.cx * a.X[q * Ntot + ebase] + a.cy * tmp;
-      }
+      else if (a.mode == 1) o = qown[q] + a.dt * r;
+      else o = a.cx * a.X[q * Ntot + ebase] + a.cy * (qown[q] + a.dt * r);
       a.out[q * Ntot + ebase] = o;
     }
   }
@@ -472,12 +530,12 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
 template <int DIM, int P>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   using S = Shape<DIM, P>;
-  constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB;
+  constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NV = VisComp<DIM>::N;
   constexpr int NFACE = 2 * DIM * NF;
 
   __shared__ double sS[EPB][NQ][ND];
-  __shared__ double sjump[EPB][2 * DIM][NF][NQ];
+  __shared__ double sjump[EPB][NFACE][NQ];
   __shared__ double sn[EPB][2 * DIM][DIM];
   __shared__ double sD[Np * Np];
 
@@ -486,51 +544,58 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   const int el = tid / ND;
   const int node = tid - el * ND;
   const bool lane_ok = el < EPB;
-  const int e = logical_block() * EPB + el;
+  const int ebase_blk = logical_block() * EPB;
+  const int e = ebase_blk + el;
   const bool valid = lane_ok && e < a.ne;
   const long long Ntot = a.Ntot;
 
-  for (int k = tid; k < Np * Np; k += S::TPB) sD[k] = a.basis.D[k];
+  for (int k = tid; k < Np * Np; k += TPB) sD[k] = a.basis.D[k];
 
-  double q[NQ];
+  // entropy variables of the element's nodes (EOS.hpp:206-234)
   const size_t ebase = (size_t)(valid ? e : 0) * ND + node;
   if (valid) {
+    double q[NQ], Sv[NQ];
 #pragma unroll
     for (int c = 0; c < NQ; ++c) q[c] = a.U[c * Ntot + ebase];
-    double Sv[NQ];
     entropy_state<double, DIM>(g, q, Sv);
 #pragma unroll
     for (int c = 0; c < NQ; ++c) sS[el][c][node] = Sv[c];
   }
-  const double *adj_e = a.adj + (size_t)(valid ? e : 0) * DIM * DIM;
-  const double Je = valid ? a.J[e] : 1.0;
-  const double w0 = a.basis.w[0];
+  // face normals of the low-side element of every face
+  for (int w = tid; w < EPB * 2 * DIM; w += TPB) {
+    const int pe = w / (2 * DIM), f = w - pe * 2 * DIM;
+    const int ee = ebase_blk + pe;
+    if (ee < a.ne) {
+      const int nb = a.nbr[(size_t)ee * 2 * DIM + f];
+      const int d = f >> 1;
+      const double *adj_L = (f & 1) ? a.adj + (size_t)ee * DIM * DIM : a.adj + (size_t)nb * DIM * DIM;
+#pragma unroll
+      for (int dd = 0; dd < DIM; ++dd) sn[pe][f][dd] = adj_L[d * DIM + dd];
+    }
+  }
   __syncthreads();
 
-  // face lift data: canonical jump 0.5*(S_R - S_L) and the face normal of the low-side element
-  if (valid) {
-    for (int fi = node; fi < NFACE; fi += ND) {
-      const int f = fi / NF;
-      const int t = fi - f * NF;
-      const int d = f >> 1, s = f & 1;
-      const int on = face_node<DIM, Np>(d, s, t);
-      const int nb = a.nbr[(size_t)e * 2 * DIM + f];
-      // (interior faces only; Navier-Stokes boundaries are rejected at create time)
-      const int nn = face_node<DIM, Np>(d, 1 - s, t);
-      double qn[NQ], Sn[NQ];
+  // canonical interior-face jumps 0.5*(S_R - S_L) (AssembleGradInteriorFacePointKernel);
+  // boundary faces do not occur (Navier-Stokes boundaries are rejected at create time)
+  for (int w = tid; w < EPB * NFACE; w += TPB) {
+    const int pe = w / NFACE;
+    const int fi = w - pe * NFACE;
+    const int ee = ebase_blk + pe;
+    if (ee >= a.ne) continue;
+    const int f = fi / NF;
+    const int t = fi - f * NF;
+    const int d = f >> 1, s = f & 1;
+    const int on = face_node<DIM, Np>(d, s, t);
+    const int nb = a.nbr[(size_t)ee * 2 * DIM + f];
+    const int nn = face_node<DIM, Np>(d, 1 - s, t);
+    double qn[NQ], Sn[NQ];
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) qn[c] = a.U[c * Ntot + (size_t)nb * ND + nn];
-      entropy_state<double, DIM>(g, qn, Sn);
+    for (int c = 0; c < NQ; ++c) qn[c] = a.U[c * Ntot + (size_t)nb * ND + nn];
+    entropy_state<double, DIM>(g, qn, Sn);
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) {
-        const double so = sS[el][c][on];
-        sjump[el][f][t][c] = s ? 0.5 * (Sn[c] - so) : 0.5 * (so - Sn[c]);
-      }
-      if (t == 0) {
-        const double *adj_L = s ? adj_e : a.adj + (size_t)nb * DIM * DIM;
-#pragma unroll
-        for (int dd = 0; dd < DIM; ++dd) sn[el][f][dd] = adj_L[d * DIM + dd];
-      }
+    for (int c = 0; c < NQ; ++c) {
+      const double so = sS[pe][c][on];
+      sjump[pe][fi][c] = s ? 0.5 * (Sn[c] - so) : 0.5 * (so - Sn[c]);
     }
   }
   __syncthreads();
@@ -540,43 +605,50 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   const int j = DIM > 1 ? (node / Np) % Np : 0;
   const int k = DIM > 2 ? node / (Np * Np) : 0;
   const int idx[3] = {i, j, k};
-  constexpr int stride[3] = {1, Np, Np * Np};
+  const double *adj_e = a.adj + (size_t)e * DIM * DIM;
+  const double invJ = 1.0 / a.J[e];
+  const double wown = 1.0 / (a.basis.w[0] * a.J[e]);
 
-  // volume gradient (AssembleGradVolumePointKernel)
-  double dref[DIM][NQ];
-#pragma unroll
+  // volume gradient (AssembleGradVolumePointKernel): acc[dd] = sum_ref (D.S)_ref adj(ref,dd),
+  // one reference direction at a time, in the reference's ref = 0,1,2 order
+  double acc[DIM][NQ];
+#pragma unroll 1
   for (int dir = 0; dir < DIM; ++dir) {
+    double dref[NQ];
 #pragma unroll
-    for (int c = 0; c < NQ; ++c) dref[dir][c] = 0.0;
-    const int mi = idx[dir];
-    const int base = node - mi * stride[dir];
+    for (int c = 0; c < NQ; ++c) dref[c] = 0.0;
+    const int mi = dir == 0 ? i : (dir == 1 ? j : k);
+    const int st = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
+    const int base = node - mi * st;
 #pragma unroll
     for (int l = 0; l < Np; ++l) {
       const double coef = sD[mi * Np + l];
-      const int sn_ = base + l * stride[dir];
+      const int sn_ = base + l * st;
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) dref[dir][c] += sS[el][c][sn_] * coef;
+      for (int c = 0; c < NQ; ++c) dref[c] += sS[el][c][sn_] * coef;
+    }
+#pragma unroll
+    for (int dd = 0; dd < DIM; ++dd) {
+      const double ad = adj_e[dir * DIM + dd];
+#pragma unroll
+      for (int c = 0; c < NQ; ++c) acc[dd][c] = dir == 0 ? dref[c] * ad : acc[dd][c] + dref[c] * ad;
     }
   }
-  const double invJ = 1.0 / Je;
-  double grad[DIM][NQ];
+  double q[NQ];
 #pragma unroll
-  for (int dd = 0; dd < DIM; ++dd)
+  for (int c = 0; c < NQ; ++c) q[c] = a.U[c * Ntot + ebase];
+  const NodeThermo<double, DIM> th = node_thermo<double, DIM>(g, q);
+
+  double dP[DIM][NQ];
+#pragma unroll
+  for (int dd = 0; dd < DIM; ++dd) {
+    double gr[NQ], lift[NQ];
 #pragma unroll
     for (int c = 0; c < NQ; ++c) {
-      double acc = dref[0][c] * adj_e[0 * DIM + dd];
-#pragma unroll
-      for (int r = 1; r < DIM; ++r) acc += dref[r][c] * adj_e[r * DIM + dd];
-      grad[dd][c] = invJ * acc;
+      gr[c] = invJ * acc[dd][c];
+      lift[c] = 0.0;
     }
-  // interior-face lift (AssembleGradInteriorFacePointKernel): + w * (jump * n_d)
-  {
-    const double wown = 1.0 / (w0 * Je);
-    double lift[DIM][NQ];
-#pragma unroll
-    for (int dd = 0; dd < DIM; ++dd)
-#pragma unroll
-      for (int c = 0; c < NQ; ++c) lift[dd][c] = 0.0;
+    // + w * (jump * n_dd) from every face through the node
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       const int t = face_index<DIM, Np>(d, i, j, k);
@@ -584,26 +656,18 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
       for (int s = 0; s < 2; ++s) {
         if (idx[d] == (s ? Np - 1 : 0)) {
           const int f = 2 * d + s;
+          const double nd = sn[el][f][dd];
 #pragma unroll
-          for (int dd = 0; dd < DIM; ++dd) {
-            const double nd = sn[el][f][dd];
-#pragma unroll
-            for (int c = 0; c < NQ; ++c) lift[dd][c] += wown * (sjump[el][f][t][c] * nd);
-          }
+          for (int c = 0; c < NQ; ++c) lift[c] += wown * (sjump[el][f * NF + t][c] * nd);
         }
       }
     }
 #pragma unroll
-    for (int dd = 0; dd < DIM; ++dd)
-#pragma unroll
-      for (int c = 0; c < NQ; ++c) grad[dd][c] += lift[dd][c];
+    for (int c = 0; c < NQ; ++c) gr[c] += lift[c];
+    grad_entropy_to_grad_prim<double, DIM>(th, q, gr, dP[dd]);
   }
-  // primitive gradients and viscous flux
-  double dP[DIM][NQ];
-#pragma unroll
-  for (int dd = 0; dd < DIM; ++dd) grad_entropy_to_grad_prim<double, DIM>(g, q, grad[dd], dP[dd]);
   double vf[NV];
-  viscous_flux<double, DIM>(g, q, dP, vf);
+  viscous_flux<double, DIM>(g, th, dP, vf);
 #pragma unroll
   for (int c = 0; c < NV; ++c) a.Fv_out[c * Ntot + ebase] = vf[c];
 }
