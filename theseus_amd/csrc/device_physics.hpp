@@ -38,6 +38,23 @@ template <typename R>
 __device__ __forceinline__ R dlog(R x) { return ln_pos(x); }
 template <>
 __device__ __forceinline__ float dlog<float>(float x) { return logf(x); }
+// Reciprocal of a normal, finite x: hardware estimate refined by two Newton
+// steps r <- r + r (1 - x r) (quadratic convergence, result within ~1 ulp),
+// 5 VALU instead of the 19 of an IEEE-rounded division.
+template <typename R>
+__device__ __forceinline__ R drcp(R x) {
+  R r = __builtin_amdgcn_rcp(x);
+  R e = fma(-x, r, R(1));
+  r = fma(r, e, r);
+  e = fma(-x, r, R(1));
+  return fma(r, e, r);
+}
+template <>
+__device__ __forceinline__ float drcp<float>(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  const float e = fmaf(-x, r, 1.0f);
+  return fmaf(r, e, r);
+}
 template <typename R>
 __device__ __forceinline__ R dabs(R x) { return fabs(x); }
 template <>
@@ -82,7 +99,7 @@ __device__ __forceinline__ LogMean<R> log_mean2(R x, R y) {
   const R s = x + y;
   const R d = y - x;
   if (d * d < R(LogMeanEps<R>::value) * (s * s)) {
-    const R rs = R(1) / s;
+    const R rs = drcp(s);
     const R t = d * rs;
     const R u = t * t;
     if constexpr (NEED_LM) r.lm = (R(0.5) * s) * (R(1) - u * (R(1.0 / 3.0) + u * (R(4.0 / 45.0) + u * R(44.0 / 945.0))));
@@ -113,7 +130,7 @@ __device__ __forceinline__ R log_mean(R x, R y) {
 // divisions by rho replaced by one reciprocal.
 template <typename R, int DIM>
 __device__ __forceinline__ void derive(const GasC<R> &g, Pt<R, DIM> &s) {
-  s.ir = R(1) / s.q[0];
+  s.ir = drcp(s.q[0]);
   R m2 = R(0);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
@@ -584,7 +601,7 @@ template <typename R, int DIM>
 __device__ __forceinline__ NodeThermo<R, DIM> node_thermo(const GasC<R> &g, const R *q) {
   NodeThermo<R, DIM> t;
   t.rho = q[0];
-  t.ir = R(1) / q[0];
+  t.ir = drcp(q[0]);
   R m2 = R(0);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {

@@ -205,15 +205,17 @@ __device__ __forceinline__ int vis_index(int q, int dd) {
   return q == DIM + 1 ? VisComp<DIM>::en(dd) : VisComp<DIM>::tau(q - 1, dd);
 }
 
+// A node's point sample from the staged state: velocity is m * (1/rho),
+// the same product derive() forms, so staged and fresh samples agree bitwise.
 template <typename R, int DIM, int ND, int NQ>
-__device__ __forceinline__ void load_pt(const double (*sq)[ND], const double (*sv)[ND], const double *sp,
-                                        const double *sir, int n, Pt<R, DIM> &o) {
+__device__ __forceinline__ void load_pt(const double (*sq)[ND], const double *sp, const double *sir, int n,
+                                        Pt<R, DIM> &o) {
 #pragma unroll
   for (int q = 0; q < NQ; ++q) o.q[q] = sq[q][n];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) o.v[d] = sv[d][n];
   o.p = sp[n];
   o.ir = sir[n];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) o.v[d] = o.q[1 + d] * o.ir;
 }
 
 template <int DIM, int P, int VOL, int SURF, bool VISC>
@@ -232,7 +234,6 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   constexpr int NSCR = (NIND > NS1 * NQ) ? (NIND + NQ - 1) / NQ : NS1;  // scratch rows
 
   __shared__ double sq[EPB][NQ][ND];
-  __shared__ double sv[EPB][DIM][ND];
   __shared__ double sp[EPB][ND];
   __shared__ double sir[EPB][ND];
   __shared__ double sFv[VISC ? EPB : 1][VISC ? NV : 1][ND];
@@ -272,8 +273,6 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
       sq[el][q][node] = me.q[q];
     }
     derive(g, me);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) sv[el][d][node] = me.v[d];
     sp[el][node] = me.p;
     sir[el][node] = me.ir;
     if constexpr (VISC) {
@@ -317,8 +316,8 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
         pair_nodes<Np>(r - line * NPL, pa, pb);
         const int base = face_node<DIM, Np>(dir, 0, line);
         Pt<double, DIM> lo, hi;
-        load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], base + pa * sdir, lo);
-        load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], base + pb * sdir, hi);
+        load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], base + pa * sdir, lo);
+        load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], base + pb * sdir, hi);
         const double *adj_p = a.adj + (size_t)(ebase_blk + pe) * DIM * DIM + dir * DIM;
         double met[DIM];
 #pragma unroll
@@ -365,8 +364,8 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
           const int L = ii / (Np - 1), pos = ii - L * (Np - 1);
           const int ln = face_node<DIM, Np>(dir, 0, L) + pos * stride[dir];
           Pt<double, DIM> sl, sr;
-          load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], ln, sl);
-          load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], ln + stride[dir], sr);
+          load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], ln, sl);
+          load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], ln + stride[dir], sr);
           const double *adj_p = a.adj + (size_t)(ebase_blk + pe) * DIM * DIM + dir * DIM;
           double nor[DIM];
 #pragma unroll
@@ -447,7 +446,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
     const int d = f >> 1, s = f & 1;
     const int on = face_node<DIM, Np>(d, s, t);
     Pt<double, DIM> own;
-    load_pt<double, DIM, ND, NQ>(sq[pe], sv[pe], sp[pe], sir[pe], on, own);
+    load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], on, own);
     const double *adj_o = a.adj + (size_t)ee * DIM * DIM;
     const double wown = 1.0 / (w0 * a.J[ee]);
     const int nb = a.nbr[(size_t)ee * 2 * DIM + f];
