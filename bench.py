@@ -30,19 +30,22 @@ def log(msg):
 
 def algorithmic_bytes(mesh_ne, order, dim, kind, stage_modes):
     """Compulsory bytes per launch of one kernel kind, averaged over the
-    launches of a step.  Per node: conservative state (dim+2 doubles), the
-    stored viscous flux (dim*(dim+1)/2 + dim doubles), the output state and,
-    for SSP-RK3 stages 2 and 3, the step's base state."""
+    launches of a step.  Per node: the conservative state (dim+2 doubles),
+    the viscous volume divergence (dim+1), n.Fv at the element's face points
+    (dim+1 per face point, 2*dim*(p+1)^(dim-1) face points per element), the
+    output state and, for SSP-RK3 stages 2 and 3, the step's base state."""
     ndof = (order + 1) ** dim
     nodes = mesh_ne * ndof
     neq = dim + 2
-    nv = dim * (dim + 1) // 2 + dim
-    if kind == 1:  # NS pass 1: read U, write Fv
-        per_node = 8 * (neq + nv)
+    nvd = dim + 1
+    nfp = 2 * dim * (order + 1) ** (dim - 1)
+    visc = nvd + nvd * nfp / ndof  # volume divergence + face normal fluxes, per node
+    if kind == 1:  # NS pass 1: read U, write both viscous arrays
+        per_node = 8 * (neq + visc)
         return nodes * per_node, per_node / neq
     per = []
-    for mode in stage_modes:  # pass 2: read U, Fv, write out, (+ read X)
-        per.append(8 * (neq + nv + neq + (neq if mode == 2 else 0)))
+    for mode in stage_modes:  # pass 2: read U and the viscous arrays, write out (+ read X)
+        per.append(8 * (neq + visc + neq + (neq if mode == 2 else 0)))
     per_node = sum(per) / len(per)
     return nodes * per_node, per_node / neq
 

@@ -219,7 +219,7 @@ struct theseus_rhs {
   double *d_J = nullptr, *d_adj = nullptr;
   int *d_nbr = nullptr;
   double *d_alpha = nullptr;
-  double *d_Fv = nullptr;       // NS viscous flux components
+  double *d_Fv = nullptr;       // NS: viscous volume divergence, then n.Fv at face points
   double *d_stage[2] = {nullptr, nullptr};
   unsigned long long *d_rates = nullptr;
   bool has_boundary = false;
@@ -234,8 +234,12 @@ struct theseus_rhs {
     a.U = U;
     a.out = out;
     a.mode = 0;
-    a.Fv = d_Fv;
-    a.Fv_out = d_Fv;
+    if (d_Fv) {
+      a.Vd = d_Fv;
+      a.Gf = d_Fv + (size_t)(d.dim + 1) * Ntot;
+      a.Vd_out = d_Fv;
+      a.Gf_out = d_Fv + (size_t)(d.dim + 1) * Ntot;
+    }
     a.alpha = d_alpha;
     a.Ntot = Ntot;
     a.ne = d.num_elements;
@@ -381,7 +385,7 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
     return rc;
   }
   if (d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
-    const size_t nv = (size_t)(dim == 2 ? 5 : 9) * h->Ntot;
+    const size_t nv = (size_t)(dim + 1) * ((size_t)h->Ntot + (size_t)ne * 2 * dim * h->nfp);
     if ((rc = upload<double>(&h->d_Fv, nullptr, nv))) {
       theseus_rhs_destroy(h);
       return rc;

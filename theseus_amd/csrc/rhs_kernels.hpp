@@ -54,8 +54,10 @@ struct RHSArgs {
   const double *X;    // RK base state for mode 2
   double cx, cy, dt;
   int mode;
-  const double *Fv;   // NS: viscous flux components (VisComp layout, SoA over nodes)
-  double *Fv_out;     // NS pass 1 output
+  const double *Vd;   // NS: viscous volume divergence, [DIM+1][Ntot] (momentum, energy)
+  const double *Gf;   // NS: n.Fv at face points, [DIM+1][ne][2*DIM][NFP]
+  double *Vd_out;     // NS pass 1 outputs
+  double *Gf_out;
   double *alpha;      // blending coefficient per element (written when blending)
   long long Ntot;     // ne * NDOF
   int ne;
@@ -91,6 +93,10 @@ __device__ __forceinline__ int face_index(int d, int i, int j, int k) {
   }
 }
 
+// Workgroup barrier for LDS traffic only: global loads still in flight stay
+// in flight across it and are waited for where their registers are used.
+__device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // XCD-aware block order: the 8 XCDs each receive a contiguous range of
 // elements so that face neighbours share an L2.
 __device__ __forceinline__ int logical_block() {
@@ -115,7 +121,7 @@ __device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, 
   constexpr int Np = S::Np, ND = S::NDOF, NP2 = S::NPOW2;
   double *A = buf, *B = buf + ND;
   if (lane_ok) A[node] = valid ? sp_el[node] * srho_el[node] : 0.0;
-  __syncthreads();
+  lds_sync();
   // sweep along x: B(a,j,k) = sum_l Vinv(a,l) A(l,j,k)
   if (lane_ok) {
     double acc = 0.0;
@@ -124,7 +130,7 @@ __device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, 
     for (int l = 0; l < Np; ++l) acc += sVi[i * Np + l] * A[base + l];
     B[node] = acc;
   }
-  __syncthreads();
+  lds_sync();
   if (lane_ok) {
     double acc = 0.0;
     const int base = node - j * Np;
@@ -132,7 +138,7 @@ __device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, 
     for (int l = 0; l < Np; ++l) acc += sVi[j * Np + l] * B[base + l * Np];
     A[node] = acc;
   }
-  __syncthreads();
+  lds_sync();
   double mode = A[lane_ok ? node : 0];
   if constexpr (DIM == 3) {
     double acc = 0.0;
@@ -141,7 +147,7 @@ __device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, 
     for (int l = 0; l < Np; ++l) acc += sVi[k * Np + l] * A[lane_ok ? base + l * Np * Np : 0];
     mode = acc;
   }
-  __syncthreads();  // the sweeps' buffers are reused for the reduction below
+  lds_sync();  // the sweeps' buffers are reused for the reduction below
   double *red = buf;  // 3 rows of NP2
   const double mode2 = mode * mode;
   const bool keep1 = i <= P - 1 && j <= P - 1 && k <= P - 1;
@@ -152,14 +158,14 @@ __device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, 
     red[2 * NP2 + node] = valid && keep2 ? mode2 : 0.0;
     for (int t = node + ND; t < NP2; t += ND) red[t] = red[NP2 + t] = red[2 * NP2 + t] = 0.0;
   }
-  __syncthreads();
+  lds_sync();
   for (int stride = NP2 / 2; stride > 0; stride /= 2) {
     if (lane_ok && node < stride) {
       red[node] += red[node + stride];
       red[NP2 + node] += red[NP2 + node + stride];
       red[2 * NP2 + node] += red[2 * NP2 + node + stride];
     }
-    __syncthreads();
+    lds_sync();
   }
   const double mm = red[0], m1m1 = red[NP2], m2m2 = red[2 * NP2];
   const double eps = 1.0e-30;
@@ -218,12 +224,101 @@ __device__ __forceinline__ void load_pt(const double (*sq)[ND], const double *sp
   for (int d = 0; d < DIM; ++d) o.v[d] = o.q[1 + d] * o.ir;
 }
 
+// Per-element geometry staged in LDS: adjugate rows, the canonical area
+// vector of each face (the low-side element's adjugate row; the outward one
+// on a boundary) and the Jacobian determinant.
+template <int DIM, int EPB>
+struct ElemMeta {
+  double adj[EPB][DIM * DIM];
+  double nrm[EPB][2 * DIM][DIM];
+  double J[EPB];
+};
+
+// Geometry loads.  Every load is unconditional (indices clamped into the
+// arrays, results selected afterwards) so that the in-order vector-memory
+// counter lets a wave wait for exactly the loads it needs; they are issued
+// in two dependent waves (neighbour indices, then the neighbours' adjugate
+// rows) and written to LDS by meta_store once the other loads are issued.
+template <int DIM, int EPB, int TPB>
+struct MetaRegs {
+  static constexpr int NA = (EPB * DIM * DIM + EPB + TPB - 1) / TPB;
+  static constexpr int NN = (EPB * 2 * DIM * DIM + TPB - 1) / TPB;
+  double va[NA];
+  double vn[NN];
+  int nb[NN];
+};
+
+template <int DIM, int EPB, int TPB>
+__device__ __forceinline__ void meta_load_index(const RHSArgs &a, int ebase_blk, MetaRegs<DIM, EPB, TPB> &m) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < MetaRegs<DIM, EPB, TPB>::NN; ++r) {
+    const int w = min(tid + r * TPB, EPB * 2 * DIM * DIM - 1);
+    const int pe = w / (2 * DIM * DIM);
+    const int f = (w - pe * 2 * DIM * DIM) / DIM;
+    const int ee = min(ebase_blk + pe, a.ne - 1);
+    m.nb[r] = a.nbr[(size_t)ee * 2 * DIM + f];
+  }
+}
+
+template <int DIM, int EPB, int TPB>
+__device__ __forceinline__ void meta_load_own(const RHSArgs &a, int ebase_blk, MetaRegs<DIM, EPB, TPB> &m) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < MetaRegs<DIM, EPB, TPB>::NA; ++r) {
+    const int w = min(tid + r * TPB, EPB * DIM * DIM + EPB - 1);
+    const bool is_adj = w < EPB * DIM * DIM;
+    const int pe = is_adj ? w / (DIM * DIM) : w - EPB * DIM * DIM;
+    const int ee = min(ebase_blk + pe, a.ne - 1);
+    const double *src = is_adj ? a.adj + (size_t)ee * DIM * DIM + (w - pe * DIM * DIM) : a.J + ee;
+    m.va[r] = *src;
+  }
+}
+
+template <int DIM, int EPB, int TPB>
+__device__ __forceinline__ void meta_load_normals(const RHSArgs &a, int ebase_blk, MetaRegs<DIM, EPB, TPB> &m) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < MetaRegs<DIM, EPB, TPB>::NN; ++r) {
+    const int w = min(tid + r * TPB, EPB * 2 * DIM * DIM - 1);
+    const int pe = w / (2 * DIM * DIM);
+    const int rem = w - pe * 2 * DIM * DIM;
+    const int f = rem / DIM, dd = rem - f * DIM;
+    const int d = f >> 1, s = f & 1;
+    const int ee = min(ebase_blk + pe, a.ne - 1);
+    const int nb = m.nb[r];
+    const int src = (nb >= 0 && !s) ? nb : ee;
+    m.vn[r] = a.adj[(size_t)src * DIM * DIM + d * DIM + dd];
+  }
+}
+
+template <int DIM, int EPB, int TPB>
+__device__ __forceinline__ void meta_store(const MetaRegs<DIM, EPB, TPB> &m, ElemMeta<DIM, EPB> &s) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < MetaRegs<DIM, EPB, TPB>::NA; ++r) {
+    const int w = tid + r * TPB;
+    if (w < EPB * DIM * DIM) s.adj[w / (DIM * DIM)][w % (DIM * DIM)] = m.va[r];
+    else if (w < EPB * DIM * DIM + EPB) s.J[w - EPB * DIM * DIM] = m.va[r];
+  }
+#pragma unroll
+  for (int r = 0; r < MetaRegs<DIM, EPB, TPB>::NN; ++r) {
+    const int w = tid + r * TPB;
+    if (w < EPB * 2 * DIM * DIM) {
+      const int f = (w % (2 * DIM * DIM)) / DIM;
+      // a boundary face takes the outward area vector of the element itself
+      (&s.nrm[0][0][0])[w] = (m.nb[r] < 0 && !(f & 1)) ? -m.vn[r] : m.vn[r];
+    }
+  }
+}
+
 template <int DIM, int P, int VOL, int SURF, bool VISC>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   using S = Shape<DIM, P>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
-  constexpr int NV = VisComp<DIM>::N;
+  constexpr int NVD = DIM + 1;                // viscous terms: momentum and energy
   constexpr int NFACE = 2 * DIM * NF;
+  constexpr int NFR = (EPB * NFACE + TPB - 1) / TPB;  // face-point rounds per lane
   constexpr int NLINE = ND / Np;              // GLL lines per direction
   constexpr int NPL = Np * (Np - 1) / 2;      // node pairs per line
   constexpr int NPAIR = NLINE * NPL;          // pairs per direction
@@ -236,11 +331,10 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   __shared__ double sq[EPB][NQ][ND];
   __shared__ double sp[EPB][ND];
   __shared__ double sir[EPB][ND];
-  __shared__ double sFv[VISC ? EPB : 1][VISC ? NV : 1][ND];
-  __shared__ double sscr[EPB][NSCR][NQ];  // pair fluxes / face terms / FV fluxes / indicator sums
+  __shared__ double sscr[EPB][NSCR][NQ];  // face terms / pair fluxes / FV fluxes / indicator sums
   __shared__ double sD2[Np * Np];
-  __shared__ double sDh[Np * Np];
   __shared__ double sVi[Np * Np];
+  __shared__ ElemMeta<DIM, EPB> sm;
   __shared__ int sblend;
 
   const GasC<double> &g = a.gas;
@@ -253,50 +347,169 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   const int e = ebase_blk + el;
   const bool valid = lane_ok && e < a.ne;
   const long long Ntot = a.Ntot;
+  const long long Gtot = (long long)a.ne * NFACE;
 
-  for (int k = tid; k < Np * Np; k += TPB) {
-    sD2[k] = a.basis.Dhat2[k];
-    sDh[k] = a.basis.Dhat[k];
-    sVi[k] = a.basis.Vinv[k];
+  // ---- global loads, all before the first barrier and unconditional:
+  //      neighbour indices first, then the element's own data, then what
+  //      depends on the indices (the neighbours' face traces and adjugate rows)
+  constexpr int NFT = EPB * NFACE;
+  int fnb[NFR];
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int fi = min(tid + r * TPB, NFT - 1);
+    const int pe = fi / NFACE;
+    const int ee = min(ebase_blk + pe, a.ne - 1);
+    fnb[r] = a.nbr[(size_t)ee * 2 * DIM + (fi - pe * NFACE) / NF];
   }
-  if (tid == 0) sblend = 0;
+  MetaRegs<DIM, EPB, TPB> mr;
+  meta_load_index<DIM, EPB, TPB>(a, ebase_blk, mr);
+  meta_load_own<DIM, EPB, TPB>(a, ebase_blk, mr);
+
+  const size_t ebase = (size_t)(valid ? e : 0) * ND + node;
+  double qown[NQ], fsum[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    qown[q] = a.U[q * Ntot + ebase];
+    fsum[q] = 0.0;
+  }
+  if constexpr (VISC) {
+    // viscous volume divergence from the gradient kernel
+#pragma unroll
+    for (int c = 0; c < NVD; ++c) fsum[1 + c] = a.Vd[c * Ntot + ebase];
+  }
+  static_assert(Np * Np <= TPB, "one basis entry per lane");
+  const int kb = min(tid, Np * Np - 1);
+  const double bD2 = a.basis.Dhat2[kb], bVi = a.basis.Vinv[kb];
+
+  meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
+  double qn[NFR][NQ];
+  double gn[NFR][VISC ? NVD : 1], go[NFR][VISC ? NVD : 1];
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int fi = min(tid + r * TPB, NFT - 1);
+    const int pe = fi / NFACE;
+    const int fl = fi - pe * NFACE;
+    const int f = fl / NF, t = fl - f * NF;
+    const int d = f >> 1, s = f & 1;
+    const int nbs = max(fnb[r], 0);
+    const size_t nbase = (size_t)nbs * ND + face_node<DIM, Np>(d, 1 - s, t);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
+    if constexpr (VISC) {
+      const size_t gb_n = (size_t)nbs * NFACE + (f ^ 1) * NF + t;
+      const size_t gb_o = (size_t)min(ebase_blk + pe, a.ne - 1) * NFACE + fl;
+#pragma unroll
+      for (int c = 0; c < NVD; ++c) {
+        gn[r][c] = a.Gf[c * Gtot + gb_n];
+        go[r][c] = a.Gf[c * Gtot + gb_o];
+      }
+    }
+  }
+  if (tid < Np * Np) {
+    sD2[tid] = bD2;
+    sVi[tid] = bVi;
+  }
 
   // ---- stage the element's state (coalesced: nodes are contiguous per equation)
-  const size_t ebase = (size_t)(valid ? e : 0) * ND + node;
-  double qown[NQ];
+  meta_store<DIM, EPB, TPB>(mr, sm);
+  if (tid == 0) sblend = 0;
   if (valid) {
     Pt<double, DIM> me;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      me.q[q] = a.U[q * Ntot + ebase];
-      qown[q] = me.q[q];
-      sq[el][q][node] = me.q[q];
+      me.q[q] = qown[q];
+      sq[el][q][node] = qown[q];
     }
     derive(g, me);
     sp[el][node] = me.p;
     sir[el][node] = me.ir;
-    if constexpr (VISC) {
-#pragma unroll
-      for (int c = 0; c < NV; ++c) sFv[el][c][node] = a.Fv[c * Ntot + ebase];
-    }
   }
-  __syncthreads();
+  lds_sync();
 
   const int i = node % Np;
   const int j = DIM > 1 ? (node / Np) % Np : 0;
   const int k = DIM > 2 ? node / (Np * Np) : 0;
   const int idx[3] = {i, j, k};
   constexpr int stride[3] = {1, Np, Np * Np};
-  const double Je = valid ? a.J[e] : 1.0;
+  const double Je = sm.J[elc];
 
-  // ---- blending coefficient first: it only needs the staged state
+  // ---- faces: Riemann flux at every point of the element's own faces
+  //      (EulerOperator_impl.hpp:211-392; NSOperator_impl.hpp:579-713),
+  //      neighbour traces from the registers loaded above
+  {
+    const double w0 = a.basis.w[0];
+#pragma unroll
+    for (int r = 0; r < NFR; ++r) {
+      const int fi = tid + r * TPB;
+      const int pe = fi / NFACE;
+      const int fl = fi - pe * NFACE;
+      if (fi < EPB * NFACE && ebase_blk + pe < a.ne) {
+        const int f = fl / NF, t = fl - f * NF;
+        const int d = f >> 1, s = f & 1;
+        Pt<double, DIM> own;
+        load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], face_node<DIM, Np>(d, s, t), own);
+        const double wown = 1.0 / (w0 * sm.J[pe]);
+        double n[DIM];
+#pragma unroll
+        for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[pe][f][dd];
+        double F[NQ], cs;
+        if (fnb[r] >= 0) {
+          Pt<double, DIM> oth;  // the neighbour's trace (loaded at the top)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) oth.q[q] = qn[r][q];
+          derive(g, oth);
+          if (s) surface_flux<SURF, double, DIM>(g, own, oth, n, F);
+          else surface_flux<SURF, double, DIM>(g, oth, own, n, F);
+          if constexpr (VISC) {
+            // F* - 0.5 n.(Fv_L + Fv_R)   (DGSEMIntegrator.hpp:261-316)
+#pragma unroll
+            for (int c = 0; c < NVD; ++c) F[1 + c] -= s ? 0.5 * (go[r][c] + gn[r][c]) : 0.5 * (gn[r][c] + go[r][c]);
+          }
+          cs = s ? -wown : wown;
+        } else {
+          // boundary: supersonic outflow F*(q,q,n_out) (bc_kernels.hpp:236-238)
+          surface_flux<SURF, double, DIM>(g, own, own, n, F);
+          cs = -wown;
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) sscr[pe][fl][q] = cs * F[q];
+      }
+    }
+  }
+  lds_sync();
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int t = face_index<DIM, Np>(d, i, j, k);
+      if (idx[d] == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) fsum[q] += sscr[el][(2 * d) * NF + t][q];
+      }
+      if (idx[d] == Np - 1) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) fsum[q] += sscr[el][(2 * d + 1) * NF + t][q];
+      }
+    }
+  }
+  lds_sync();
+
+  // RK base state: only read now, so that its loads overlap the volume work
+  double xb[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) xb[q] = 0.0;
+  if (a.mode == 2) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) xb[q] = a.X[q * Ntot + ebase];
+  }
+
+  // ---- blending coefficient (needs only the staged state)
   double alpha = 0.0;
   if (a.blending) {
     alpha = blending_alpha<DIM, P>(a, &sscr[elc][0][0], sVi, &sp[elc][0], &sq[elc][0][0], node, i, j, k, lane_ok,
                                    valid);
     if (valid && node == 0) a.alpha[e] = alpha;
     if (valid && alpha > 1e-16) sblend = 1;
-    __syncthreads();
+    lds_sync();
   }
 
   // ---- volume: split-form flux differencing, each pair evaluated once
@@ -318,17 +531,16 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
         Pt<double, DIM> lo, hi;
         load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], base + pa * sdir, lo);
         load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], base + pb * sdir, hi);
-        const double *adj_p = a.adj + (size_t)(ebase_blk + pe) * DIM * DIM + dir * DIM;
         double met[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) met[d] = adj_p[d];
+        for (int d = 0; d < DIM; ++d) met[d] = sm.adj[pe][dir * DIM + d];
         double F[NQ];
         volume_flux<VOL, double, DIM>(g, lo, hi, met, F);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) sscr[pe][r][q] = F[q];
       }
     }
-    __syncthreads();
+    lds_sync();
     if (valid) {
       const int mi = dir == 0 ? i : (dir == 1 ? j : k);
       const int line = face_index<DIM, Np>(dir, i, j, k);
@@ -341,7 +553,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
         for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr[el][line * NPL + pr][q];
       }
     }
-    __syncthreads();
+    lds_sync();
   }
   double du[NQ];
   {
@@ -366,17 +578,16 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
           Pt<double, DIM> sl, sr;
           load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], ln, sl);
           load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], ln + stride[dir], sr);
-          const double *adj_p = a.adj + (size_t)(ebase_blk + pe) * DIM * DIM + dir * DIM;
           double nor[DIM];
 #pragma unroll
-          for (int d = 0; d < DIM; ++d) nor[d] = adj_p[d];
+          for (int d = 0; d < DIM; ++d) nor[d] = sm.adj[pe][dir * DIM + d];
           double F[NQ];
           surface_flux<SURF, double, DIM>(g, sl, sr, nor, F);
 #pragma unroll
           for (int q = 0; q < NQ; ++q) sscr[pe][ii][q] = F[q];
         }
       }
-      __syncthreads();
+      lds_sync();
       if (valid) {
         const int L = face_index<DIM, Np>(dir, i, j, k);
         const int pos = dir == 0 ? i : (dir == 1 ? j : k);
@@ -389,7 +600,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
           dfv[q] = dir == 0 ? val : dfv[q] + val;
         }
       }
-      __syncthreads();
+      lds_sync();
     }
     if (valid && alpha > 1e-16) {
       const double a_dg = 1.0 - alpha;
@@ -398,123 +609,16 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
     }
   }
 
-  // ---- viscous volume divergence: (1/J) sum_dir sum_l Dhat(i,l) (adj_dir . Fv)(l)
-  //      (DGSEMIntegrator.hpp:486-622)
-  if constexpr (VISC) {
-    if (valid) {
-      const double *adj_e = a.adj + (size_t)e * DIM * DIM;
-      double dv[NQ];
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) dv[q] = 0.0;
-#pragma unroll 1
-      for (int dir = 0; dir < DIM; ++dir) {
-        double row[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) row[d] = adj_e[dir * DIM + d];
-        const int mi = dir == 0 ? i : (dir == 1 ? j : k);
-        const int sdir = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
-        const int base = node - mi * sdir;
-#pragma unroll
-        for (int l = 0; l < Np; ++l) {
-          const int sn = base + l * sdir;
-          const double coef = sDh[mi * Np + l];
-#pragma unroll
-          for (int q = 1; q < NQ; ++q) {
-            double fr = row[0] * sFv[el][vis_index<DIM>(q, 0)][sn];
-#pragma unroll
-            for (int ph = 1; ph < DIM; ++ph) fr += row[ph] * sFv[el][vis_index<DIM>(q, ph)][sn];
-            dv[q] += coef * fr;
-          }
-        }
-      }
-      const double sc = 1.0 / Je;
-#pragma unroll
-      for (int q = 1; q < NQ; ++q) du[q] += sc * dv[q];
-    }
-  }
-
-  // ---- faces: Riemann flux at every point of the element's own faces
-  //      (EulerOperator_impl.hpp:211-392; NSOperator_impl.hpp:579-713)
-  const double w0 = a.basis.w[0];
-  for (int w = tid; w < EPB * NFACE; w += TPB) {
-    const int pe = w / NFACE;
-    const int fi = w - pe * NFACE;
-    const int ee = ebase_blk + pe;
-    if (ee >= a.ne) continue;
-    const int f = fi / NF;
-    const int t = fi - f * NF;
-    const int d = f >> 1, s = f & 1;
-    const int on = face_node<DIM, Np>(d, s, t);
-    Pt<double, DIM> own;
-    load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], on, own);
-    const double *adj_o = a.adj + (size_t)ee * DIM * DIM;
-    const double wown = 1.0 / (w0 * a.J[ee]);
-    const int nb = a.nbr[(size_t)ee * 2 * DIM + f];
-    double F[NQ], cs;
-    if (nb >= 0) {
-      const int nn = face_node<DIM, Np>(d, 1 - s, t);
-      const size_t nbase = (size_t)nb * ND + nn;
-      Pt<double, DIM> oth;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) oth.q[q] = a.U[q * Ntot + nbase];
-      derive(g, oth);
-      const double *adj_L = s ? adj_o : a.adj + (size_t)nb * DIM * DIM;
-      double n[DIM];
-#pragma unroll
-      for (int dd = 0; dd < DIM; ++dd) n[dd] = adj_L[d * DIM + dd];
-      if (s) surface_flux<SURF, double, DIM>(g, own, oth, n, F);
-      else surface_flux<SURF, double, DIM>(g, oth, own, n, F);
-      if constexpr (VISC) {
-        // F* - n . 0.5 (Fv_L + Fv_R)   (DGSEMIntegrator.hpp:261-316)
-#pragma unroll
-        for (int q = 1; q < NQ; ++q)
-#pragma unroll
-          for (int dd = 0; dd < DIM; ++dd) {
-            const int c = vis_index<DIM>(q, dd);
-            const double vo = sFv[pe][c][on];
-            const double vn = a.Fv[c * Ntot + nbase];
-            const double avg = s ? 0.5 * (vo + vn) : 0.5 * (vn + vo);
-            F[q] -= n[dd] * avg;
-          }
-      }
-      cs = s ? -wown : wown;
-    } else {
-      // boundary: supersonic outflow F*(q,q,n_out) (bc_kernels.hpp:236-238)
-      double n[DIM];
-#pragma unroll
-      for (int dd = 0; dd < DIM; ++dd) n[dd] = s ? adj_o[d * DIM + dd] : -adj_o[d * DIM + dd];
-      surface_flux<SURF, double, DIM>(g, own, own, n, F);
-      cs = -wown;
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) sscr[pe][fi][q] = cs * F[q];
-  }
-  __syncthreads();
-
-  // ---- gather face contributions and write
+  // ---- epilogue: dU, or the fused SSP-RK3 stage combination
   if (valid) {
-    double fsum[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) fsum[q] = 0.0;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      const int t = face_index<DIM, Np>(d, i, j, k);
-      if (idx[d] == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) fsum[q] += sscr[el][(2 * d) * NF + t][q];
-      }
-      if (idx[d] == Np - 1) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) fsum[q] += sscr[el][(2 * d + 1) * NF + t][q];
-      }
-    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const double r = du[q] + fsum[q];
+      const double qo = sq[el][q][node];
       double o;
       if (a.mode == 0) o = r;
-      else if (a.mode == 1) o = qown[q] + a.dt * r;
-      else o = a.cx * a.X[q * Ntot + ebase] + a.cy * (qown[q] + a.dt * r);
+      else if (a.mode == 1) o = qo + a.dt * r;
+      else o = a.cx * xb[q] + a.cy * (qo + a.dt * r);
       a.out[q * Ntot + ebase] = o;
     }
   }
@@ -524,151 +628,241 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
 // Navier-Stokes pass 1 (BR1): entropy variables, their gradient with the
 // interior-face lift, conversion to primitive gradients and the physical
 // viscous flux (NSOperator_impl.hpp:10-577, DGSEMIntegrator.hpp:764-1044,
-// EOS.hpp:206-263, NavierStokesFlux.hpp:389-519).
+// EOS.hpp:206-263, NavierStokesFlux.hpp:389-519).  It stores what the
+// second kernel consumes: the viscous volume divergence
+// Vd = (1/J) sum_dir sum_l Dhat(i,l) (adj_dir . Fv)(l)  (DGSEMIntegrator.hpp:486-622)
+// per node, and n.Fv at every point of the element's faces, n the face's
+// canonical area vector.
 // ---------------------------------------------------------------------------
 template <int DIM, int P>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   using S = Shape<DIM, P>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NV = VisComp<DIM>::N;
+  constexpr int NVD = DIM + 1;
   constexpr int NFACE = 2 * DIM * NF;
+  constexpr int NFR = (EPB * NFACE + TPB - 1) / TPB;
+  constexpr int NA = EPB * NQ * ND + EPB * NFACE * NQ;  // entropy variables + face jumps
+  constexpr int NB = EPB * DIM * NVD * ND;              // contravariant viscous fluxes
+  constexpr int NBUF = NA > NB ? NA : NB;
 
-  __shared__ double sS[EPB][NQ][ND];
-  __shared__ double sjump[EPB][NFACE][NQ];
-  __shared__ double sn[EPB][2 * DIM][DIM];
+  __shared__ double sbuf[NBUF];
   __shared__ double sD[Np * Np];
+  __shared__ double sDh[Np * Np];
+  __shared__ ElemMeta<DIM, EPB> sm;
+  double (*sS)[NQ][ND] = reinterpret_cast<double (*)[NQ][ND]>(sbuf);
+  double (*sjump)[NFACE][NQ] = reinterpret_cast<double (*)[NFACE][NQ]>(sbuf + EPB * NQ * ND);
+  double (*sG)[DIM][NVD][ND] = reinterpret_cast<double (*)[DIM][NVD][ND]>(sbuf);
 
   const GasC<double> &g = a.gas;
   const int tid = threadIdx.x;
   const int el = tid / ND;
   const int node = tid - el * ND;
   const bool lane_ok = el < EPB;
+  const int elc = lane_ok ? el : 0;
   const int ebase_blk = logical_block() * EPB;
   const int e = ebase_blk + el;
   const bool valid = lane_ok && e < a.ne;
   const long long Ntot = a.Ntot;
+  const long long Gtot = (long long)a.ne * NFACE;
 
-  for (int k = tid; k < Np * Np; k += TPB) sD[k] = a.basis.D[k];
+  // ---- loads (unconditional, clamped): neighbour indices, own data, then
+  //      the neighbours' face traces
+  constexpr int NFT = EPB * NFACE;
+  int fnb[NFR];
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int fi = min(tid + r * TPB, NFT - 1);
+    const int pe = fi / NFACE;
+    const int ee = min(ebase_blk + pe, a.ne - 1);
+    fnb[r] = a.nbr[(size_t)ee * 2 * DIM + (fi - pe * NFACE) / NF];
+  }
+  MetaRegs<DIM, EPB, TPB> mr;
+  meta_load_index<DIM, EPB, TPB>(a, ebase_blk, mr);
+  meta_load_own<DIM, EPB, TPB>(a, ebase_blk, mr);
+  const size_t ebase = (size_t)(valid ? e : 0) * ND + node;
+  double q[NQ];
+#pragma unroll
+  for (int c = 0; c < NQ; ++c) q[c] = a.U[c * Ntot + ebase];
+  static_assert(Np * Np <= TPB, "one basis entry per lane");
+  const int kb = min(tid, Np * Np - 1);
+  const double bD = a.basis.D[kb], bDh = a.basis.Dhat[kb];
+  meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
+  double qn[NFR][NQ];
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int fi = min(tid + r * TPB, NFT - 1);
+    const int pe = fi / NFACE;
+    const int fl = fi - pe * NFACE;
+    const int f = fl / NF, t = fl - f * NF;
+    const int d = f >> 1, s = f & 1;
+    const size_t nbase = (size_t)max(fnb[r], 0) * ND + face_node<DIM, Np>(d, 1 - s, t);
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) qn[r][c] = a.U[c * Ntot + nbase];
+  }
+  if (tid < Np * Np) {
+    sD[tid] = bD;
+    sDh[tid] = bDh;
+  }
 
   // entropy variables of the element's nodes (EOS.hpp:206-234)
-  const size_t ebase = (size_t)(valid ? e : 0) * ND + node;
+  meta_store<DIM, EPB, TPB>(mr, sm);
   if (valid) {
-    double q[NQ], Sv[NQ];
-#pragma unroll
-    for (int c = 0; c < NQ; ++c) q[c] = a.U[c * Ntot + ebase];
+    double Sv[NQ];
     entropy_state<double, DIM>(g, q, Sv);
 #pragma unroll
     for (int c = 0; c < NQ; ++c) sS[el][c][node] = Sv[c];
   }
-  // face normals of the low-side element of every face
-  for (int w = tid; w < EPB * 2 * DIM; w += TPB) {
-    const int pe = w / (2 * DIM), f = w - pe * 2 * DIM;
-    const int ee = ebase_blk + pe;
-    if (ee < a.ne) {
-      const int nb = a.nbr[(size_t)ee * 2 * DIM + f];
-      const int d = f >> 1;
-      const double *adj_L = (f & 1) ? a.adj + (size_t)ee * DIM * DIM : a.adj + (size_t)nb * DIM * DIM;
-#pragma unroll
-      for (int dd = 0; dd < DIM; ++dd) sn[pe][f][dd] = adj_L[d * DIM + dd];
-    }
-  }
-  __syncthreads();
+  lds_sync();
 
   // canonical interior-face jumps 0.5*(S_R - S_L) (AssembleGradInteriorFacePointKernel);
   // boundary faces do not occur (Navier-Stokes boundaries are rejected at create time)
-  for (int w = tid; w < EPB * NFACE; w += TPB) {
-    const int pe = w / NFACE;
-    const int fi = w - pe * NFACE;
-    const int ee = ebase_blk + pe;
-    if (ee >= a.ne) continue;
-    const int f = fi / NF;
-    const int t = fi - f * NF;
-    const int d = f >> 1, s = f & 1;
-    const int on = face_node<DIM, Np>(d, s, t);
-    const int nb = a.nbr[(size_t)ee * 2 * DIM + f];
-    const int nn = face_node<DIM, Np>(d, 1 - s, t);
-    double qn[NQ], Sn[NQ];
 #pragma unroll
-    for (int c = 0; c < NQ; ++c) qn[c] = a.U[c * Ntot + (size_t)nb * ND + nn];
-    entropy_state<double, DIM>(g, qn, Sn);
+  for (int r = 0; r < NFR; ++r) {
+    const int fi = tid + r * TPB;
+    const int pe = fi / NFACE;
+    const int fl = fi - pe * NFACE;
+    if (fi < NFT && ebase_blk + pe < a.ne && fnb[r] >= 0) {
+      const int f = fl / NF, t = fl - f * NF;
+      const int d = f >> 1, s = f & 1;
+      const int on = face_node<DIM, Np>(d, s, t);
+      double Sn[NQ];
+      entropy_state<double, DIM>(g, qn[r], Sn);
 #pragma unroll
-    for (int c = 0; c < NQ; ++c) {
-      const double so = sS[pe][c][on];
-      sjump[pe][fi][c] = s ? 0.5 * (Sn[c] - so) : 0.5 * (so - Sn[c]);
+      for (int c = 0; c < NQ; ++c) {
+        const double so = sS[pe][c][on];
+        sjump[pe][fl][c] = s ? 0.5 * (Sn[c] - so) : 0.5 * (so - Sn[c]);
+      }
     }
   }
-  __syncthreads();
-  if (!valid) return;
+  lds_sync();
 
   const int i = node % Np;
   const int j = DIM > 1 ? (node / Np) % Np : 0;
   const int k = DIM > 2 ? node / (Np * Np) : 0;
   const int idx[3] = {i, j, k};
-  const double *adj_e = a.adj + (size_t)e * DIM * DIM;
-  const double invJ = 1.0 / a.J[e];
-  const double wown = 1.0 / (a.basis.w[0] * a.J[e]);
+  const double Je = sm.J[elc];
+  const double invJ = 1.0 / Je;
+  const double wown = 1.0 / (a.basis.w[0] * Je);
 
-  // volume gradient (AssembleGradVolumePointKernel): acc[dd] = sum_ref (D.S)_ref adj(ref,dd),
-  // one reference direction at a time, in the reference's ref = 0,1,2 order
-  double acc[DIM][NQ];
+  double vf[NV];
+  if (valid) {
+    // volume gradient (AssembleGradVolumePointKernel): acc[dd] = sum_ref (D.S)_ref adj(ref,dd),
+    // one reference direction at a time, in the reference's ref = 0,1,2 order
+    double acc[DIM][NQ];
 #pragma unroll 1
-  for (int dir = 0; dir < DIM; ++dir) {
-    double dref[NQ];
+    for (int dir = 0; dir < DIM; ++dir) {
+      double dref[NQ];
 #pragma unroll
-    for (int c = 0; c < NQ; ++c) dref[c] = 0.0;
-    const int mi = dir == 0 ? i : (dir == 1 ? j : k);
-    const int st = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
-    const int base = node - mi * st;
+      for (int c = 0; c < NQ; ++c) dref[c] = 0.0;
+      const int mi = dir == 0 ? i : (dir == 1 ? j : k);
+      const int st = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
+      const int base = node - mi * st;
 #pragma unroll
-    for (int l = 0; l < Np; ++l) {
-      const double coef = sD[mi * Np + l];
-      const int sn_ = base + l * st;
+      for (int l = 0; l < Np; ++l) {
+        const double coef = sD[mi * Np + l];
+        const int sn_ = base + l * st;
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) dref[c] += sS[el][c][sn_] * coef;
+        for (int c = 0; c < NQ; ++c) dref[c] += sS[el][c][sn_] * coef;
+      }
+#pragma unroll
+      for (int dd = 0; dd < DIM; ++dd) {
+        const double ad = sm.adj[el][dir * DIM + dd];
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) acc[dd][c] = dir == 0 ? dref[c] * ad : acc[dd][c] + dref[c] * ad;
+      }
     }
+    const NodeThermo<double, DIM> th = node_thermo<double, DIM>(g, q);
+
+    double dP[DIM][NQ];
 #pragma unroll
     for (int dd = 0; dd < DIM; ++dd) {
-      const double ad = adj_e[dir * DIM + dd];
+      double gr[NQ], lift[NQ];
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) acc[dd][c] = dir == 0 ? dref[c] * ad : acc[dd][c] + dref[c] * ad;
+      for (int c = 0; c < NQ; ++c) {
+        gr[c] = invJ * acc[dd][c];
+        lift[c] = 0.0;
+      }
+      // + w * (jump * n_dd) from every face through the node
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int t = face_index<DIM, Np>(d, i, j, k);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          if (idx[d] == (s ? Np - 1 : 0)) {
+            const int f = 2 * d + s;
+            const double nd = sm.nrm[el][f][dd];
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) lift[c] += wown * (sjump[el][f * NF + t][c] * nd);
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NQ; ++c) gr[c] += lift[c];
+      grad_entropy_to_grad_prim<double, DIM>(th, q, gr, dP[dd]);
     }
-  }
-  double q[NQ];
-#pragma unroll
-  for (int c = 0; c < NQ; ++c) q[c] = a.U[c * Ntot + ebase];
-  const NodeThermo<double, DIM> th = node_thermo<double, DIM>(g, q);
+    viscous_flux<double, DIM>(g, th, dP, vf);
 
-  double dP[DIM][NQ];
-#pragma unroll
-  for (int dd = 0; dd < DIM; ++dd) {
-    double gr[NQ], lift[NQ];
-#pragma unroll
-    for (int c = 0; c < NQ; ++c) {
-      gr[c] = invJ * acc[dd][c];
-      lift[c] = 0.0;
-    }
-    // + w * (jump * n_dd) from every face through the node
+    // n.Fv at the element's own face points, n the canonical area vector
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      const int t = face_index<DIM, Np>(d, i, j, k);
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         if (idx[d] == (s ? Np - 1 : 0)) {
           const int f = 2 * d + s;
-          const double nd = sn[el][f][dd];
+          const size_t gi = (size_t)e * NFACE + f * NF + face_index<DIM, Np>(d, i, j, k);
+          double n[DIM];
 #pragma unroll
-          for (int c = 0; c < NQ; ++c) lift[c] += wown * (sjump[el][f * NF + t][c] * nd);
+          for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[el][f][dd];
+#pragma unroll
+          for (int c = 0; c < NVD; ++c) {
+            double v = n[0] * vf[vis_index<DIM>(1 + c, 0)];
+#pragma unroll
+            for (int dd = 1; dd < DIM; ++dd) v += n[dd] * vf[vis_index<DIM>(1 + c, dd)];
+            a.Gf_out[c * Gtot + gi] = v;
+          }
         }
       }
     }
-#pragma unroll
-    for (int c = 0; c < NQ; ++c) gr[c] += lift[c];
-    grad_entropy_to_grad_prim<double, DIM>(th, q, gr, dP[dd]);
   }
-  double vf[NV];
-  viscous_flux<double, DIM>(g, th, dP, vf);
+  lds_sync();  // the entropy/jump buffers are reused below
+
+  // contravariant viscous fluxes adj_dir . Fv per node
+  if (valid) {
 #pragma unroll
-  for (int c = 0; c < NV; ++c) a.Fv_out[c * Ntot + ebase] = vf[c];
+    for (int dir = 0; dir < DIM; ++dir) {
+      double row[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) row[d] = sm.adj[el][dir * DIM + d];
+#pragma unroll
+      for (int c = 0; c < NVD; ++c) {
+        double fr = row[0] * vf[vis_index<DIM>(1 + c, 0)];
+#pragma unroll
+        for (int ph = 1; ph < DIM; ++ph) fr += row[ph] * vf[vis_index<DIM>(1 + c, ph)];
+        sG[el][dir][c][node] = fr;
+      }
+    }
+  }
+  lds_sync();
+  if (valid) {
+    double dv[NVD];
+#pragma unroll
+    for (int c = 0; c < NVD; ++c) dv[c] = 0.0;
+#pragma unroll
+    for (int dir = 0; dir < DIM; ++dir) {
+      const int mi = dir == 0 ? i : (dir == 1 ? j : k);
+      const int st = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
+      const int base = node - mi * st;
+#pragma unroll
+      for (int l = 0; l < Np; ++l) {
+        const double coef = sDh[mi * Np + l];
+#pragma unroll
+        for (int c = 0; c < NVD; ++c) dv[c] += coef * sG[el][dir][c][base + l * st];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NVD; ++c) a.Vd_out[c * Ntot + ebase] = invJ * dv[c];
+  }
 }
 
 // ---------------------------------------------------------------------------
