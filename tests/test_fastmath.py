@@ -39,4 +39,5 @@ def test_ln_pos_accuracy(ln_lib):
             assert yi == 0.0
             continue
         worst = max(worst, abs(float(mp.mpf(yi) - ex)) / np.spacing(abs(float(ex))))
+    print("worst ulp", worst)
     assert worst < 1.6, worst

@@ -594,7 +594,7 @@ __device__ __forceinline__ void surface_flux(const GasC<R> &g, const Pt<R, DIM> 
 // internal energy density, pressure.
 template <typename R, int DIM>
 struct NodeThermo {
-  R rho, ir, u[DIM], ke, ie, p;
+  R rho, ir, u[DIM], ke, ie, p, beta;  // beta = rho / p
 };
 
 template <typename R, int DIM>
@@ -611,6 +611,7 @@ __device__ __forceinline__ NodeThermo<R, DIM> node_thermo(const GasC<R> &g, cons
   t.ke = R(0.5) * m2 * t.ir;
   t.ie = q[DIM + 1] - t.ke;
   t.p = g.gammaM1 * t.ie;
+  t.beta = t.rho * drcp(t.p);
   return t;
 }
 
@@ -620,7 +621,7 @@ template <typename R, int DIM>
 __device__ __forceinline__ void entropy_state(const GasC<R> &g, const R *q, R *E) {
   const NodeThermo<R, DIM> t = node_thermo<R, DIM>(g, q);
   const R s = dlog(t.p) - g.gamma * dlog(t.rho);
-  const R beta = t.rho / t.p;
+  const R beta = t.beta;
   const R v2o2 = t.ke * t.ir;
   E[0] = (g.gamma - s) * g.gammaM1Inverse - beta * v2o2;
 #pragma unroll
@@ -639,7 +640,7 @@ __device__ __forceinline__ void grad_entropy_to_grad_prim(const NodeThermo<R, DI
     dP[1 + d] = p_over_rho * (dE[1 + d] + t.u[d] * dE[DIM + 1]);
     drho += q[1 + d] * dP[1 + d];
   }
-  drho = t.rho * dE[0] - dE[DIM + 1] * (t.ke - t.ie) + t.rho * drho / t.p;
+  drho = t.rho * dE[0] - dE[DIM + 1] * (t.ke - t.ie) + t.beta * drho;
   dP[0] = drho;
   dP[DIM + 1] = p_over_rho * (dP[0] + t.p * dE[DIM + 1]);
 }

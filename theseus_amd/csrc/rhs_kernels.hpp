@@ -323,7 +323,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   constexpr int NPL = Np * (Np - 1) / 2;      // node pairs per line
   constexpr int NPAIR = NLINE * NPL;          // pairs per direction
   constexpr int NI = (Np - 1) * NLINE;        // subcell interfaces per direction
-  constexpr int NS0 = NPAIR > NFACE ? NPAIR : NFACE;
+  constexpr int NS0 = 2 * NPAIR > NFACE ? 2 * NPAIR : NFACE;  // pair fluxes: two directions in flight
   constexpr int NS1 = NS0 > NI ? NS0 : NI;
   constexpr int NIND = 2 * ND + 3 * S::NPOW2;  // indicator sweeps + reduction
   constexpr int NSCR = (NIND > NS1 * NQ) ? (NIND + NQ - 1) / NQ : NS1;  // scratch rows
@@ -513,17 +513,26 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   }
 
   // ---- volume: split-form flux differencing, each pair evaluated once
-  //      (DGSEMIntegrator.hpp:124-218: dU_i = -(1/J) sum_dir sum_{m != i} Dhat2(i,m) F#(u_lo,u_hi))
+  //      (DGSEMIntegrator.hpp:124-218: dU_i = -(1/J) sum_dir sum_{m != i} Dhat2(i,m) F#(u_lo,u_hi)).
+  //      The pairs of all directions form one list processed in full
+  //      rounds; direction dir's fluxes go to scratch slot dir & 1 and are
+  //      gathered as soon as the round holding its last pair is done.
+  constexpr int NPT = EPB * NPAIR;                     // pairs of one direction
+  constexpr int NRND = (DIM * NPT + TPB - 1) / TPB;
+  static_assert(DIM < 3 || (2 * NPT) / TPB > (NPT - 1) / TPB, "slot reuse needs a round boundary");
   double rate[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) rate[q] = 0.0;
 #pragma unroll 1
-  for (int dir = 0; dir < DIM; ++dir) {
-    const int sdir = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
-    for (int w = tid; w < EPB * NPAIR; w += TPB) {
-      const int pe = w / NPAIR;
-      const int r = w - pe * NPAIR;
+  for (int rr = 0; rr < NRND; ++rr) {
+    const int w = rr * TPB + tid;
+    if (w < DIM * NPT) {
+      const int dir = w / NPT;
+      const int rem = w - dir * NPT;
+      const int pe = rem / NPAIR;
+      const int r = rem - pe * NPAIR;
       if (ebase_blk + pe < a.ne) {
+        const int sdir = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
         const int line = r / NPL;
         int pa, pb;
         pair_nodes<Np>(r - line * NPL, pa, pb);
@@ -537,23 +546,27 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
         double F[NQ];
         volume_flux<VOL, double, DIM>(g, lo, hi, met, F);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) sscr[pe][r][q] = F[q];
+        for (int q = 0; q < NQ; ++q) sscr[pe][(dir & 1) * NPAIR + r][q] = F[q];
       }
     }
-    lds_sync();
-    if (valid) {
-      const int mi = dir == 0 ? i : (dir == 1 ? j : k);
-      const int line = face_index<DIM, Np>(dir, i, j, k);
 #pragma unroll
-      for (int m = 0; m < Np; ++m) {
-        if (m == mi) continue;
-        const int pr = m < mi ? pair_index<Np>(m, mi) : pair_index<Np>(mi, m);
-        const double coef = sD2[mi * Np + m];
+    for (int dir = 0; dir < DIM; ++dir) {
+      if (((dir + 1) * NPT - 1) / TPB != rr) continue;
+      lds_sync();
+      if (valid) {
+        const int mi = dir == 0 ? i : (dir == 1 ? j : k);
+        const int line = face_index<DIM, Np>(dir, i, j, k);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr[el][line * NPL + pr][q];
+        for (int m = 0; m < Np; ++m) {
+          if (m == mi) continue;
+          const int pr = m < mi ? pair_index<Np>(m, mi) : pair_index<Np>(mi, m);
+          const double coef = sD2[mi * Np + m];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr[el][(dir & 1) * NPAIR + line * NPL + pr][q];
+        }
       }
+      lds_sync();
     }
-    lds_sync();
   }
   double du[NQ];
   {
