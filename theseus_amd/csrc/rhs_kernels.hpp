@@ -224,6 +224,16 @@ __device__ __forceinline__ void load_pt(const double (*sq)[ND], const double *sp
   for (int d = 0; d < DIM; ++d) o.v[d] = o.q[1 + d] * o.ir;
 }
 
+template <typename R, int DIM>
+__device__ __forceinline__ void pt_select(bool c, const Pt<R, DIM> &a, const Pt<R, DIM> &b, Pt<R, DIM> &o) {
+#pragma unroll
+  for (int q = 0; q < DIM + 2; ++q) o.q[q] = c ? a.q[q] : b.q[q];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) o.v[d] = c ? a.v[d] : b.v[d];
+  o.p = c ? a.p : b.p;
+  o.ir = c ? a.ir : b.ir;
+}
+
 // Per-element geometry staged in LDS: adjugate rows, the canonical area
 // vector of each face (the low-side element's adjugate row; the outward one
 // on a boundary) and the Jacobian determinant.
@@ -383,7 +393,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
 
   meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
   double qn[NFR][NQ];
-  double gn[NFR][VISC ? NVD : 1], go[NFR][VISC ? NVD : 1];
+  double gn[NFR][VISC ? NVD : 1];
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
     const int fi = min(tid + r * TPB, NFT - 1);
@@ -396,13 +406,10 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
     if constexpr (VISC) {
+      // the neighbour's n.Fv at this point (the own side's half is already in Vd)
       const size_t gb_n = (size_t)nbs * NFACE + (f ^ 1) * NF + t;
-      const size_t gb_o = (size_t)min(ebase_blk + pe, a.ne - 1) * NFACE + fl;
 #pragma unroll
-      for (int c = 0; c < NVD; ++c) {
-        gn[r][c] = a.Gf[c * Gtot + gb_n];
-        go[r][c] = a.Gf[c * Gtot + gb_o];
-      }
+      for (int c = 0; c < NVD; ++c) gn[r][c] = a.Gf[c * Gtot + gb_n];
     }
   }
   if (tid < Np * Np) {
@@ -432,75 +439,6 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   const int idx[3] = {i, j, k};
   constexpr int stride[3] = {1, Np, Np * Np};
   const double Je = sm.J[elc];
-
-  // ---- faces: Riemann flux at every point of the element's own faces
-  //      (EulerOperator_impl.hpp:211-392; NSOperator_impl.hpp:579-713),
-  //      neighbour traces from the registers loaded above
-  {
-    const double w0 = a.basis.w[0];
-#pragma unroll
-    for (int r = 0; r < NFR; ++r) {
-      const int fi = tid + r * TPB;
-      const int pe = fi / NFACE;
-      const int fl = fi - pe * NFACE;
-      if (fi < EPB * NFACE && ebase_blk + pe < a.ne) {
-        const int f = fl / NF, t = fl - f * NF;
-        const int d = f >> 1, s = f & 1;
-        Pt<double, DIM> own;
-        load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], face_node<DIM, Np>(d, s, t), own);
-        const double wown = 1.0 / (w0 * sm.J[pe]);
-        double n[DIM];
-#pragma unroll
-        for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[pe][f][dd];
-        double F[NQ], cs;
-        if (fnb[r] >= 0) {
-          Pt<double, DIM> oth;  // the neighbour's trace (loaded at the top)
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) oth.q[q] = qn[r][q];
-          derive(g, oth);
-          if (s) surface_flux<SURF, double, DIM>(g, own, oth, n, F);
-          else surface_flux<SURF, double, DIM>(g, oth, own, n, F);
-          if constexpr (VISC) {
-            // F* - 0.5 n.(Fv_L + Fv_R)   (DGSEMIntegrator.hpp:261-316)
-#pragma unroll
-            for (int c = 0; c < NVD; ++c) F[1 + c] -= s ? 0.5 * (go[r][c] + gn[r][c]) : 0.5 * (gn[r][c] + go[r][c]);
-          }
-          cs = s ? -wown : wown;
-        } else {
-          // boundary: supersonic outflow F*(q,q,n_out) (bc_kernels.hpp:236-238)
-          surface_flux<SURF, double, DIM>(g, own, own, n, F);
-          cs = -wown;
-        }
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) sscr[pe][fl][q] = cs * F[q];
-      }
-    }
-  }
-  lds_sync();
-  if (valid) {
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      const int t = face_index<DIM, Np>(d, i, j, k);
-      if (idx[d] == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) fsum[q] += sscr[el][(2 * d) * NF + t][q];
-      }
-      if (idx[d] == Np - 1) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) fsum[q] += sscr[el][(2 * d + 1) * NF + t][q];
-      }
-    }
-  }
-  lds_sync();
-
-  // RK base state: only read now, so that its loads overlap the volume work
-  double xb[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) xb[q] = 0.0;
-  if (a.mode == 2) {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) xb[q] = a.X[q * Ntot + ebase];
-  }
 
   // ---- blending coefficient (needs only the staged state)
   double alpha = 0.0;
@@ -622,6 +560,78 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
     }
   }
 
+  // RK base state (mode 2), needed by the epilogue; its loads overlap the
+  // face work.  Loaded unconditionally, from the stage state itself in the
+  // other modes (lines already in cache), so that waits stay exact.
+  double xb[NQ];
+  {
+    const double *Xp = a.mode == 2 ? a.X : a.U;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) xb[q] = Xp[q * Ntot + ebase];
+  }
+
+  // ---- faces: Riemann flux at every point of the element's own faces
+  //      (EulerOperator_impl.hpp:211-392; NSOperator_impl.hpp:579-713),
+  //      neighbour traces from the registers loaded at the top, which
+  //      arrived while the volume terms were computed
+  {
+    const double w0 = a.basis.w[0];
+#pragma unroll
+    for (int r = 0; r < NFR; ++r) {
+      const int fi = tid + r * TPB;
+      const int pe = fi / NFACE;
+      const int fl = fi - pe * NFACE;
+      if (fi < EPB * NFACE && ebase_blk + pe < a.ne) {
+        const int f = fl / NF, t = fl - f * NF;
+        const int d = f >> 1, s = f & 1;
+        Pt<double, DIM> own;
+        load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], face_node<DIM, Np>(d, s, t), own);
+        const double wown = 1.0 / (w0 * sm.J[pe]);
+        double n[DIM];
+#pragma unroll
+        for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[pe][f][dd];
+        // One flux evaluation per face point, operands put in canonical
+        // (low side first) order by selects, so that lanes on the two sides
+        // of the element do not diverge.  A boundary point (supersonic
+        // outflow, bc_kernels.hpp:236-238) evaluates F*(q, q, n_out).
+        const bool interior = fnb[r] >= 0;
+        Pt<double, DIM> oth;  // the neighbour's trace (loaded at the top)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) oth.q[q] = interior ? qn[r][q] : own.q[q];
+        derive(g, oth);
+        Pt<double, DIM> pl, pr;
+        pt_select(s != 0, own, oth, pl);
+        pt_select(s != 0, oth, own, pr);
+        double F[NQ];
+        surface_flux<SURF, double, DIM>(g, pl, pr, n, F);
+        if constexpr (VISC) {
+          // F* - 0.5 n.(Fv_L + Fv_R)   (DGSEMIntegrator.hpp:261-316); the own
+          // side's -0.5 n.Fv was folded into Vd by the gradient kernel
+#pragma unroll
+          for (int c = 0; c < NVD; ++c) F[1 + c] -= interior ? 0.5 * gn[r][c] : 0.0;
+        }
+        const double cs = (interior && !s) ? wown : -wown;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) sscr[pe][fl][q] = cs * F[q];
+      }
+    }
+  }
+  lds_sync();
+  if (valid) {
+    // a node lies on at most one face per direction (Np >= 2)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const bool hi = idx[d] == Np - 1;
+      const bool on = hi || idx[d] == 0;
+      const int row = (2 * d + (hi ? 1 : 0)) * NF + face_index<DIM, Np>(d, i, j, k);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const double v = sscr[el][row][q];
+        fsum[q] += on ? v : 0.0;
+      }
+    }
+  }
+
   // ---- epilogue: dU, or the fused SSP-RK3 stage combination
   if (valid) {
 #pragma unroll
@@ -642,10 +652,11 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
 // interior-face lift, conversion to primitive gradients and the physical
 // viscous flux (NSOperator_impl.hpp:10-577, DGSEMIntegrator.hpp:764-1044,
 // EOS.hpp:206-263, NavierStokesFlux.hpp:389-519).  It stores what the
-// second kernel consumes: the viscous volume divergence
-// Vd = (1/J) sum_dir sum_l Dhat(i,l) (adj_dir . Fv)(l)  (DGSEMIntegrator.hpp:486-622)
-// per node, and n.Fv at every point of the element's faces, n the face's
-// canonical area vector.
+// second kernel consumes: per node the viscous volume divergence
+// (1/J) sum_dir sum_l Dhat(i,l) (adj_dir . Fv)(l)  (DGSEMIntegrator.hpp:486-622)
+// plus the element's own half of the central viscous face flux,
+// -0.5 n.Fv lifted with the face weight, and n.Fv at every point of the
+// element's faces for the neighbours' halves (n the canonical area vector).
 // ---------------------------------------------------------------------------
 template <int DIM, int P>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
@@ -759,6 +770,9 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   const double wown = 1.0 / (a.basis.w[0] * Je);
 
   double vf[NV];
+  double hv[NVD];  // the own side's share -0.5 n.Fv of the central viscous face flux, lifted
+#pragma unroll
+  for (int c = 0; c < NVD; ++c) hv[c] = 0.0;
   if (valid) {
     // volume gradient (AssembleGradVolumePointKernel): acc[dd] = sum_ref (D.S)_ref adj(ref,dd),
     // one reference direction at a time, in the reference's ref = 0,1,2 order
@@ -787,53 +801,54 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
     }
     const NodeThermo<double, DIM> th = node_thermo<double, DIM>(g, q);
 
-    double dP[DIM][NQ];
+    // gradient = volume part / J + w * jump * n from the face through the
+    // node in each direction (a node lies on at most one face per direction)
 #pragma unroll
-    for (int dd = 0; dd < DIM; ++dd) {
-      double gr[NQ], lift[NQ];
+    for (int dd = 0; dd < DIM; ++dd)
+#pragma unroll
+      for (int c = 0; c < NQ; ++c) acc[dd][c] *= invJ;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const bool hi = idx[d] == Np - 1;
+      const bool on = hi || idx[d] == 0;
+      const int f = 2 * d + (hi ? 1 : 0);
+      const int row = f * NF + face_index<DIM, Np>(d, i, j, k);
+      double jw[NQ];
 #pragma unroll
       for (int c = 0; c < NQ; ++c) {
-        gr[c] = invJ * acc[dd][c];
-        lift[c] = 0.0;
-      }
-      // + w * (jump * n_dd) from every face through the node
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        const int t = face_index<DIM, Np>(d, i, j, k);
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          if (idx[d] == (s ? Np - 1 : 0)) {
-            const int f = 2 * d + s;
-            const double nd = sm.nrm[el][f][dd];
-#pragma unroll
-            for (int c = 0; c < NQ; ++c) lift[c] += wown * (sjump[el][f * NF + t][c] * nd);
-          }
-        }
+        const double v = wown * sjump[el][row][c];
+        jw[c] = on ? v : 0.0;
       }
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) gr[c] += lift[c];
-      grad_entropy_to_grad_prim<double, DIM>(th, q, gr, dP[dd]);
+      for (int dd = 0; dd < DIM; ++dd) {
+        const double nd = sm.nrm[el][f][dd];
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) acc[dd][c] += jw[c] * nd;
+      }
     }
+    double dP[DIM][NQ];
+#pragma unroll
+    for (int dd = 0; dd < DIM; ++dd) grad_entropy_to_grad_prim<double, DIM>(th, q, acc[dd], dP[dd]);
     viscous_flux<double, DIM>(g, th, dP, vf);
 
     // n.Fv at the element's own face points, n the canonical area vector
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
+      const bool hi = idx[d] == Np - 1;
+      if (hi || idx[d] == 0) {
+        const int f = 2 * d + (hi ? 1 : 0);
+        const size_t gi = (size_t)e * NFACE + f * NF + face_index<DIM, Np>(d, i, j, k);
+        double n[DIM];
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        if (idx[d] == (s ? Np - 1 : 0)) {
-          const int f = 2 * d + s;
-          const size_t gi = (size_t)e * NFACE + f * NF + face_index<DIM, Np>(d, i, j, k);
-          double n[DIM];
+        for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[el][f][dd];
 #pragma unroll
-          for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[el][f][dd];
+        for (int c = 0; c < NVD; ++c) {
+          double v = n[0] * vf[vis_index<DIM>(1 + c, 0)];
 #pragma unroll
-          for (int c = 0; c < NVD; ++c) {
-            double v = n[0] * vf[vis_index<DIM>(1 + c, 0)];
-#pragma unroll
-            for (int dd = 1; dd < DIM; ++dd) v += n[dd] * vf[vis_index<DIM>(1 + c, dd)];
-            a.Gf_out[c * Gtot + gi] = v;
-          }
+          for (int dd = 1; dd < DIM; ++dd) v += n[dd] * vf[vis_index<DIM>(1 + c, dd)];
+          a.Gf_out[c * Gtot + gi] = v;
+          // face term scale: -w on the element's high face, +w on its low face
+          hv[c] += (hi ? wown : -wown) * (0.5 * v);
         }
       }
     }
@@ -874,7 +889,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
       }
     }
 #pragma unroll
-    for (int c = 0; c < NVD; ++c) a.Vd_out[c * Ntot + ebase] = invJ * dv[c];
+    for (int c = 0; c < NVD; ++c) a.Vd_out[c * Ntot + ebase] = invJ * dv[c] + hv[c];
   }
 }
 
