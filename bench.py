@@ -28,26 +28,34 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(mesh_ne, order, dim, kind, stage_modes):
-    """Compulsory bytes per launch of one kernel kind, averaged over the
-    launches of a step.  Per node: the conservative state (dim+2 doubles),
-    the viscous volume divergence (dim+1), n.Fv at the element's face points
-    (dim+1 per face point, 2*dim*(p+1)^(dim-1) face points per element), the
-    output state and, for SSP-RK3 stages 2 and 3, the step's base state."""
+# Algorithmic bytes per DOF of one NS-BR1 FP64 RHS split over the two
+# passes, the two-pass minimum: pass 1 reads U (8) and writes the gradient
+# (24); pass 2 reads U (8) and the gradient (24) and writes dU (8).
+ALG_B_PER_DOF = {0: 40.0, 1: 32.0}
+
+
+def algorithmic_bytes(mesh_ne, order, dim, kind):
+    """Algorithmic bytes per launch of one kernel kind (pass 2 = 0, pass 1 = 1):
+    the per-DOF figure above times the DOFs one launch processes."""
+    dofs = mesh_ne * (order + 1) ** dim * (dim + 2)
+    return dofs * ALG_B_PER_DOF[kind], ALG_B_PER_DOF[kind]
+
+
+def moved_bytes_per_dof(order, dim, kind, stage_modes):
+    """This engine's own compulsory bytes per DOF, averaged over a step's
+    launches.  Per node: the conservative state (dim+2 doubles), the viscous
+    volume divergence (dim+1), n.Fv at the element's face points (dim+1 per
+    face point, 2*dim*(p+1)^(dim-1) face points per element), the output
+    state and, for SSP-RK3 stages 2 and 3, the step's base state."""
     ndof = (order + 1) ** dim
-    nodes = mesh_ne * ndof
     neq = dim + 2
     nvd = dim + 1
     nfp = 2 * dim * (order + 1) ** (dim - 1)
-    visc = nvd + nvd * nfp / ndof  # volume divergence + face normal fluxes, per node
-    if kind == 1:  # NS pass 1: read U, write both viscous arrays
-        per_node = 8 * (neq + visc)
-        return nodes * per_node, per_node / neq
-    per = []
-    for mode in stage_modes:  # pass 2: read U and the viscous arrays, write out (+ read X)
-        per.append(8 * (neq + visc + neq + (neq if mode == 2 else 0)))
-    per_node = sum(per) / len(per)
-    return nodes * per_node, per_node / neq
+    visc = nvd + nvd * nfp / ndof
+    if kind == 1:
+        return 8 * (neq + visc) / neq
+    per = [8 * (neq + visc + neq + (neq if mode == 2 else 0)) for mode in stage_modes]
+    return sum(per) / len(per) / neq
 
 
 def cpu_baseline(rt, mesh, U0, seconds):
@@ -114,47 +122,84 @@ def main():
                                    "not a measured path", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    import dataclasses
+
     import numpy as np
     import torch
 
+    from theseus_amd import _abi as A
     from theseus_amd.cases import c2_tgv3d
     from theseus_amd.operator import MakeRHSOperator
 
     rt, mesh, U0 = c2_tgv3d(n=args.n, order=args.order)
-    op = MakeRHSOperator(rt, mesh)
     dof = U0.size
-    u = torch.from_numpy(U0).cuda()
-    stab = op.EstimateStability(u)
+    op = MakeRHSOperator(rt, mesh)
+    stab = op.EstimateStability(torch.from_numpy(U0).cuda())
     dt = args.cfl / stab.TotalRate()
     log(f"workload: TGV {args.n}^3 p={args.order} NS, DOF={dof}, dt={dt:.4e} (CFL {args.cfl})")
 
-    t = 0.0
-    for _ in range(args.warmup):
-        t = op.Step(u, t, dt)
-    torch.cuda.synchronize()
-    op.enable_kernel_timing(3 * args.steps)
-    stream = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.steps):
-        t = op.Step(u, t, dt)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    if not torch.isfinite(u).all():
-        raise RuntimeError("non-finite state after the timed steps")
+    def timed_run(op):
+        """warmup + timed SSP-RK3 steps from U0; returns ms, per-kernel timings, final state."""
+        u = torch.from_numpy(U0).cuda()
+        t = 0.0
+        for _ in range(args.warmup):
+            t = op.Step(u, t, dt)
+        torch.cuda.synchronize()
+        op.enable_kernel_timing(3 * args.steps)
+        stream = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            t = op.Step(u, t, dt)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        if not torch.isfinite(u).all():
+            raise RuntimeError("non-finite state after the timed steps")
+        return ms, op.kernel_timing(0), op.kernel_timing(1), u
+
+    def per_field(a, b):
+        a = a.reshape(5, -1)
+        b = b.reshape(5, -1)
+        return [float(torch.linalg.norm(a[q] - b[q]) / torch.linalg.norm(b[q])) for q in range(5)]
+
+    ms, (k2_ms, k2_n), (k1_ms, k1_n), u64 = timed_run(op)
     value = dof * 3 * args.steps / (ms * 1e-3)
 
-    k2_ms, k2_n = op.kernel_timing(0)
-    k1_ms, k1_n = op.kernel_timing(1)
+    # mixed precision on the same workload, dt and step count
+    rtm = dataclasses.replace(rt, precision=A.PRECISION_MIXED)
+    opm = MakeRHSOperator(rtm, mesh)
+    msm, (m2_ms, _), (m1_ms, _), u32 = timed_run(opm)
+    value_m = dof * 3 * args.steps / (msm * 1e-3)
+    u0 = torch.from_numpy(U0).cuda()
+    r64 = torch.empty_like(u0)
+    r32 = torch.empty_like(u0)
+    op.Mult(u0, r64)
+    opm.Mult(u0, r32)
+    torch.cuda.synchronize()
+    mixed = {
+        "value": value_m,
+        "ms_per_step": msm / args.steps,
+        "speedup_vs_fp64": value_m / value,
+        "rhs_rel_l2_vs_fp64_per_field": per_field(r32, r64),
+        "state_rel_l2_vs_fp64_per_field": per_field(u32, u64),
+        "state_after_steps": args.warmup + args.steps,
+        "kernels_mean_ms": {"ns_pass2_volume_face": m2_ms, "ns_pass1_br1_gradient": m1_ms},
+        "arithmetic": "FP32 fluxes, gradients and LDS staging; FP64 state, stage combination and indicator",
+    }
+    log(f"mixed: {msm/args.steps:.3f} ms/step, {value_m:.4e} DOF*RHS/s ({value_m/value:.2f}x), "
+        f"RHS err {mixed['rhs_rel_l2_vs_fp64_per_field']}")
+
     modes = [1, 2, 2]
-    b2, b2_dof = algorithmic_bytes(mesh.num_elements, args.order, 3, 0, modes)
-    b1, b1_dof = algorithmic_bytes(mesh.num_elements, args.order, 3, 1, modes)
+    b2, b2_dof = algorithmic_bytes(mesh.num_elements, args.order, 3, 0)
+    b1, b1_dof = algorithmic_bytes(mesh.num_elements, args.order, 3, 1)
     kernels = {
-        "ns_pass2_volume_face": {"mean_ms": k2_ms, "launches": k2_n, "alg_bytes": b2, "alg_B_per_dof": b2_dof},
-        "ns_pass1_br1_gradient": {"mean_ms": k1_ms, "launches": k1_n, "alg_bytes": b1, "alg_B_per_dof": b1_dof},
+        "ns_pass2_volume_face": {"mean_ms": k2_ms, "launches": k2_n, "alg_bytes": b2, "alg_B_per_dof": b2_dof,
+                                 "moved_B_per_dof": moved_bytes_per_dof(args.order, 3, 0, modes)},
+        "ns_pass1_br1_gradient": {"mean_ms": k1_ms, "launches": k1_n, "alg_bytes": b1, "alg_B_per_dof": b1_dof,
+                                  "moved_B_per_dof": moved_bytes_per_dof(args.order, 3, 1, modes)},
     }
     dom_name = max(kernels, key=lambda k: kernels[k]["mean_ms"])
     dom = kernels[dom_name]
@@ -204,6 +249,7 @@ def main():
             "traffic": traffic,
         },
         "kernels": kernels,
+        "mixed": mixed,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")

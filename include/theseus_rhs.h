@@ -75,7 +75,11 @@ extern "C" {
 
 /* ---- precision modes ---------------------------------------------------- */
 #define THESEUS_PRECISION_FP64 0
-#define THESEUS_PRECISION_MIXED 1 /* FP32 fluxes, FP64 stage accumulation */
+#define THESEUS_PRECISION_MIXED 1 /* FP32 flux arithmetic, gradients and LDS staging; FP64 state, SSP-RK3
+                                     stage combination, blending indicator and stability estimate.  The
+                                     momentum fluxes leave out a background pressure (the pressure at the
+                                     first node of the first state the handle evaluates, fixed for the
+                                     handle's lifetime), formed in FP64 before rounding to FP32. */
 
 /*
  * Case descriptor.  Host pointers; copied into device memory by create, so

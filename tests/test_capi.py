@@ -49,7 +49,7 @@ def _desc(**over):
     ("num_elements", 0, A.ERR_INVALID_ARGUMENT),
     ("flow_model", 7, A.ERR_INVALID_ARGUMENT),
     ("gamma", 1.0, A.ERR_INVALID_ARGUMENT),
-    ("precision", A.PRECISION_MIXED, A.ERR_UNSUPPORTED),
+    ("precision", 5, A.ERR_UNSUPPORTED),
     ("volume_flux", 9, A.ERR_UNSUPPORTED),
 ])
 def test_create_rejects_invalid_descriptors(field, value, code):

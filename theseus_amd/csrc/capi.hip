@@ -190,6 +190,18 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
   if (t < n) p[t] = 0ull;
 }
 
+// Background pressure of the mixed mode: the pressure of the first node of
+// the state (one lane; a vector store).
+template <int DIM>
+__global__ void k_pref(const double *U, long long Ntot, GasC<double> g, double *out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  Pt<double, DIM> s;
+#pragma unroll
+  for (int q = 0; q < DIM + 2; ++q) s.q[q] = U[q * Ntot];
+  derive(g, s);
+  out[0] = s.p;
+}
+
 }  // namespace thx
 
 using namespace thx;
@@ -215,11 +227,15 @@ struct theseus_rhs {
   const ShapeKernels *sk = nullptr;
   BasisArgs basis;
   GasC<double> gas;
+  GasC<float> gas32;
+  int pm = 0;  // kernel variant: 0 FP64, 1 FP32 flux arithmetic
   double adv_scale, diff_scale, surf_scale;
   double *d_J = nullptr, *d_adj = nullptr;
   int *d_nbr = nullptr;
   double *d_alpha = nullptr;
   double *d_Fv = nullptr;       // NS: viscous volume divergence, then n.Fv at face points
+  double *d_pref = nullptr;     // mixed mode: background pressure (one double)
+  bool pref_set = false;
   double *d_stage[2] = {nullptr, nullptr};
   unsigned long long *d_rates = nullptr;
   bool has_boundary = false;
@@ -235,10 +251,13 @@ struct theseus_rhs {
     a.out = out;
     a.mode = 0;
     if (d_Fv) {
-      a.Vd = d_Fv;
-      a.Gf = d_Fv + (size_t)(d.dim + 1) * Ntot;
-      a.Vd_out = d_Fv;
-      a.Gf_out = d_Fv + (size_t)(d.dim + 1) * Ntot;
+      // Vd then Gf, in the kernels' compute type
+      const size_t esz = pm ? sizeof(float) : sizeof(double);
+      char *base = reinterpret_cast<char *>(d_Fv);
+      a.Vd = base;
+      a.Gf = base + esz * (size_t)(d.dim + 1) * Ntot;
+      a.Vd_out = base;
+      a.Gf_out = base + esz * (size_t)(d.dim + 1) * Ntot;
     }
     a.alpha = d_alpha;
     a.Ntot = Ntot;
@@ -252,6 +271,8 @@ struct theseus_rhs {
     a.adj = d_adj;
     a.nbr = d_nbr;
     a.gas = gas;
+    a.gas32 = gas32;
+    a.pref = d_pref;
     a.basis = basis;
     return a;
   }
@@ -272,11 +293,17 @@ static void timed(theseus_rhs *h, int kind, hipStream_t s, F &&launch) {
 
 static int enqueue_rhs(theseus_rhs *h, RHSArgs a, hipStream_t s) {
   const int nb = h->nblocks();
+  if (h->pm && !h->pref_set) {
+    // the mixed mode's background pressure, fixed from the first state evaluated
+    if (h->d.dim == 2) hipLaunchKernelGGL(k_pref<2>, dim3(1), dim3(64), 0, s, a.U, h->Ntot, h->gas, h->d_pref);
+    else hipLaunchKernelGGL(k_pref<3>, dim3(1), dim3(64), 0, s, a.U, h->Ntot, h->gas, h->d_pref);
+    h->pref_set = true;
+  }
   if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
-    timed(h, 1, s, [&] { h->sk->ns_grad(a, nb, s); });
-    timed(h, 0, s, [&] { h->fk->ns(a, nb, s); });
+    timed(h, 1, s, [&] { h->sk->ns_grad[h->pm](a, nb, s); });
+    timed(h, 0, s, [&] { h->fk->ns[h->pm](a, nb, s); });
   } else {
-    timed(h, 0, s, [&] { h->fk->euler(a, nb, s); });
+    timed(h, 0, s, [&] { h->fk->euler[h->pm](a, nb, s); });
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(THESEUS_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -305,8 +332,8 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
     return fail(THESEUS_ERR_INVALID_ARGUMENT, "geometry and connectivity arrays are required");
   if (d.flow_model != THESEUS_FLOW_EULER && d.flow_model != THESEUS_FLOW_NAVIER_STOKES)
     return fail(THESEUS_ERR_INVALID_ARGUMENT, "unknown flow_model");
-  if (d.precision != THESEUS_PRECISION_FP64)
-    return fail(THESEUS_ERR_UNSUPPORTED, "only THESEUS_PRECISION_FP64 is available in this build");
+  if (d.precision != THESEUS_PRECISION_FP64 && d.precision != THESEUS_PRECISION_MIXED)
+    return fail(THESEUS_ERR_UNSUPPORTED, "precision must be THESEUS_PRECISION_FP64 or THESEUS_PRECISION_MIXED");
   if (!(d.gamma > 1.0) || !std::isfinite(d.gamma)) return fail(THESEUS_ERR_INVALID_ARGUMENT, "gamma must be > 1");
   if (d.flow_model == THESEUS_FLOW_NAVIER_STOKES && (!(d.Pr > 0) || !(d.mu >= 0)))
     return fail(THESEUS_ERR_INVALID_ARGUMENT, "Navier-Stokes needs Pr > 0 and mu >= 0");
@@ -370,6 +397,15 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
   h->gas.mu = d.mu;
   h->gas.mu_bulk = d.mu_bulk;
   h->gas.R_gas = d.R_gas;
+  h->gas32.gamma = (float)h->gas.gamma;
+  h->gas32.gammaM1 = (float)h->gas.gammaM1;
+  h->gas32.gammaM1Inverse = (float)h->gas.gammaM1Inverse;
+  h->gas32.cp = (float)h->gas.cp;
+  h->gas32.PrInverse = (float)h->gas.PrInverse;
+  h->gas32.mu = (float)h->gas.mu;
+  h->gas32.mu_bulk = (float)h->gas.mu_bulk;
+  h->gas32.R_gas = (float)h->gas.R_gas;
+  h->pm = d.precision == THESEUS_PRECISION_MIXED ? 1 : 0;
   h->adv_scale = advection_scale(d.order);
   h->diff_scale = diffusion_scale(d.order);
   h->surf_scale = h->adv_scale / (1.0 / w[0]);
@@ -379,7 +415,7 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
   if ((rc = upload(&h->d_J, d.elem_jac, ne)) || (rc = upload(&h->d_adj, d.elem_adj, (size_t)ne * dim * dim)) ||
       (rc = upload(&h->d_nbr, d.face_neighbor, (size_t)ne * 2 * dim)) ||
       (rc = upload<double>(&h->d_alpha, nullptr, ne)) ||
-      (rc = upload<unsigned long long>(&h->d_rates, nullptr, 4)) ||
+      (rc = upload<unsigned long long>(&h->d_rates, nullptr, 4)) || (rc = upload<double>(&h->d_pref, nullptr, 1)) ||
       (rc = upload<double>(&h->d_stage[0], nullptr, M)) || (rc = upload<double>(&h->d_stage[1], nullptr, M))) {
     theseus_rhs_destroy(h);
     return rc;
@@ -445,6 +481,7 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_nbr);
   hipFree(h->d_alpha);
   hipFree(h->d_Fv);
+  hipFree(h->d_pref);
   hipFree(h->d_stage[0]);
   hipFree(h->d_stage[1]);
   hipFree(h->d_rates);

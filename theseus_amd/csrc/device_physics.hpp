@@ -28,7 +28,16 @@ struct Pt {
   R v[DIM];      // velocity m/rho
   R p;           // pressure
   R ir;          // 1/rho
+  R pd;          // pressure minus a constant background, for the momentum flux's pressure term
 };
+
+// The momentum fluxes below take their pressure term from Pt::pd.  A
+// constant background pressure contributes nothing to the divergence (the
+// discretisation preserves free streams), so it may be left out of every
+// momentum flux; in FP64 pd == p (no background), in the mixed mode pd is
+// p - p_ref formed in FP64 before rounding to FP32, which keeps the O(1)
+// pressure fluctuation of a low-Mach flow from drowning in the rounding of
+// a pressure 1/(gamma Ma^2) times larger.
 
 template <typename R>
 __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
@@ -139,6 +148,7 @@ __device__ __forceinline__ void derive(const GasC<R> &g, Pt<R, DIM> &s) {
   }
   const R ke = R(0.5) * m2 * s.ir;
   s.p = g.gammaM1 * (s.q[DIM + 1] - ke);
+  s.pd = s.p;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) s.v[d] = s.q[1 + d] * s.ir;
 }
@@ -151,6 +161,19 @@ __device__ __forceinline__ R sound_speed(const GasC<R> &g, const Pt<R, DIM> &s) 
 // ---------------------------------------------------------------------------
 // Two-point volume fluxes along the (mean) metric row `met`.
 // ---------------------------------------------------------------------------
+
+// Pressure term of Chandrashekar's momentum flux, p_hat = (rho1+rho2)/(2(beta1+beta2))
+// = (beta1 p1 + beta2 p2)/(beta1 + beta2) with beta = rho/(2p); the second
+// form carries a background shift through unchanged, so the mixed mode
+// evaluates it on pd.  FP64 keeps the reference's expression.
+template <typename R, int DIM>
+__device__ __forceinline__ R chandrashekar_p_mom(const Pt<R, DIM> &a, const Pt<R, DIM> &b, R beta1, R beta2, R p_hat) {
+  if constexpr (sizeof(R) == sizeof(double)) {
+    return p_hat;
+  } else {
+    return (beta1 * a.pd + beta2 * b.pd) / (beta1 + beta2);
+  }
+}
 
 // ChandrashekarFlux.hpp:20-89
 template <typename R, int DIM>
@@ -176,8 +199,9 @@ __device__ __forceinline__ void vol_chandrashekar(const GasC<R> &g, const Pt<R, 
   const R gm1_av_inv = g.gammaM1Inverse;  // 2/(gamma1 + gamma2 - 2) with one gamma
   h_hat += R(0.5) * inv_beta_ln * gm1_av_inv + p_hat * lr.inv;
   F[0] = rho_ln * vn;
+  const R p_mom = chandrashekar_p_mom(a, b, beta1, beta2, p_hat);
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) F[1 + d] = vn * mom_hat[d] + p_hat * met[d];
+  for (int d = 0; d < DIM; ++d) F[1 + d] = vn * mom_hat[d] + p_mom * met[d];
   F[DIM + 1] = rho_ln * vn * h_hat;
 }
 
@@ -188,7 +212,7 @@ __device__ __forceinline__ void vol_ranocha(const GasC<R> &g, const Pt<R, DIM> &
   const R rho1 = a.q[0], rho2 = b.q[0];
   const R rho_ln = log_mean2<R, true, false>(rho1, rho2).lm;
   const R p1 = a.p, p2 = b.p;
-  const R pbar = R(0.5) * (p1 + p2);
+  const R pdbar = R(0.5) * (a.pd + b.pd);
   R vbar[DIM], ppm[DIM];
   R ke_flux = R(0), vn = R(0);
 #pragma unroll
@@ -205,7 +229,7 @@ __device__ __forceinline__ void vol_ranocha(const GasC<R> &g, const Pt<R, DIM> &
   R fe = R(0);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    F[1 + d] = rho_ln * vn * vbar[d] + pbar * met[d];
+    F[1 + d] = rho_ln * vn * vbar[d] + pdbar * met[d];
     fe += ppm[d] * met[d];
   }
   F[0] = rho_ln * vn;
@@ -219,7 +243,6 @@ __device__ __forceinline__ void vol_pirozzoli(const GasC<R> &g, const Pt<R, DIM>
   const R rho1 = a.q[0], rho2 = b.q[0];
   const R rho_mean = R(0.5) * (rho1 + rho2);
   const R p1 = a.p, p2 = b.p;
-  const R pbar = R(0.5) * (p1 + p2);
   R vbar[DIM];
   R ke_mean = R(0), vn = R(0);
 #pragma unroll
@@ -241,7 +264,7 @@ __device__ __forceinline__ void vol_pirozzoli(const GasC<R> &g, const Pt<R, DIM>
   const R h_mean = R(0.5) * (e1 + p1 * a.ir + e2 + p2 * b.ir);
   F[0] = rho_mean * vn;
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) F[1 + d] = rho_mean * vn * vbar[d] + pbar * met[d];
+  for (int d = 0; d < DIM; ++d) F[1 + d] = rho_mean * vn * vbar[d] + R(0.5) * (a.pd + b.pd) * met[d];
   F[DIM + 1] = rho_mean * (h_mean + ke_mean) * vn;
 }
 
@@ -263,8 +286,8 @@ __device__ __forceinline__ void vol_central(const GasC<R> &g, const Pt<R, DIM> &
     for (int i = 0; i < DIM; ++i) {
       R f1 = a.q[1 + i] * m1d * sv1, f2 = b.q[1 + i] * m2d * sv2;
       if (i == d) {
-        f1 += a.p;
-        f2 += b.p;
+        f1 += a.pd;
+        f2 += b.pd;
       }
       f[1 + i] += R(0.5) * (f1 + f2) * met[d];
     }
@@ -292,7 +315,7 @@ __device__ __forceinline__ void normal_flux(const Pt<R, DIM> &s, const R *n, R *
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
       R pf = s.q[1 + c] * m * inv_rho;
-      if (c == d) pf += s.p;
+      if (c == d) pf += s.pd;
       fn[1 + c] += pf * n[d];
     }
     fn[DIM + 1] += m * H * n[d];
@@ -428,8 +451,9 @@ __device__ __forceinline__ void face_chandrashekar(const GasC<R> &g, const Pt<R,
   R diss[DIM + 2];
   scalar_dissipation(g, a, b, nor, diss);
   F[0] = rho_ln * vn - diss[0];
+  const R p_mom = chandrashekar_p_mom(a, b, beta1, beta2, p_hat);
 #pragma unroll
-  for (int d = 0; d < DIM; d++) F[1 + d] = vn * mom[d] + p_hat * nor[d] - diss[1 + d];
+  for (int d = 0; d < DIM; d++) F[1 + d] = vn * mom[d] + p_mom * nor[d] - diss[1 + d];
   F[DIM + 1] = rho_ln * vn * hhat - diss[DIM + 1];
 }
 

@@ -13,13 +13,14 @@ namespace thx {
 using RHSLaunch = void (*)(const RHSArgs &, int nblocks, hipStream_t);
 using StabLaunch = void (*)(const StabArgs &, int nblocks, hipStream_t);
 
+// [0]: FP64 arithmetic; [1]: FP32 flux arithmetic (mixed precision)
 struct FluxKernels {
-  RHSLaunch euler = nullptr;  // fused volume + face
-  RHSLaunch ns = nullptr;     // Navier-Stokes pass 2
+  RHSLaunch euler[2] = {nullptr, nullptr};  // fused volume + face
+  RHSLaunch ns[2] = {nullptr, nullptr};     // Navier-Stokes pass 2
 };
 
 struct ShapeKernels {
-  RHSLaunch ns_grad = nullptr;  // Navier-Stokes pass 1
+  RHSLaunch ns_grad[2] = {nullptr, nullptr};  // Navier-Stokes pass 1
   StabLaunch stability = nullptr;
   int tpb = 0, epb = 0;
 };
@@ -29,13 +30,13 @@ void register_shape(int dim, int p, ShapeKernels k);
 const FluxKernels *find_flux(int dim, int p, int vol, int surf);
 const ShapeKernels *find_shape(int dim, int p);
 
-template <int DIM, int P, int VOL, int SURF, bool VISC>
+template <typename R, int DIM, int P, int VOL, int SURF, bool VISC>
 void launch_rhs(const RHSArgs &a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_rhs<DIM, P, VOL, SURF, VISC>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+  hipLaunchKernelGGL((k_rhs<R, DIM, P, VOL, SURF, VISC>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
 }
-template <int DIM, int P>
+template <typename R, int DIM, int P>
 void launch_ns_grad(const RHSArgs &a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_ns_grad<DIM, P>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+  hipLaunchKernelGGL((k_ns_grad<R, DIM, P>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
 }
 template <int DIM, int P>
 void launch_stability(const StabArgs &a, int nb, hipStream_t s) {
@@ -46,8 +47,10 @@ template <int DIM, int VOL, int SURF, int P>
 struct FluxRegistrar {
   FluxRegistrar() {
     FluxKernels k;
-    k.euler = &launch_rhs<DIM, P, VOL, SURF, false>;
-    k.ns = &launch_rhs<DIM, P, VOL, SURF, true>;
+    k.euler[0] = &launch_rhs<double, DIM, P, VOL, SURF, false>;
+    k.ns[0] = &launch_rhs<double, DIM, P, VOL, SURF, true>;
+    k.euler[1] = &launch_rhs<float, DIM, P, VOL, SURF, false>;
+    k.ns[1] = &launch_rhs<float, DIM, P, VOL, SURF, true>;
     register_flux(DIM, P, VOL, SURF, k);
   }
 };
@@ -56,7 +59,8 @@ template <int DIM, int P>
 struct ShapeRegistrar {
   ShapeRegistrar() {
     ShapeKernels k;
-    k.ns_grad = &launch_ns_grad<DIM, P>;
+    k.ns_grad[0] = &launch_ns_grad<double, DIM, P>;
+    k.ns_grad[1] = &launch_ns_grad<float, DIM, P>;
     k.stability = &launch_stability<DIM, P>;
     k.tpb = Shape<DIM, P>::TPB;
     k.epb = Shape<DIM, P>::EPB;

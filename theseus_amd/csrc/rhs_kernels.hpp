@@ -54,10 +54,10 @@ struct RHSArgs {
   const double *X;    // RK base state for mode 2
   double cx, cy, dt;
   int mode;
-  const double *Vd;   // NS: viscous volume divergence, [DIM+1][Ntot] (momentum, energy)
-  const double *Gf;   // NS: n.Fv at face points, [DIM+1][ne][2*DIM][NFP]
-  double *Vd_out;     // NS pass 1 outputs
-  double *Gf_out;
+  const void *Vd;     // NS: viscous volume divergence, [DIM+1][Ntot] (momentum, energy)
+  const void *Gf;     // NS: n.Fv at face points, [DIM+1][ne][2*DIM][NFP]
+  void *Vd_out;       // NS pass 1 outputs; element type = the kernels' compute type
+  void *Gf_out;
   double *alpha;      // blending coefficient per element (written when blending)
   long long Ntot;     // ne * NDOF
   int ne;
@@ -67,8 +67,17 @@ struct RHSArgs {
   const double *adj;    // [ne][DIM][DIM]
   const int *nbr;       // [ne][2*DIM]
   GasC<double> gas;
+  GasC<float> gas32;  // the same constants rounded to FP32 (mixed precision)
+  const double *pref; // mixed precision: background pressure left out of the momentum fluxes (device scalar)
   BasisArgs basis;
 };
+
+template <typename R>
+__device__ __forceinline__ const GasC<R> &gas_of(const RHSArgs &a);
+template <>
+__device__ __forceinline__ const GasC<double> &gas_of<double>(const RHSArgs &a) { return a.gas; }
+template <>
+__device__ __forceinline__ const GasC<float> &gas_of<float>(const RHSArgs &a) { return a.gas32; }
 
 // Node index of in-face point t on local face (d, side)
 template <int DIM, int Np>
@@ -112,15 +121,17 @@ __device__ __forceinline__ int logical_block() {
 // alpha_max.  The reference applies the inverse of the full Legendre
 // Vandermonde (ModalBasis.cpp:115-140); that matrix is the Kronecker
 // product of the 1-D inverse, applied here as one 1-D sweep per direction.
-// `buf` holds at least 2*ND + 3*NPOW2 doubles of the element's scratch.
+// `buf` holds at least 2*ND + 3*NPOW2 doubles of the element's scratch;
+// `srp_el` is the element's pressure*density per node, in FP64 whatever the
+// precision of the flux arithmetic.
 template <int DIM, int P>
 __device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, const double *sVi,
-                                                 const double *sp_el, const double *srho_el, int node, int i, int j,
-                                                 int k, bool lane_ok, bool valid) {
+                                                 const double *srp_el, int node, int i, int j, int k, bool lane_ok,
+                                                 bool valid) {
   using S = Shape<DIM, P>;
   constexpr int Np = S::Np, ND = S::NDOF, NP2 = S::NPOW2;
   double *A = buf, *B = buf + ND;
-  if (lane_ok) A[node] = valid ? sp_el[node] * srho_el[node] : 0.0;
+  if (lane_ok) A[node] = valid ? srp_el[node] : 0.0;
   lds_sync();
   // sweep along x: B(a,j,k) = sum_l Vinv(a,l) A(l,j,k)
   if (lane_ok) {
@@ -214,12 +225,13 @@ __device__ __forceinline__ int vis_index(int q, int dd) {
 // A node's point sample from the staged state: velocity is m * (1/rho),
 // the same product derive() forms, so staged and fresh samples agree bitwise.
 template <typename R, int DIM, int ND, int NQ>
-__device__ __forceinline__ void load_pt(const double (*sq)[ND], const double *sp, const double *sir, int n,
+__device__ __forceinline__ void load_pt(const R (*sq)[ND], const R *sp, const R *sir, const R *spd, int n,
                                         Pt<R, DIM> &o) {
 #pragma unroll
   for (int q = 0; q < NQ; ++q) o.q[q] = sq[q][n];
   o.p = sp[n];
   o.ir = sir[n];
+  o.pd = spd[n];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) o.v[d] = o.q[1 + d] * o.ir;
 }
@@ -232,6 +244,7 @@ __device__ __forceinline__ void pt_select(bool c, const Pt<R, DIM> &a, const Pt<
   for (int d = 0; d < DIM; ++d) o.v[d] = c ? a.v[d] : b.v[d];
   o.p = c ? a.p : b.p;
   o.ir = c ? a.ir : b.ir;
+  o.pd = c ? a.pd : b.pd;
 }
 
 // Per-element geometry staged in LDS: adjugate rows, the canonical area
@@ -322,32 +335,58 @@ __device__ __forceinline__ void meta_store(const MetaRegs<DIM, EPB, TPB> &m, Ele
   }
 }
 
-template <int DIM, int P, int VOL, int SURF, bool VISC>
+// Scratch rows of one element in k_rhs: the larger of the flux buffers (in
+// the compute type R) and the indicator's FP64 sweeps, in doubles.
+template <typename R, int DIM, int P>
+struct RhsScratch {
+  using S = Shape<DIM, P>;
+  static constexpr int Np = S::Np, ND = S::NDOF, NQ = S::NEQ;
+  static constexpr int NF = S::NFP;
+  static constexpr int NFACE = 2 * DIM * NF;
+  static constexpr int NLINE = ND / Np;
+  static constexpr int NPL = Np * (Np - 1) / 2;
+  static constexpr int NPAIR = NLINE * NPL;
+  static constexpr int NI = (Np - 1) * NLINE;
+  static constexpr int NS0 = 2 * NPAIR > NFACE ? 2 * NPAIR : NFACE;  // pair fluxes: two directions in flight
+  static constexpr int ROWS = NS0 > NI ? NS0 : NI;
+  static constexpr int FLUX_D = (ROWS * NQ * (int)sizeof(R) + 7) / 8;
+  static constexpr int IND_D = 2 * ND + 3 * S::NPOW2;  // indicator sweeps + reduction
+  static constexpr int DOUBLES = FLUX_D > IND_D ? FLUX_D : IND_D;
+};
+
+// Fused volume + face kernel: the Euler right-hand side, or the second
+// Navier-Stokes pass when VISC.  R is the type of the flux arithmetic and of
+// the LDS staging (double, or float in the mixed mode); the state, the RK
+// combination and the blending indicator are FP64 in both.
+template <typename R, int DIM, int P, int VOL, int SURF, bool VISC>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   using S = Shape<DIM, P>;
+  using SC = RhsScratch<R, DIM, P>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NVD = DIM + 1;                // viscous terms: momentum and energy
   constexpr int NFACE = 2 * DIM * NF;
   constexpr int NFR = (EPB * NFACE + TPB - 1) / TPB;  // face-point rounds per lane
-  constexpr int NLINE = ND / Np;              // GLL lines per direction
-  constexpr int NPL = Np * (Np - 1) / 2;      // node pairs per line
-  constexpr int NPAIR = NLINE * NPL;          // pairs per direction
-  constexpr int NI = (Np - 1) * NLINE;        // subcell interfaces per direction
-  constexpr int NS0 = 2 * NPAIR > NFACE ? 2 * NPAIR : NFACE;  // pair fluxes: two directions in flight
-  constexpr int NS1 = NS0 > NI ? NS0 : NI;
-  constexpr int NIND = 2 * ND + 3 * S::NPOW2;  // indicator sweeps + reduction
-  constexpr int NSCR = (NIND > NS1 * NQ) ? (NIND + NQ - 1) / NQ : NS1;  // scratch rows
+  constexpr int NPL = SC::NPL, NPAIR = SC::NPAIR, NI = SC::NI;
+  constexpr bool FP64 = sizeof(R) == sizeof(double);
 
-  __shared__ double sq[EPB][NQ][ND];
-  __shared__ double sp[EPB][ND];
-  __shared__ double sir[EPB][ND];
-  __shared__ double sscr[EPB][NSCR][NQ];  // face terms / pair fluxes / FV fluxes / indicator sums
-  __shared__ double sD2[Np * Np];
+  __shared__ R sq[EPB][NQ][ND];
+  __shared__ R sp[EPB][ND];
+  __shared__ R sir[EPB][ND];
+  __shared__ R spd_[FP64 ? 1 : EPB][FP64 ? 1 : ND];  // p - p_ref (mixed precision)
+  __shared__ double sraw[EPB][SC::DOUBLES];  // flux scratch (as R) or indicator sweeps (FP64)
+  __shared__ double srp[EPB][ND];            // pressure * density, FP64, for the indicator
+  __shared__ R sD2[Np * Np];
   __shared__ double sVi[Np * Np];
   __shared__ ElemMeta<DIM, EPB> sm;
   __shared__ int sblend;
+  auto sscr = [&](int pe, int row, int q) -> R & { return reinterpret_cast<R *>(sraw[pe])[row * NQ + q]; };
+  // pressure for the momentum fluxes: p itself in FP64, p - p_ref in the mixed mode
+  auto spd = [&](int pe) -> const R * {
+    if constexpr (FP64) return sp[pe];
+    else return spd_[pe];
+  };
 
-  const GasC<double> &g = a.gas;
+  const GasC<R> &g = gas_of<R>(a);
   const int tid = threadIdx.x;
   const int el = tid / ND;
   const int node = tid - el * ND;
@@ -358,6 +397,9 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   const bool valid = lane_ok && e < a.ne;
   const long long Ntot = a.Ntot;
   const long long Gtot = (long long)a.ne * NFACE;
+  const R *Vd = static_cast<const R *>(a.Vd);
+  const R *Gf = static_cast<const R *>(a.Gf);
+  const double pref = FP64 ? 0.0 : *a.pref;
 
   // ---- global loads, all before the first barrier and unconditional:
   //      neighbour indices first, then the element's own data, then what
@@ -376,16 +418,17 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   meta_load_own<DIM, EPB, TPB>(a, ebase_blk, mr);
 
   const size_t ebase = (size_t)(valid ? e : 0) * ND + node;
-  double qown[NQ], fsum[NQ];
+  double qown[NQ];
+  R fsum[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     qown[q] = a.U[q * Ntot + ebase];
-    fsum[q] = 0.0;
+    fsum[q] = R(0);
   }
   if constexpr (VISC) {
-    // viscous volume divergence from the gradient kernel
+    // viscous volume divergence and the own half of the face term, from the gradient kernel
 #pragma unroll
-    for (int c = 0; c < NVD; ++c) fsum[1 + c] = a.Vd[c * Ntot + ebase];
+    for (int c = 0; c < NVD; ++c) fsum[1 + c] = Vd[c * Ntot + ebase];
   }
   static_assert(Np * Np <= TPB, "one basis entry per lane");
   const int kb = min(tid, Np * Np - 1);
@@ -393,7 +436,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
 
   meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
   double qn[NFR][NQ];
-  double gn[NFR][VISC ? NVD : 1];
+  R gn[NFR][VISC ? NVD : 1];
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
     const int fi = min(tid + r * TPB, NFT - 1);
@@ -409,11 +452,11 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
       // the neighbour's n.Fv at this point (the own side's half is already in Vd)
       const size_t gb_n = (size_t)nbs * NFACE + (f ^ 1) * NF + t;
 #pragma unroll
-      for (int c = 0; c < NVD; ++c) gn[r][c] = a.Gf[c * Gtot + gb_n];
+      for (int c = 0; c < NVD; ++c) gn[r][c] = Gf[c * Gtot + gb_n];
     }
   }
   if (tid < Np * Np) {
-    sD2[tid] = bD2;
+    sD2[tid] = R(bD2);
     sVi[tid] = bVi;
   }
 
@@ -421,15 +464,27 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   meta_store<DIM, EPB, TPB>(mr, sm);
   if (tid == 0) sblend = 0;
   if (valid) {
-    Pt<double, DIM> me;
+    Pt<R, DIM> me;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      me.q[q] = qown[q];
-      sq[el][q][node] = qown[q];
+      me.q[q] = R(qown[q]);
+      sq[el][q][node] = me.q[q];
     }
     derive(g, me);
-    sp[el][node] = me.p;
     sir[el][node] = me.ir;
+    if constexpr (FP64) {
+      sp[el][node] = me.p;
+      if (a.blending) srp[el][node] = me.p * me.q[0];
+    } else {
+      // pressure from the FP64 state: p rounded once, p - p_ref formed before rounding
+      Pt<double, DIM> m64;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) m64.q[q] = qown[q];
+      derive(a.gas, m64);
+      sp[el][node] = R(m64.p);
+      spd_[el][node] = R(m64.p - pref);
+      if (a.blending) srp[el][node] = m64.p * m64.q[0];
+    }
   }
   lds_sync();
 
@@ -440,11 +495,10 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   constexpr int stride[3] = {1, Np, Np * Np};
   const double Je = sm.J[elc];
 
-  // ---- blending coefficient (needs only the staged state)
+  // ---- blending coefficient (needs only the staged state; FP64)
   double alpha = 0.0;
   if (a.blending) {
-    alpha = blending_alpha<DIM, P>(a, &sscr[elc][0][0], sVi, &sp[elc][0], &sq[elc][0][0], node, i, j, k, lane_ok,
-                                   valid);
+    alpha = blending_alpha<DIM, P>(a, sraw[elc], sVi, srp[elc], node, i, j, k, lane_ok, valid);
     if (valid && node == 0) a.alpha[e] = alpha;
     if (valid && alpha > 1e-16) sblend = 1;
     lds_sync();
@@ -458,9 +512,9 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   constexpr int NPT = EPB * NPAIR;                     // pairs of one direction
   constexpr int NRND = (DIM * NPT + TPB - 1) / TPB;
   static_assert(DIM < 3 || (2 * NPT) / TPB > (NPT - 1) / TPB, "slot reuse needs a round boundary");
-  double rate[NQ];
+  R rate[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) rate[q] = 0.0;
+  for (int q = 0; q < NQ; ++q) rate[q] = R(0);
 #pragma unroll 1
   for (int rr = 0; rr < NRND; ++rr) {
     const int w = rr * TPB + tid;
@@ -475,16 +529,16 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
         int pa, pb;
         pair_nodes<Np>(r - line * NPL, pa, pb);
         const int base = face_node<DIM, Np>(dir, 0, line);
-        Pt<double, DIM> lo, hi;
-        load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], base + pa * sdir, lo);
-        load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], base + pb * sdir, hi);
-        double met[DIM];
+        Pt<R, DIM> lo, hi;
+        load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), base + pa * sdir, lo);
+        load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), base + pb * sdir, hi);
+        R met[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) met[d] = sm.adj[pe][dir * DIM + d];
-        double F[NQ];
-        volume_flux<VOL, double, DIM>(g, lo, hi, met, F);
+        for (int d = 0; d < DIM; ++d) met[d] = R(sm.adj[pe][dir * DIM + d]);
+        R F[NQ];
+        volume_flux<VOL, R, DIM>(g, lo, hi, met, F);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) sscr[pe][(dir & 1) * NPAIR + r][q] = F[q];
+        for (int q = 0; q < NQ; ++q) sscr(pe, (dir & 1) * NPAIR + r, q) = F[q];
       }
     }
 #pragma unroll
@@ -498,26 +552,26 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
         for (int m = 0; m < Np; ++m) {
           if (m == mi) continue;
           const int pr = m < mi ? pair_index<Np>(m, mi) : pair_index<Np>(mi, m);
-          const double coef = sD2[mi * Np + m];
+          const R coef = sD2[mi * Np + m];
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr[el][(dir & 1) * NPAIR + line * NPL + pr][q];
+          for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr(el, (dir & 1) * NPAIR + line * NPL + pr, q);
         }
       }
       lds_sync();
     }
   }
-  double du[NQ];
+  R du[NQ];
   {
-    const double scale = -1.0 / Je;
+    const R scale = R(-1.0 / Je);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) du[q] = scale * rate[q];
   }
 
   // ---- subcell FV residual and blend (EulerOperator_impl.hpp:120-157, DGSEMIntegrator.hpp:338-483)
   if (a.blending && sblend) {
-    double dfv[NQ];
+    R dfv[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) dfv[q] = 0.0;
+    for (int q = 0; q < NQ; ++q) dfv[q] = R(0);
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
       for (int w = tid; w < EPB * NI; w += TPB) {
@@ -526,48 +580,53 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
         if (ebase_blk + pe < a.ne) {
           const int L = ii / (Np - 1), pos = ii - L * (Np - 1);
           const int ln = face_node<DIM, Np>(dir, 0, L) + pos * stride[dir];
-          Pt<double, DIM> sl, sr;
-          load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], ln, sl);
-          load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], ln + stride[dir], sr);
-          double nor[DIM];
+          Pt<R, DIM> sl, sr;
+          load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), ln, sl);
+          load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), ln + stride[dir], sr);
+          R nor[DIM];
 #pragma unroll
-          for (int d = 0; d < DIM; ++d) nor[d] = sm.adj[pe][dir * DIM + d];
-          double F[NQ];
-          surface_flux<SURF, double, DIM>(g, sl, sr, nor, F);
+          for (int d = 0; d < DIM; ++d) nor[d] = R(sm.adj[pe][dir * DIM + d]);
+          R F[NQ];
+          surface_flux<SURF, R, DIM>(g, sl, sr, nor, F);
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) sscr[pe][ii][q] = F[q];
+          for (int q = 0; q < NQ; ++q) sscr(pe, ii, q) = F[q];
         }
       }
       lds_sync();
       if (valid) {
         const int L = face_index<DIM, Np>(dir, i, j, k);
         const int pos = dir == 0 ? i : (dir == 1 ? j : k);
-        const double den = Je * a.basis.w[pos];
+        const R den = R(Je * a.basis.w[pos]);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          const double fm = pos > 0 ? sscr[el][L * (Np - 1) + pos - 1][q] : 0.0;
-          const double fp = pos < Np - 1 ? sscr[el][L * (Np - 1) + pos][q] : 0.0;
-          const double val = (fm - fp) / den;
+          const R fm = pos > 0 ? sscr(el, L * (Np - 1) + pos - 1, q) : R(0);
+          const R fp = pos < Np - 1 ? sscr(el, L * (Np - 1) + pos, q) : R(0);
+          const R val = (fm - fp) / den;
           dfv[q] = dir == 0 ? val : dfv[q] + val;
         }
       }
       lds_sync();
     }
     if (valid && alpha > 1e-16) {
-      const double a_dg = 1.0 - alpha;
+      const R a_dg = R(1.0 - alpha), a_fv = R(alpha);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) du[q] = a_dg * du[q] + alpha * dfv[q];
+      for (int q = 0; q < NQ; ++q) du[q] = a_dg * du[q] + a_fv * dfv[q];
     }
   }
 
   // RK base state (mode 2), needed by the epilogue; its loads overlap the
   // face work.  Loaded unconditionally, from the stage state itself in the
-  // other modes (lines already in cache), so that waits stay exact.
-  double xb[NQ];
+  // other modes (lines already in cache), so that waits stay exact.  The
+  // mixed mode also re-reads the FP64 stage state here (its LDS copy is FP32).
+  double xb[NQ], q64[FP64 ? 1 : NQ];
   {
     const double *Xp = a.mode == 2 ? a.X : a.U;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) xb[q] = Xp[q * Ntot + ebase];
+    if constexpr (!FP64) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) q64[q] = a.U[q * Ntot + ebase];
+    }
   }
 
   // ---- faces: Riemann flux at every point of the element's own faces
@@ -584,35 +643,44 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
       if (fi < EPB * NFACE && ebase_blk + pe < a.ne) {
         const int f = fl / NF, t = fl - f * NF;
         const int d = f >> 1, s = f & 1;
-        Pt<double, DIM> own;
-        load_pt<double, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], face_node<DIM, Np>(d, s, t), own);
-        const double wown = 1.0 / (w0 * sm.J[pe]);
-        double n[DIM];
+        Pt<R, DIM> own;
+        load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), face_node<DIM, Np>(d, s, t), own);
+        const R wown = R(1.0 / (w0 * sm.J[pe]));
+        R n[DIM];
 #pragma unroll
-        for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[pe][f][dd];
+        for (int dd = 0; dd < DIM; ++dd) n[dd] = R(sm.nrm[pe][f][dd]);
         // One flux evaluation per face point, operands put in canonical
         // (low side first) order by selects, so that lanes on the two sides
         // of the element do not diverge.  A boundary point (supersonic
         // outflow, bc_kernels.hpp:236-238) evaluates F*(q, q, n_out).
         const bool interior = fnb[r] >= 0;
-        Pt<double, DIM> oth;  // the neighbour's trace (loaded at the top)
+        Pt<R, DIM> oth;  // the neighbour's trace (loaded at the top)
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) oth.q[q] = interior ? qn[r][q] : own.q[q];
+        for (int q = 0; q < NQ; ++q) oth.q[q] = interior ? R(qn[r][q]) : own.q[q];
         derive(g, oth);
-        Pt<double, DIM> pl, pr;
+        if constexpr (!FP64) {
+          // pressure from the FP64 trace, as for the staged nodes
+          Pt<double, DIM> o64;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) o64.q[q] = qn[r][q];
+          derive(a.gas, o64);
+          oth.p = interior ? R(o64.p) : own.p;
+          oth.pd = interior ? R(o64.p - pref) : own.pd;
+        }
+        Pt<R, DIM> pl, pr;
         pt_select(s != 0, own, oth, pl);
         pt_select(s != 0, oth, own, pr);
-        double F[NQ];
-        surface_flux<SURF, double, DIM>(g, pl, pr, n, F);
+        R F[NQ];
+        surface_flux<SURF, R, DIM>(g, pl, pr, n, F);
         if constexpr (VISC) {
           // F* - 0.5 n.(Fv_L + Fv_R)   (DGSEMIntegrator.hpp:261-316); the own
           // side's -0.5 n.Fv was folded into Vd by the gradient kernel
 #pragma unroll
-          for (int c = 0; c < NVD; ++c) F[1 + c] -= interior ? 0.5 * gn[r][c] : 0.0;
+          for (int c = 0; c < NVD; ++c) F[1 + c] -= interior ? R(0.5) * gn[r][c] : R(0);
         }
-        const double cs = (interior && !s) ? wown : -wown;
+        const R cs = (interior && !s) ? wown : -wown;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) sscr[pe][fl][q] = cs * F[q];
+        for (int q = 0; q < NQ; ++q) sscr(pe, fl, q) = cs * F[q];
       }
     }
   }
@@ -626,18 +694,20 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
       const int row = (2 * d + (hi ? 1 : 0)) * NF + face_index<DIM, Np>(d, i, j, k);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        const double v = sscr[el][row][q];
-        fsum[q] += on ? v : 0.0;
+        const R v = sscr(el, row, q);
+        fsum[q] += on ? v : R(0);
       }
     }
   }
 
-  // ---- epilogue: dU, or the fused SSP-RK3 stage combination
+  // ---- epilogue: dU, or the fused SSP-RK3 stage combination (FP64)
   if (valid) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const double r = du[q] + fsum[q];
-      const double qo = sq[el][q][node];
+      const double r = FP64 ? double(du[q] + fsum[q]) : double(du[q]) + double(fsum[q]);
+      double qo;
+      if constexpr (FP64) qo = sq[el][q][node];
+      else qo = q64[q];
       double o;
       if (a.mode == 0) o = r;
       else if (a.mode == 1) o = qo + a.dt * r;
@@ -658,7 +728,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
 // -0.5 n.Fv lifted with the face weight, and n.Fv at every point of the
 // element's faces for the neighbours' halves (n the canonical area vector).
 // ---------------------------------------------------------------------------
-template <int DIM, int P>
+template <typename R, int DIM, int P>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   using S = Shape<DIM, P>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
@@ -670,15 +740,17 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   constexpr int NB = EPB * DIM * NVD * ND;              // contravariant viscous fluxes
   constexpr int NBUF = NA > NB ? NA : NB;
 
-  __shared__ double sbuf[NBUF];
-  __shared__ double sD[Np * Np];
-  __shared__ double sDh[Np * Np];
+  __shared__ R sbuf[NBUF];
+  __shared__ R sD[Np * Np];
+  __shared__ R sDh[Np * Np];
   __shared__ ElemMeta<DIM, EPB> sm;
-  double (*sS)[NQ][ND] = reinterpret_cast<double (*)[NQ][ND]>(sbuf);
-  double (*sjump)[NFACE][NQ] = reinterpret_cast<double (*)[NFACE][NQ]>(sbuf + EPB * NQ * ND);
-  double (*sG)[DIM][NVD][ND] = reinterpret_cast<double (*)[DIM][NVD][ND]>(sbuf);
+  R (*sS)[NQ][ND] = reinterpret_cast<R (*)[NQ][ND]>(sbuf);
+  R (*sjump)[NFACE][NQ] = reinterpret_cast<R (*)[NFACE][NQ]>(sbuf + EPB * NQ * ND);
+  R (*sG)[DIM][NVD][ND] = reinterpret_cast<R (*)[DIM][NVD][ND]>(sbuf);
 
-  const GasC<double> &g = a.gas;
+  const GasC<R> &g = gas_of<R>(a);
+  R *Vd_out = static_cast<R *>(a.Vd_out);
+  R *Gf_out = static_cast<R *>(a.Gf_out);
   const int tid = threadIdx.x;
   const int el = tid / ND;
   const int node = tid - el * ND;
@@ -705,9 +777,9 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   meta_load_index<DIM, EPB, TPB>(a, ebase_blk, mr);
   meta_load_own<DIM, EPB, TPB>(a, ebase_blk, mr);
   const size_t ebase = (size_t)(valid ? e : 0) * ND + node;
-  double q[NQ];
+  double q64[NQ];
 #pragma unroll
-  for (int c = 0; c < NQ; ++c) q[c] = a.U[c * Ntot + ebase];
+  for (int c = 0; c < NQ; ++c) q64[c] = a.U[c * Ntot + ebase];
   static_assert(Np * Np <= TPB, "one basis entry per lane");
   const int kb = min(tid, Np * Np - 1);
   const double bD = a.basis.D[kb], bDh = a.basis.Dhat[kb];
@@ -725,15 +797,18 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
     for (int c = 0; c < NQ; ++c) qn[r][c] = a.U[c * Ntot + nbase];
   }
   if (tid < Np * Np) {
-    sD[tid] = bD;
-    sDh[tid] = bDh;
+    sD[tid] = R(bD);
+    sDh[tid] = R(bDh);
   }
 
   // entropy variables of the element's nodes (EOS.hpp:206-234)
   meta_store<DIM, EPB, TPB>(mr, sm);
+  R q[NQ];
+#pragma unroll
+  for (int c = 0; c < NQ; ++c) q[c] = R(q64[c]);
   if (valid) {
-    double Sv[NQ];
-    entropy_state<double, DIM>(g, q, Sv);
+    R Sv[NQ];
+    entropy_state<R, DIM>(g, q, Sv);
 #pragma unroll
     for (int c = 0; c < NQ; ++c) sS[el][c][node] = Sv[c];
   }
@@ -750,12 +825,14 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
       const int f = fl / NF, t = fl - f * NF;
       const int d = f >> 1, s = f & 1;
       const int on = face_node<DIM, Np>(d, s, t);
-      double Sn[NQ];
-      entropy_state<double, DIM>(g, qn[r], Sn);
+      R qr[NQ], Sn[NQ];
+#pragma unroll
+      for (int c = 0; c < NQ; ++c) qr[c] = R(qn[r][c]);
+      entropy_state<R, DIM>(g, qr, Sn);
 #pragma unroll
       for (int c = 0; c < NQ; ++c) {
-        const double so = sS[pe][c][on];
-        sjump[pe][fl][c] = s ? 0.5 * (Sn[c] - so) : 0.5 * (so - Sn[c]);
+        const R so = sS[pe][c][on];
+        sjump[pe][fl][c] = s ? R(0.5) * (Sn[c] - so) : R(0.5) * (so - Sn[c]);
       }
     }
   }
@@ -766,40 +843,40 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   const int k = DIM > 2 ? node / (Np * Np) : 0;
   const int idx[3] = {i, j, k};
   const double Je = sm.J[elc];
-  const double invJ = 1.0 / Je;
-  const double wown = 1.0 / (a.basis.w[0] * Je);
+  const R invJ = R(1.0 / Je);
+  const R wown = R(1.0 / (a.basis.w[0] * Je));
 
-  double vf[NV];
-  double hv[NVD];  // the own side's share -0.5 n.Fv of the central viscous face flux, lifted
+  R vf[NV];
+  R hv[NVD];  // the own side's share -0.5 n.Fv of the central viscous face flux, lifted
 #pragma unroll
-  for (int c = 0; c < NVD; ++c) hv[c] = 0.0;
+  for (int c = 0; c < NVD; ++c) hv[c] = R(0);
   if (valid) {
     // volume gradient (AssembleGradVolumePointKernel): acc[dd] = sum_ref (D.S)_ref adj(ref,dd),
     // one reference direction at a time, in the reference's ref = 0,1,2 order
-    double acc[DIM][NQ];
+    R acc[DIM][NQ];
 #pragma unroll 1
     for (int dir = 0; dir < DIM; ++dir) {
-      double dref[NQ];
+      R dref[NQ];
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) dref[c] = 0.0;
+      for (int c = 0; c < NQ; ++c) dref[c] = R(0);
       const int mi = dir == 0 ? i : (dir == 1 ? j : k);
       const int st = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
       const int base = node - mi * st;
 #pragma unroll
       for (int l = 0; l < Np; ++l) {
-        const double coef = sD[mi * Np + l];
+        const R coef = sD[mi * Np + l];
         const int sn_ = base + l * st;
 #pragma unroll
         for (int c = 0; c < NQ; ++c) dref[c] += sS[el][c][sn_] * coef;
       }
 #pragma unroll
       for (int dd = 0; dd < DIM; ++dd) {
-        const double ad = sm.adj[el][dir * DIM + dd];
+        const R ad = R(sm.adj[el][dir * DIM + dd]);
 #pragma unroll
         for (int c = 0; c < NQ; ++c) acc[dd][c] = dir == 0 ? dref[c] * ad : acc[dd][c] + dref[c] * ad;
       }
     }
-    const NodeThermo<double, DIM> th = node_thermo<double, DIM>(g, q);
+    const NodeThermo<R, DIM> th = node_thermo<R, DIM>(g, q);
 
     // gradient = volume part / J + w * jump * n from the face through the
     // node in each direction (a node lies on at most one face per direction)
@@ -813,23 +890,23 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
       const bool on = hi || idx[d] == 0;
       const int f = 2 * d + (hi ? 1 : 0);
       const int row = f * NF + face_index<DIM, Np>(d, i, j, k);
-      double jw[NQ];
+      R jw[NQ];
 #pragma unroll
       for (int c = 0; c < NQ; ++c) {
-        const double v = wown * sjump[el][row][c];
-        jw[c] = on ? v : 0.0;
+        const R v = wown * sjump[el][row][c];
+        jw[c] = on ? v : R(0);
       }
 #pragma unroll
       for (int dd = 0; dd < DIM; ++dd) {
-        const double nd = sm.nrm[el][f][dd];
+        const R nd = R(sm.nrm[el][f][dd]);
 #pragma unroll
         for (int c = 0; c < NQ; ++c) acc[dd][c] += jw[c] * nd;
       }
     }
-    double dP[DIM][NQ];
+    R dP[DIM][NQ];
 #pragma unroll
-    for (int dd = 0; dd < DIM; ++dd) grad_entropy_to_grad_prim<double, DIM>(th, q, acc[dd], dP[dd]);
-    viscous_flux<double, DIM>(g, th, dP, vf);
+    for (int dd = 0; dd < DIM; ++dd) grad_entropy_to_grad_prim<R, DIM>(th, q, acc[dd], dP[dd]);
+    viscous_flux<R, DIM>(g, th, dP, vf);
 
     // n.Fv at the element's own face points, n the canonical area vector
 #pragma unroll
@@ -838,17 +915,17 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
       if (hi || idx[d] == 0) {
         const int f = 2 * d + (hi ? 1 : 0);
         const size_t gi = (size_t)e * NFACE + f * NF + face_index<DIM, Np>(d, i, j, k);
-        double n[DIM];
+        R n[DIM];
 #pragma unroll
-        for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[el][f][dd];
+        for (int dd = 0; dd < DIM; ++dd) n[dd] = R(sm.nrm[el][f][dd]);
 #pragma unroll
         for (int c = 0; c < NVD; ++c) {
-          double v = n[0] * vf[vis_index<DIM>(1 + c, 0)];
+          R v = n[0] * vf[vis_index<DIM>(1 + c, 0)];
 #pragma unroll
           for (int dd = 1; dd < DIM; ++dd) v += n[dd] * vf[vis_index<DIM>(1 + c, dd)];
-          a.Gf_out[c * Gtot + gi] = v;
+          Gf_out[c * Gtot + gi] = v;
           // face term scale: -w on the element's high face, +w on its low face
-          hv[c] += (hi ? wown : -wown) * (0.5 * v);
+          hv[c] += (hi ? wown : -wown) * (R(0.5) * v);
         }
       }
     }
@@ -859,12 +936,12 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   if (valid) {
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
-      double row[DIM];
+      R row[DIM];
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) row[d] = sm.adj[el][dir * DIM + d];
+      for (int d = 0; d < DIM; ++d) row[d] = R(sm.adj[el][dir * DIM + d]);
 #pragma unroll
       for (int c = 0; c < NVD; ++c) {
-        double fr = row[0] * vf[vis_index<DIM>(1 + c, 0)];
+        R fr = row[0] * vf[vis_index<DIM>(1 + c, 0)];
 #pragma unroll
         for (int ph = 1; ph < DIM; ++ph) fr += row[ph] * vf[vis_index<DIM>(1 + c, ph)];
         sG[el][dir][c][node] = fr;
@@ -873,9 +950,9 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   }
   lds_sync();
   if (valid) {
-    double dv[NVD];
+    R dv[NVD];
 #pragma unroll
-    for (int c = 0; c < NVD; ++c) dv[c] = 0.0;
+    for (int c = 0; c < NVD; ++c) dv[c] = R(0);
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
       const int mi = dir == 0 ? i : (dir == 1 ? j : k);
@@ -883,13 +960,13 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
       const int base = node - mi * st;
 #pragma unroll
       for (int l = 0; l < Np; ++l) {
-        const double coef = sDh[mi * Np + l];
+        const R coef = sDh[mi * Np + l];
 #pragma unroll
         for (int c = 0; c < NVD; ++c) dv[c] += coef * sG[el][dir][c][base + l * st];
       }
     }
 #pragma unroll
-    for (int c = 0; c < NVD; ++c) a.Vd_out[c * Ntot + ebase] = invJ * dv[c] + hv[c];
+    for (int c = 0; c < NVD; ++c) Vd_out[c * Ntot + ebase] = invJ * dv[c] + hv[c];
   }
 }
 
