@@ -51,6 +51,7 @@ extern "C" {
 #define THESEUS_ERR_UNSUPPORTED (-2)
 #define THESEUS_ERR_DEVICE (-3)
 #define THESEUS_ERR_NONFINITE (-4)
+#define THESEUS_ERR_CAPACITY (-5) /* serving pool: not enough free elements or case slots */
 
 /* ---- flow model (SimFactory_impl.hpp:119-123 "flow_model") ------------- */
 #define THESEUS_FLOW_EULER 0
@@ -178,6 +179,49 @@ THESEUS_API int theseus_rhs_kernel_timing(theseus_rhs *h, int kind, double *mean
 
 /* Thread-local message of the last failing call. */
 THESEUS_API const char *theseus_last_error(void);
+
+/* ---- serving pool (no counterpart in the reference, whose driver runs one
+ * case per process, src/Simulation.cpp:1255) ---------------------------------
+ * Many independent cases of one (dim, order, flux pairing, precision) share
+ * one device-resident element array: each admitted case occupies a
+ * contiguous range of element slots (first fit), its connectivity offset
+ * into the array, so one kernel launch per stage advances every case and a
+ * case's result does not depend on what else is in the pool.  The pool's
+ * capacity is set by an HBM budget; admission fails with
+ * THESEUS_ERR_CAPACITY when no free range is large enough, and a released
+ * case's slots are reused.  The SSP-RK3 step of all admitted cases is one
+ * hipGraph, captured at the first step and replayed; each case advances with
+ * its own dt, fixed at admission from its stability estimate.  The tile
+ * shape (elements per workgroup, block order) is picked at create time by
+ * timing the candidates on a synthetic case (tile/xcd_remap = -1), or fixed. */
+typedef struct theseus_pool theseus_pool; /* opaque */
+
+typedef struct theseus_pool_desc {
+  int dim, order, flow_model, volume_flux, surface_flux, precision, blending;
+  double gamma, Pr, R_gas, mu, mu_bulk, alpha_max, alpha_min;
+  long long hbm_budget_bytes; /* device memory the pool may hold */
+  int max_cases;              /* case slots */
+  int tile;                   /* -1: autotune; 0: default elements per workgroup; 1: twice that (Euler) */
+  int xcd_remap;              /* -1: autotune; 1: contiguous element ranges per XCD; 0: launch order */
+} theseus_pool_desc;
+
+THESEUS_API int theseus_pool_create(const theseus_pool_desc *desc, theseus_pool **out);
+THESEUS_API int theseus_pool_destroy(theseus_pool *p);
+/* Capacity and occupancy; any output may be NULL. */
+THESEUS_API int theseus_pool_info(const theseus_pool *p, int *capacity_elements, int *free_elements,
+                                  int *largest_free_range, int *active_cases, int *tile, int *xcd_remap);
+/* Admit one case (physics and kernel fields must match the pool's; its
+ * geometry/connectivity arrays are host pointers as for theseus_rhs_create),
+ * with initial state u0 (its L-vector, host or device memory) and
+ * dt = cfl / total stability rate of u0.  Synchronises `stream`. */
+THESEUS_API int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double *u0, double cfl,
+                                   int *case_id, double *dt, void *stream);
+/* One SSP-RK3 step of every admitted case (graph replay on `stream`). */
+THESEUS_API int theseus_pool_step(theseus_pool *p, void *stream);
+/* Copy a case's current state (its L-vector) to `u` (host or device memory); synchronises `stream`. */
+THESEUS_API int theseus_pool_read(theseus_pool *p, int case_id, double *u, void *stream);
+/* Free a case's slots for later admissions. */
+THESEUS_API int theseus_pool_release(theseus_pool *p, int case_id, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,139 @@
+"""Serving pool and continuous-batching server on the GPU: a case's result
+is bitwise the same alone and in a mix of other cases (and at a different
+slot offset), equals the single-case operator's, matches the FP64 oracle;
+admission respects the HBM-budget capacity and reuses released slots; the
+server drains a Poisson stream and reports p50/p99."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from theseus_amd import _abi as A
+
+from .gpu_helpers import oracle_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _run_pool_case(pool, factory, cases, target, steps):
+    """Admit `cases` (list of (n, amp, phases)) in order, step, return the state of case index `target`."""
+    import torch
+
+    ids = []
+    for n, amp, ph in cases:
+        u0 = factory.state(pool.rt, n, amp, ph)
+        cid, dt = pool.admit(factory.mesh(n), u0, 0.5)
+        ids.append((cid, dt, n))
+    for _ in range(steps):
+        pool.step()
+    torch.cuda.synchronize()
+    cid, dt, n = ids[target]
+    out = torch.empty(5 * n ** 3 * (pool.rt.order + 1) ** 3, dtype=torch.float64, device="cuda")
+    pool.read(cid, out)
+    return out.cpu().numpy(), dt
+
+
+@pytest.mark.parametrize("precision", [A.PRECISION_FP64, A.PRECISION_MIXED])
+@pytest.mark.parametrize("order", [2, 3])
+def test_case_alone_equals_case_in_mix(precision, order):
+    from theseus_amd.serving import CaseFactory, Pool, euler_runtime
+
+    _say(f"case pool alone-vs-mix p={order} precision={precision}")
+    f = CaseFactory()
+    rt = euler_runtime(order, precision)
+    target = (4, 2.0, (0.3, 1.1, 2.0))
+    pool = Pool(rt, 256 << 20, 16, tile=0, xcd_remap=1)
+    alone, dt_alone = _run_pool_case(pool, f, [target], 0, 6)
+    pool.close()
+    pool = Pool(rt, 256 << 20, 16, tile=0, xcd_remap=1)
+    mix = [(3, 4.0, (0.0, 0.5, 1.0)), (5, 0.7, (2.0, 0.1, 0.4)), target, (4, 3.3, (1.0, 1.0, 1.0))]
+    mixed, dt_mix = _run_pool_case(pool, f, mix, 2, 6)
+    pool.close()
+    assert dt_alone == dt_mix
+    assert np.isfinite(alone).all()
+    assert np.array_equal(alone, mixed), np.abs(alone - mixed).max()
+
+
+def test_pool_matches_single_case_operator_and_oracle(oracle):
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+    from theseus_amd.serving import CaseFactory, Pool, euler_runtime
+
+    _say("case pool vs single-case operator vs oracle p=3")
+    f = CaseFactory()
+    rt = euler_runtime(3, A.PRECISION_FP64)
+    n, amp, ph = 3, 1.5, (0.2, 0.4, 0.6)
+    pool = Pool(rt, 128 << 20, 8, tile=0, xcd_remap=1)
+    got, dt = _run_pool_case(pool, f, [(5, 1.0, (0, 0, 0)), (n, amp, ph)], 1, 10)
+    pool.close()
+    mesh = f.mesh(n)
+    u0 = f.state(rt, n, amp, ph)
+    op = MakeRHSOperator(rt, mesh)
+    u = u0.clone()
+    t = 0.0
+    for _ in range(10):
+        t = op.Step(u, t, dt)
+    torch.cuda.synchronize()
+    single = u.cpu().numpy()
+    assert rel_l2(got, single) <= 1e-14, rel_l2(got, single)
+    ora = oracle_case(oracle, rt, mesh)
+    ref = u0.cpu().numpy().copy()
+    for _ in range(10):
+        ref = ora.step(ref, dt)
+    assert rel_l2(got, ref) <= 1e-10, rel_l2(got, ref)
+
+
+def test_pool_capacity_release_and_reuse():
+    from theseus_amd.serving import CapacityError, CaseFactory, Pool, euler_runtime
+
+    _say("case pool capacity")
+    f = CaseFactory()
+    rt = euler_runtime(2, A.PRECISION_MIXED)
+    # budget for 64 element slots
+    per = 3 * 5 * 27 * 8 + 8 * 10 + 4 * 7 + 8
+    pool = Pool(rt, 64 * per + per // 2, 8, tile=0, xcd_remap=0)
+    info = pool.info()
+    assert info["capacity_elements"] == 64, info
+    a, _ = pool.admit(f.mesh(3), f.state(rt, 3, 1.0, (0, 0, 0)), 0.5)  # 27 elements
+    b, _ = pool.admit(f.mesh(3), f.state(rt, 3, 1.0, (1, 0, 0)), 0.5)  # 27 elements
+    with pytest.raises(CapacityError):
+        pool.admit(f.mesh(3), f.state(rt, 3, 1.0, (2, 0, 0)), 0.5)
+    pool.release(a)
+    info = pool.info()
+    assert info["active_cases"] == 1 and info["free_elements"] == 64 - 27
+    c, _ = pool.admit(f.mesh(3), f.state(rt, 3, 1.0, (2, 0, 0)), 0.5)
+    for _ in range(2):
+        pool.step()
+    pool.close()
+    assert c == a  # the freed case slot is reused
+
+
+def test_autotuner_picks_a_tile():
+    from theseus_amd.serving import Pool, euler_runtime
+
+    _say("case pool autotune p=4")
+    pool = Pool(euler_runtime(4, A.PRECISION_MIXED), 512 << 20, 8, tile=-1, xcd_remap=-1)
+    info = pool.info()
+    pool.close()
+    assert info["tile"] in (0, 1) and info["xcd_remap"] in (0, 1), info
+
+
+def test_server_drains_a_poisson_stream():
+    from theseus_amd.serving import Server, make_requests
+
+    _say("case server stream")
+    reqs = make_requests(num=24, rate_per_s=400.0, n_range=(3, 6), steps=8)
+    srv = Server(hbm_budget_bytes=1 << 30, max_cases=8, keep_results=True)
+    res = srv.run(reqs, time_limit_s=120.0)
+    srv.close()
+    _say(f"  {res}")
+    assert res["cases_done"] == 24
+    assert math.isfinite(res["tts_p50_s"]) and res["tts_p99_s"] >= res["tts_p50_s"] > 0
+    assert all(r.result is not None and np.isfinite(r.result).all() for r in reqs)

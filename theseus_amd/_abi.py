@@ -15,6 +15,8 @@ OK = 0
 ERR_INVALID_ARGUMENT = -1
 ERR_UNSUPPORTED = -2
 ERR_DEVICE = -3
+ERR_NONFINITE = -4
+ERR_CAPACITY = -5
 
 FLOW_EULER = 0
 FLOW_NAVIER_STOKES = 1
@@ -47,6 +49,13 @@ EXPORTED_SYMBOLS = (
     "theseus_rhs_enable_kernel_timing",
     "theseus_rhs_kernel_timing",
     "theseus_last_error",
+    "theseus_pool_create",
+    "theseus_pool_destroy",
+    "theseus_pool_info",
+    "theseus_pool_admit",
+    "theseus_pool_step",
+    "theseus_pool_read",
+    "theseus_pool_release",
 )
 
 
@@ -72,6 +81,29 @@ class CaseDesc(C.Structure):
         ("face_neighbor", C.POINTER(C.c_int)),
         ("num_bc", C.c_int),
         ("bc_type", C.POINTER(C.c_int)),
+    ]
+
+
+class PoolDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("order", C.c_int),
+        ("flow_model", C.c_int),
+        ("volume_flux", C.c_int),
+        ("surface_flux", C.c_int),
+        ("precision", C.c_int),
+        ("blending", C.c_int),
+        ("gamma", C.c_double),
+        ("Pr", C.c_double),
+        ("R_gas", C.c_double),
+        ("mu", C.c_double),
+        ("mu_bulk", C.c_double),
+        ("alpha_max", C.c_double),
+        ("alpha_min", C.c_double),
+        ("hbm_budget_bytes", C.c_longlong),
+        ("max_cases", C.c_int),
+        ("tile", C.c_int),
+        ("xcd_remap", C.c_int),
     ]
 
 
@@ -111,6 +143,17 @@ def load(path: Path | str | None = None) -> C.CDLL:
     lib.theseus_rhs_kernel_timing.restype = C.c_int
     lib.theseus_last_error.argtypes = []
     lib.theseus_last_error.restype = C.c_char_p
+    vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+    lib.theseus_pool_create.argtypes = [C.POINTER(PoolDesc), C.POINTER(C.c_void_p)]
+    lib.theseus_pool_destroy.argtypes = [vp]
+    lib.theseus_pool_info.argtypes = [vp, ip, ip, ip, ip, ip, ip]
+    lib.theseus_pool_admit.argtypes = [vp, C.POINTER(CaseDesc), vp, C.c_double, ip, dp, vp]
+    lib.theseus_pool_step.argtypes = [vp, vp]
+    lib.theseus_pool_read.argtypes = [vp, C.c_int, vp, vp]
+    lib.theseus_pool_release.argtypes = [vp, C.c_int, vp]
+    for name in ("theseus_pool_create", "theseus_pool_destroy", "theseus_pool_info", "theseus_pool_admit",
+                 "theseus_pool_step", "theseus_pool_read", "theseus_pool_release"):
+        getattr(lib, name).restype = C.c_int
     if path is None:
         _lib = lib
     return lib

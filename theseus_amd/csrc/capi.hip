@@ -36,6 +36,15 @@ std::map<std::tuple<int, int>, ShapeKernels> &shape_table() {
 
 void register_flux(int dim, int p, int vol, int surf, FluxKernels k) { flux_table()[{dim, p, vol, surf}] = k; }
 void register_shape(int dim, int p, ShapeKernels k) { shape_table()[{dim, p}] = k; }
+std::map<std::tuple<int, int, int, int>, TileKernels> &tile_table() {
+  static std::map<std::tuple<int, int, int, int>, TileKernels> t;
+  return t;
+}
+void register_tiles(int dim, int p, int vol, int surf, TileKernels k) { tile_table()[{dim, p, vol, surf}] = k; }
+const TileKernels *find_tiles(int dim, int p, int vol, int surf) {
+  auto it = tile_table().find({dim, p, vol, surf});
+  return it == tile_table().end() ? nullptr : &it->second;
+}
 const FluxKernels *find_flux(int dim, int p, int vol, int surf) {
   auto it = flux_table().find({dim, p, vol, surf});
   return it == flux_table().end() ? nullptr : &it->second;
@@ -273,6 +282,7 @@ struct theseus_rhs {
     a.gas = gas;
     a.gas32 = gas32;
     a.pref = d_pref;
+    a.xcd_remap = 1;
     a.basis = basis;
     return a;
   }
@@ -317,6 +327,55 @@ static int upload(T **dst, const T *src, size_t n) {
   return THESEUS_OK;
 }
 
+static int validate_mesh(const theseus_case_desc &d, bool &has_boundary) {
+  const int ne = d.num_elements, dim = d.dim;
+  has_boundary = false;
+  for (long long f = 0; f < (long long)ne * 2 * dim; ++f) {
+    const int nb = d.face_neighbor[f];
+    const int e = (int)(f / (2 * dim)), lf = (int)(f % (2 * dim));
+    if (nb >= ne) return fail(THESEUS_ERR_INVALID_ARGUMENT, "face_neighbor index out of range");
+    if (nb >= 0) {
+      const int opp = (lf & 1) ? lf - 1 : lf + 1;
+      if (d.face_neighbor[(long long)nb * 2 * dim + opp] != e)
+        return fail(THESEUS_ERR_INVALID_ARGUMENT, "face_neighbor is not symmetric (element " + std::to_string(e) +
+                                                      ", face " + std::to_string(lf) + ")");
+    } else {
+      has_boundary = true;
+      const int bc = -(nb + 1);
+      if (bc >= d.num_bc || !d.bc_type) return fail(THESEUS_ERR_INVALID_ARGUMENT, "boundary index out of range");
+      if (d.bc_type[bc] != THESEUS_BC_SUPERSONIC_OUTFLOW)
+        return fail(THESEUS_ERR_UNSUPPORTED, "only supersonic-outflow (transmissive) boundaries are implemented");
+    }
+  }
+  if (has_boundary && d.flow_model == THESEUS_FLOW_NAVIER_STOKES)
+    return fail(THESEUS_ERR_UNSUPPORTED, "Navier-Stokes boundary conditions are not implemented (periodic only)");
+  for (int e = 0; e < ne; ++e)
+    if (!(d.elem_jac[e] > 0) || !std::isfinite(d.elem_jac[e]))
+      return fail(THESEUS_ERR_INVALID_ARGUMENT, "element Jacobian must be positive and finite");
+
+  return THESEUS_OK;
+}
+
+// PhysicsConstants (Physics.hpp:33-38) in FP64 and rounded to FP32
+static void init_gas(const theseus_case_desc &d, GasC<double> &g, GasC<float> &g32) {
+  g.gamma = d.gamma;
+  g.gammaM1 = d.gamma - 1.0;
+  g.gammaM1Inverse = 1.0 / g.gammaM1;
+  g.cp = (d.gamma * g.gammaM1Inverse) * d.R_gas;
+  g.PrInverse = d.Pr > 0 ? 1.0 / d.Pr : 0.0;
+  g.mu = d.mu;
+  g.mu_bulk = d.mu_bulk;
+  g.R_gas = d.R_gas;
+  g32.gamma = (float)g.gamma;
+  g32.gammaM1 = (float)g.gammaM1;
+  g32.gammaM1Inverse = (float)g.gammaM1Inverse;
+  g32.cp = (float)g.cp;
+  g32.PrInverse = (float)g.PrInverse;
+  g32.mu = (float)g.mu;
+  g32.mu_bulk = (float)g.mu_bulk;
+  g32.R_gas = (float)g.R_gas;
+}
+
 extern "C" {
 
 const char *theseus_last_error(void) { return g_last_error.c_str(); }
@@ -345,30 +404,10 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
                   d.order, d.volume_flux, d.surface_flux);
     return fail(THESEUS_ERR_UNSUPPORTED, buf);
   }
-  const int ne = d.num_elements, dim = d.dim;
   bool has_boundary = false;
-  for (long long f = 0; f < (long long)ne * 2 * dim; ++f) {
-    const int nb = d.face_neighbor[f];
-    const int e = (int)(f / (2 * dim)), lf = (int)(f % (2 * dim));
-    if (nb >= ne) return fail(THESEUS_ERR_INVALID_ARGUMENT, "face_neighbor index out of range");
-    if (nb >= 0) {
-      const int opp = (lf & 1) ? lf - 1 : lf + 1;
-      if (d.face_neighbor[(long long)nb * 2 * dim + opp] != e)
-        return fail(THESEUS_ERR_INVALID_ARGUMENT, "face_neighbor is not symmetric (element " + std::to_string(e) +
-                                                      ", face " + std::to_string(lf) + ")");
-    } else {
-      has_boundary = true;
-      const int bc = -(nb + 1);
-      if (bc >= d.num_bc || !d.bc_type) return fail(THESEUS_ERR_INVALID_ARGUMENT, "boundary index out of range");
-      if (d.bc_type[bc] != THESEUS_BC_SUPERSONIC_OUTFLOW)
-        return fail(THESEUS_ERR_UNSUPPORTED, "only supersonic-outflow (transmissive) boundaries are implemented");
-    }
-  }
-  if (has_boundary && d.flow_model == THESEUS_FLOW_NAVIER_STOKES)
-    return fail(THESEUS_ERR_UNSUPPORTED, "Navier-Stokes boundary conditions are not implemented (periodic only)");
-  for (int e = 0; e < ne; ++e)
-    if (!(d.elem_jac[e] > 0) || !std::isfinite(d.elem_jac[e]))
-      return fail(THESEUS_ERR_INVALID_ARGUMENT, "element Jacobian must be positive and finite");
+  int rc0 = validate_mesh(d, has_boundary);
+  if (rc0) return rc0;
+  const int ne = d.num_elements, dim = d.dim;
 
   theseus_rhs *h = new theseus_rhs();
   h->d = d;
@@ -388,23 +427,7 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
   gll01(d.order, x, w);
   derivative_ops(d.order, x, w, h->basis);
   legendre_vandermonde_inverse(d.order, x, h->basis.Vinv);
-  // PhysicsConstants (Physics.hpp:33-38)
-  h->gas.gamma = d.gamma;
-  h->gas.gammaM1 = d.gamma - 1.0;
-  h->gas.gammaM1Inverse = 1.0 / h->gas.gammaM1;
-  h->gas.cp = (d.gamma * h->gas.gammaM1Inverse) * d.R_gas;
-  h->gas.PrInverse = d.Pr > 0 ? 1.0 / d.Pr : 0.0;
-  h->gas.mu = d.mu;
-  h->gas.mu_bulk = d.mu_bulk;
-  h->gas.R_gas = d.R_gas;
-  h->gas32.gamma = (float)h->gas.gamma;
-  h->gas32.gammaM1 = (float)h->gas.gammaM1;
-  h->gas32.gammaM1Inverse = (float)h->gas.gammaM1Inverse;
-  h->gas32.cp = (float)h->gas.cp;
-  h->gas32.PrInverse = (float)h->gas.PrInverse;
-  h->gas32.mu = (float)h->gas.mu;
-  h->gas32.mu_bulk = (float)h->gas.mu_bulk;
-  h->gas32.R_gas = (float)h->gas.R_gas;
+  init_gas(d, h->gas, h->gas32);
   h->pm = d.precision == THESEUS_PRECISION_MIXED ? 1 : 0;
   h->adv_scale = advection_scale(d.order);
   h->diff_scale = diffusion_scale(d.order);
@@ -570,6 +593,517 @@ int theseus_rhs_get_alpha(const theseus_rhs *h, double *alpha, void *stream) {
   if (!h->d.blending) return fail(THESEUS_ERR_INVALID_ARGUMENT, "blending is off for this operator");
   HIP_TRY(hipMemcpyAsync(alpha, h->d_alpha, sizeof(double) * h->d.num_elements, hipMemcpyDeviceToDevice,
                          (hipStream_t)stream));
+  return THESEUS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving pool: many cases in one element array (see theseus_rhs.h).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct PoolCase {
+  int start = -1, ne = 0;
+  bool active = false;
+};
+
+// Process-wide memory of the autotuner: (dim, order, vol, surf, precision) -> (tile, xcd_remap)
+std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> &tune_cache() {
+  static std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> c;
+  return c;
+}
+std::mutex &tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+}  // namespace
+
+struct theseus_pool {
+  theseus_pool_desc d;
+  theseus_case_desc cd;  // the pool's physics as a case descriptor (no arrays)
+  int Np = 0, ndof = 0, nfp = 0, neq = 0, cap = 0;
+  long long Ntot = 0;
+  const FluxKernels *fk = nullptr;
+  const ShapeKernels *sk = nullptr;
+  const TileKernels *tk = nullptr;
+  BasisArgs basis;
+  GasC<double> gas;
+  GasC<float> gas32;
+  int pm = 0, tile = 0, xcd = 1;
+  double adv_scale = 0, diff_scale = 0, surf_scale = 0;
+  double *d_J = nullptr, *d_adj = nullptr, *d_alpha = nullptr, *d_Fv = nullptr, *d_U = nullptr;
+  double *d_stage[2] = {nullptr, nullptr};
+  int *d_nbr = nullptr, *d_elem_case = nullptr;
+  double *d_case_dt = nullptr, *d_case_pref = nullptr;
+  unsigned long long *d_rates = nullptr;
+  std::vector<PoolCase> cases;
+  std::vector<unsigned char> slot_used;  // per element slot
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  int epb() const { return (tk && cd.flow_model == THESEUS_FLOW_EULER) ? tk->epb[tile] : sk->epb; }
+  int nblocks() const { return (cap + epb() - 1) / epb(); }
+
+  RHSArgs args(const double *U, double *out) const {
+    RHSArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.U = U;
+    a.out = out;
+    if (d_Fv) {
+      const size_t esz = pm ? sizeof(float) : sizeof(double);
+      char *base = reinterpret_cast<char *>(d_Fv);
+      a.Vd = a.Vd_out = base;
+      a.Gf = a.Gf_out = base + esz * (size_t)(cd.dim + 1) * Ntot;
+    }
+    a.alpha = d_alpha;
+    a.Ntot = Ntot;
+    a.ne = cap;
+    a.blending = cd.blending;
+    a.alpha_max = cd.alpha_max;
+    a.alpha_min = cd.alpha_min;
+    a.sharp = 9.21024;
+    a.thresh = 0.5 * std::pow(10.0, -1.8 * std::pow(cd.order, 0.25));
+    a.J = d_J;
+    a.adj = d_adj;
+    a.nbr = d_nbr;
+    a.gas = gas;
+    a.gas32 = gas32;
+    a.elem_case = d_elem_case;
+    a.case_dt = d_case_dt;
+    a.case_pref = d_case_pref;
+    a.xcd_remap = xcd;
+    a.basis = basis;
+    return a;
+  }
+
+  void launch(const RHSArgs &a, hipStream_t s) const {
+    if (cd.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
+      sk->ns_grad[pm](a, nblocks(), s);
+      fk->ns[pm](a, nblocks(), s);
+    } else if (tk) {
+      tk->euler[tile][pm](a, nblocks(), s);
+    } else {
+      fk->euler[pm](a, nblocks(), s);
+    }
+  }
+
+  // the three SSP-RK3 stages of every case (MFEM RK3SSPSolver, src/Simulation.cpp:265-268)
+  void enqueue_step(hipStream_t s) const {
+    double *u = d_U, *A = d_stage[0], *B = d_stage[1];
+    RHSArgs a = args(u, A);
+    a.mode = 1;
+    launch(a, s);
+    a = args(A, B);
+    a.mode = 2;
+    a.X = u;
+    a.cx = 3. / 4;
+    a.cy = 1. / 4;
+    launch(a, s);
+    a = args(B, u);
+    a.mode = 2;
+    a.X = u;
+    a.cx = 1. / 3;
+    a.cy = 2. / 3;
+    launch(a, s);
+  }
+};
+
+static size_t pool_bytes_per_element(const theseus_pool_desc &d, int ndof, int nfp) {
+  const int neq = d.dim + 2;
+  size_t b = 3 * (size_t)neq * ndof * sizeof(double);              // state + two stages
+  b += sizeof(double) * (1 + d.dim * d.dim) + sizeof(int) * (2 * d.dim + 1) + sizeof(double);  // J, adj, nbr, owner, alpha
+  if (d.flow_model == THESEUS_FLOW_NAVIER_STOKES)
+    b += (size_t)(d.dim + 1) * (ndof + 2 * d.dim * nfp) * sizeof(double);
+  return b;
+}
+
+// Free slots point at themselves, so the kernels' unconditional neighbour
+// loads stay inside the arrays; their owner is -1 (skipped).
+static int pool_reset_slots(theseus_pool *p, int start, int n, hipStream_t s) {
+  const int dim = p->cd.dim;
+  std::vector<int> nbr((size_t)n * 2 * dim), owner(n, -1);
+  std::vector<double> J(n, 1.0), adj((size_t)n * dim * dim, 0.0);
+  for (int e = 0; e < n; ++e) {
+    for (int f = 0; f < 2 * dim; ++f) nbr[(size_t)e * 2 * dim + f] = start + e;
+    for (int dd = 0; dd < dim; ++dd) adj[(size_t)e * dim * dim + dd * dim + dd] = 1.0;
+  }
+  HIP_TRY(hipMemcpyAsync(p->d_nbr + (size_t)start * 2 * dim, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_elem_case + start, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_J + start, J.data(), J.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_adj + (size_t)start * dim * dim, adj.data(), adj.size() * sizeof(double),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return THESEUS_OK;
+}
+
+static int pool_find_range(const theseus_pool *p, int n) {
+  int run = 0;
+  for (int e = 0; e < p->cap; ++e) {
+    run = p->slot_used[e] ? 0 : run + 1;
+    if (run == n) return e - n + 1;
+  }
+  return -1;
+}
+
+// Times the candidate tile shapes on a synthetic periodic box that fills (up
+// to 32^3 elements of) the pool: a perturbed uniform flow, 3 warmup + 8
+// timed RHS evaluations each; keeps the fastest.
+static int pool_autotune(theseus_pool *p, hipStream_t s) {
+  const int dim = p->cd.dim;
+  int n = 1;
+  while (n < 32 && (long long)(n + 1) * (n + 1) * (dim == 3 ? (n + 1) : 1) <= p->cap) ++n;
+  const int ne = dim == 3 ? n * n * n : n * n;
+  if (ne < 8) return THESEUS_OK;
+  std::vector<double> J(ne), adj((size_t)ne * dim * dim, 0.0);
+  std::vector<int> nbr((size_t)ne * 2 * dim), owner(ne, 0);
+  const double h = 1.0 / n;
+  for (int e = 0; e < ne; ++e) {
+    const int ix = e % n, iy = (e / n) % n, iz = dim == 3 ? e / (n * n) : 0;
+    const int c[3] = {ix, iy, iz};
+    J[e] = dim == 3 ? h * h * h : h * h;
+    for (int dd = 0; dd < dim; ++dd) adj[(size_t)e * dim * dim + dd * dim + dd] = dim == 3 ? h * h : h;
+    for (int dd = 0; dd < dim; ++dd)
+      for (int sd = 0; sd < 2; ++sd) {
+        int cc[3] = {c[0], c[1], c[2]};
+        cc[dd] = (cc[dd] + (sd ? 1 : n - 1)) % n;
+        nbr[(size_t)e * 2 * dim + 2 * dd + sd] = cc[0] + n * cc[1] + n * n * cc[2];
+      }
+  }
+  const long long nd = (long long)ne * p->ndof;
+  std::vector<double> U((size_t)p->neq * p->Ntot, 0.0);
+  for (long long i = 0; i < nd; ++i) {
+    const double w = 0.05 * std::sin(0.37 * (double)i);
+    const double rho = 1.0 + w, u0 = 0.3 + w, p0 = 1.0 / p->gas.gamma;
+    U[0 * p->Ntot + i] = rho;
+    U[1 * p->Ntot + i] = rho * u0;
+    U[(size_t)(p->neq - 1) * p->Ntot + i] = p0 / p->gas.gammaM1 + 0.5 * rho * u0 * u0;
+  }
+  const double dt1 = 0.0, pref = 1.0 / p->gas.gamma;
+  HIP_TRY(hipMemcpyAsync(p->d_J, J.data(), J.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_adj, adj.data(), adj.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_elem_case, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_U, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_case_dt, &dt1, sizeof dt1, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_case_pref, &pref, sizeof pref, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+
+  const bool tune_tile = p->d.tile < 0 && p->tk && p->cd.flow_model == THESEUS_FLOW_EULER;
+  const bool tune_xcd = p->d.xcd_remap < 0;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_tile = p->tile, best_xcd = p->xcd;
+  for (int t = 0; t < 2; ++t) {
+    if (!tune_tile && t != p->tile) continue;
+    for (int x = 0; x < 2; ++x) {
+      if (!tune_xcd && x != p->xcd) continue;
+      p->tile = t;
+      p->xcd = x;
+      RHSArgs a = p->args(p->d_U, p->d_stage[0]);
+      for (int it = 0; it < 3; ++it) p->launch(a, s);
+      hipEventRecord(e0, s);
+      for (int it = 0; it < 8; ++it) p->launch(a, s);
+      hipEventRecord(e1, s);
+      HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        best_tile = t;
+        best_xcd = x;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  HIP_TRY(hipGetLastError());
+  p->tile = best_tile;
+  p->xcd = best_xcd;
+  return pool_reset_slots(p, 0, ne, s);
+}
+
+extern "C" {
+
+int theseus_pool_create(const theseus_pool_desc *desc, theseus_pool **out) {
+  if (!desc || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null descriptor or output");
+  *out = nullptr;
+  const theseus_pool_desc &d = *desc;
+  if (d.dim != 2 && d.dim != 3) return fail(THESEUS_ERR_INVALID_ARGUMENT, "dim must be 2 or 3");
+  if (d.order < 1 || d.order > 7) return fail(THESEUS_ERR_INVALID_ARGUMENT, "order must be in [1,7]");
+  if (d.flow_model != THESEUS_FLOW_EULER && d.flow_model != THESEUS_FLOW_NAVIER_STOKES)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "unknown flow_model");
+  if (d.precision != THESEUS_PRECISION_FP64 && d.precision != THESEUS_PRECISION_MIXED)
+    return fail(THESEUS_ERR_UNSUPPORTED, "precision must be THESEUS_PRECISION_FP64 or THESEUS_PRECISION_MIXED");
+  if (!(d.gamma > 1.0) || !std::isfinite(d.gamma)) return fail(THESEUS_ERR_INVALID_ARGUMENT, "gamma must be > 1");
+  if (d.max_cases < 1) return fail(THESEUS_ERR_INVALID_ARGUMENT, "max_cases must be positive");
+  if (d.tile < -1 || d.tile > 1 || d.xcd_remap < -1 || d.xcd_remap > 1)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "tile and xcd_remap must be -1, 0 or 1");
+  const FluxKernels *fk = find_flux(d.dim, d.order, d.volume_flux, d.surface_flux);
+  const ShapeKernels *sk = find_shape(d.dim, d.order);
+  if (!fk || !sk) return fail(THESEUS_ERR_UNSUPPORTED, "no kernel compiled for this dim/order/flux pairing");
+  const TileKernels *tk = find_tiles(d.dim, d.order, d.volume_flux, d.surface_flux);
+  if (d.tile == 1 && (!tk || d.flow_model != THESEUS_FLOW_EULER))
+    return fail(THESEUS_ERR_UNSUPPORTED, "tile 1 is compiled only for 3-D Euler, Ranocha + Lax-Friedrichs, p = 2..4");
+
+  theseus_pool *p = new theseus_pool();
+  p->d = d;
+  std::memset(&p->cd, 0, sizeof(p->cd));
+  p->cd.dim = d.dim;
+  p->cd.order = d.order;
+  p->cd.flow_model = d.flow_model;
+  p->cd.volume_flux = d.volume_flux;
+  p->cd.surface_flux = d.surface_flux;
+  p->cd.precision = d.precision;
+  p->cd.blending = d.blending;
+  p->cd.gamma = d.gamma;
+  p->cd.Pr = d.Pr;
+  p->cd.R_gas = d.R_gas;
+  p->cd.mu = d.mu;
+  p->cd.mu_bulk = d.mu_bulk;
+  p->cd.alpha_max = d.alpha_max;
+  p->cd.alpha_min = d.alpha_min;
+  p->fk = fk;
+  p->sk = sk;
+  p->tk = d.flow_model == THESEUS_FLOW_EULER ? tk : nullptr;
+  p->Np = d.order + 1;
+  p->ndof = d.dim == 2 ? p->Np * p->Np : p->Np * p->Np * p->Np;
+  p->nfp = d.dim == 2 ? p->Np : p->Np * p->Np;
+  p->neq = d.dim + 2;
+  p->pm = d.precision == THESEUS_PRECISION_MIXED ? 1 : 0;
+  p->tile = d.tile == 1 ? 1 : 0;
+  p->xcd = d.xcd_remap == 0 ? 0 : 1;
+  // capacity from the budget, a multiple of 16 elements (block counts divisible by 8 for either tile)
+  const size_t per = pool_bytes_per_element(d, p->ndof, p->nfp);
+  const long long cap = d.hbm_budget_bytes > 0 ? (long long)(d.hbm_budget_bytes / (long long)per) : 0;
+  if (cap < 16) {
+    delete p;
+    return fail(THESEUS_ERR_CAPACITY, "hbm_budget_bytes holds fewer than 16 elements");
+  }
+  if (cap * (long long)p->ndof > (1LL << 31) / 2) {
+    delete p;
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "hbm_budget_bytes exceeds the pool's 32-bit element indexing");
+  }
+  p->cap = (int)(cap / 16 * 16);
+  p->Ntot = (long long)p->cap * p->ndof;
+  std::vector<double> x, w;
+  gll01(d.order, x, w);
+  derivative_ops(d.order, x, w, p->basis);
+  legendre_vandermonde_inverse(d.order, x, p->basis.Vinv);
+  init_gas(p->cd, p->gas, p->gas32);
+  p->adv_scale = advection_scale(d.order);
+  p->diff_scale = diffusion_scale(d.order);
+  p->surf_scale = p->adv_scale / (1.0 / w[0]);
+  p->cases.resize(d.max_cases);
+  p->slot_used.assign(p->cap, 0);
+
+  int rc = THESEUS_OK;
+  const size_t M = (size_t)p->Ntot * p->neq;
+  const int dim = d.dim;
+  if ((rc = upload<double>(&p->d_J, nullptr, p->cap)) || (rc = upload<double>(&p->d_adj, nullptr, (size_t)p->cap * dim * dim)) ||
+      (rc = upload<int>(&p->d_nbr, nullptr, (size_t)p->cap * 2 * dim)) ||
+      (rc = upload<int>(&p->d_elem_case, nullptr, p->cap)) || (rc = upload<double>(&p->d_alpha, nullptr, p->cap)) ||
+      (rc = upload<double>(&p->d_case_dt, nullptr, d.max_cases)) ||
+      (rc = upload<double>(&p->d_case_pref, nullptr, d.max_cases)) ||
+      (rc = upload<unsigned long long>(&p->d_rates, nullptr, 4)) || (rc = upload<double>(&p->d_U, nullptr, M)) ||
+      (rc = upload<double>(&p->d_stage[0], nullptr, M)) || (rc = upload<double>(&p->d_stage[1], nullptr, M))) {
+    theseus_pool_destroy(p);
+    return rc;
+  }
+  if (d.flow_model == THESEUS_FLOW_NAVIER_STOKES &&
+      (rc = upload<double>(&p->d_Fv, nullptr, (size_t)(dim + 1) * ((size_t)p->Ntot + (size_t)p->cap * 2 * dim * p->nfp)))) {
+    theseus_pool_destroy(p);
+    return rc;
+  }
+  hipError_t e = hipStreamCreate(&p->capture_stream);  // default priority
+  if (e != hipSuccess) {
+    theseus_pool_destroy(p);
+    return fail(THESEUS_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  hipStream_t s = p->capture_stream;
+  if ((rc = pool_reset_slots(p, 0, p->cap, s))) {
+    theseus_pool_destroy(p);
+    return rc;
+  }
+  // tile shape: remembered per (dim, order, pairing, precision), else timed now
+  if (d.tile < 0 || d.xcd_remap < 0) {
+    const auto key = std::make_tuple(d.dim, d.order, d.volume_flux, d.surface_flux, d.precision * 2 + (d.flow_model == THESEUS_FLOW_NAVIER_STOKES));
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    auto it = tune_cache().find(key);
+    if (it != tune_cache().end()) {
+      if (d.tile < 0) p->tile = it->second.first;
+      if (d.xcd_remap < 0) p->xcd = it->second.second;
+    } else {
+      if ((rc = pool_autotune(p, s))) {
+        theseus_pool_destroy(p);
+        return rc;
+      }
+      tune_cache()[key] = {p->tile, p->xcd};
+    }
+  }
+  *out = p;
+  return THESEUS_OK;
+}
+
+int theseus_pool_destroy(theseus_pool *p) {
+  if (!p) return THESEUS_OK;
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  if (p->capture_stream) hipStreamDestroy(p->capture_stream);
+  hipFree(p->d_J);
+  hipFree(p->d_adj);
+  hipFree(p->d_nbr);
+  hipFree(p->d_elem_case);
+  hipFree(p->d_alpha);
+  hipFree(p->d_case_dt);
+  hipFree(p->d_case_pref);
+  hipFree(p->d_rates);
+  hipFree(p->d_U);
+  hipFree(p->d_stage[0]);
+  hipFree(p->d_stage[1]);
+  hipFree(p->d_Fv);
+  delete p;
+  return THESEUS_OK;
+}
+
+int theseus_pool_info(const theseus_pool *p, int *capacity_elements, int *free_elements, int *largest_free_range,
+                      int *active_cases, int *tile, int *xcd_remap) {
+  if (!p) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null pool");
+  int fr = 0, run = 0, best = 0, act = 0;
+  for (int e = 0; e < p->cap; ++e) {
+    if (p->slot_used[e]) {
+      run = 0;
+    } else {
+      ++fr;
+      best = std::max(best, ++run);
+    }
+  }
+  for (const PoolCase &c : p->cases) act += c.active;
+  if (capacity_elements) *capacity_elements = p->cap;
+  if (free_elements) *free_elements = fr;
+  if (largest_free_range) *largest_free_range = best;
+  if (active_cases) *active_cases = act;
+  if (tile) *tile = p->tile;
+  if (xcd_remap) *xcd_remap = p->xcd;
+  return THESEUS_OK;
+}
+
+int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double *u0, double cfl, int *case_id,
+                       double *dt_out, void *stream) {
+  if (!p || !c || !u0) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null pool, case or state");
+  if (!(cfl > 0) || !std::isfinite(cfl)) return fail(THESEUS_ERR_INVALID_ARGUMENT, "cfl must be positive");
+  const theseus_case_desc &d = *c;
+  if (d.dim != p->cd.dim || d.order != p->cd.order || d.flow_model != p->cd.flow_model ||
+      d.volume_flux != p->cd.volume_flux || d.surface_flux != p->cd.surface_flux || d.precision != p->cd.precision ||
+      (d.blending != 0) != (p->cd.blending != 0))
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "case kernel fields (dim, order, flow, fluxes, precision, blending) differ from the pool's");
+  if (d.num_elements < 1 || !d.elem_jac || !d.elem_adj || !d.face_neighbor)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "case needs elements, geometry and connectivity");
+  bool has_boundary = false;
+  int rc = validate_mesh(d, has_boundary);
+  if (rc) return rc;
+  int cid = -1;
+  for (int i = 0; i < (int)p->cases.size(); ++i)
+    if (!p->cases[i].active) {
+      cid = i;
+      break;
+    }
+  if (cid < 0) return fail(THESEUS_ERR_CAPACITY, "all case slots are in use");
+  const int ne = d.num_elements, dim = d.dim;
+  const int start = pool_find_range(p, ne);
+  if (start < 0) return fail(THESEUS_ERR_CAPACITY, "no free range of " + std::to_string(ne) + " elements");
+  hipStream_t s = (hipStream_t)stream;
+  // geometry and connectivity at the slot offset (boundary codes kept)
+  std::vector<int> nbr((size_t)ne * 2 * dim), owner(ne, cid);
+  for (size_t f = 0; f < nbr.size(); ++f) nbr[f] = d.face_neighbor[f] >= 0 ? d.face_neighbor[f] + start : d.face_neighbor[f];
+  HIP_TRY(hipMemcpyAsync(p->d_J + start, d.elem_jac, ne * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_adj + (size_t)start * dim * dim, d.elem_adj, (size_t)ne * dim * dim * sizeof(double),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_nbr + (size_t)start * 2 * dim, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  const long long nd = (long long)ne * p->ndof, off = (long long)start * p->ndof;
+  for (int q = 0; q < p->neq; ++q)
+    HIP_TRY(hipMemcpyAsync(p->d_U + q * p->Ntot + off, u0 + q * nd, nd * sizeof(double), hipMemcpyDefault, s));
+  // stability estimate of the case alone (its element range), then dt and the background pressure
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p->d_rates, 3);
+  StabArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.U = p->d_U;
+  a.rates = p->d_rates;
+  a.Ntot = p->Ntot;
+  a.e0 = start;
+  a.ne = ne;
+  a.viscous = d.flow_model == THESEUS_FLOW_NAVIER_STOKES;
+  a.adv_scale = p->adv_scale;
+  a.diff_scale = p->diff_scale;
+  a.surf_scale = p->surf_scale;
+  a.J = p->d_J;
+  a.adj = p->d_adj;
+  a.nbr = p->d_nbr;
+  a.gas = p->gas;
+  a.w0 = p->basis.w[0];
+  p->sk->stability(a, (ne + p->sk->epb - 1) / p->sk->epb, s);
+  if (dim == 2) hipLaunchKernelGGL(k_pref<2>, dim3(1), dim3(64), 0, s, p->d_U + off, p->Ntot, p->gas, p->d_case_pref + cid);
+  else hipLaunchKernelGGL(k_pref<3>, dim3(1), dim3(64), 0, s, p->d_U + off, p->Ntot, p->gas, p->d_case_pref + cid);
+  HIP_TRY(hipGetLastError());
+  unsigned long long r[3];
+  HIP_TRY(hipMemcpyAsync(r, p->d_rates, sizeof r, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  double v[3];
+  std::memcpy(v, r, sizeof v);
+  const double total = std::max(v[0], v[1]) + v[2];
+  if (!(total > 0) || !std::isfinite(total)) return fail(THESEUS_ERR_NONFINITE, "stability estimate of the case is not positive and finite");
+  const double dt = cfl / total;
+  HIP_TRY(hipMemcpyAsync(p->d_case_dt + cid, &dt, sizeof dt, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_elem_case + start, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int e = 0; e < ne; ++e) p->slot_used[start + e] = 1;
+  p->cases[cid].start = start;
+  p->cases[cid].ne = ne;
+  p->cases[cid].active = true;
+  if (case_id) *case_id = cid;
+  if (dt_out) *dt_out = dt;
+  return THESEUS_OK;
+}
+
+int theseus_pool_step(theseus_pool *p, void *stream) {
+  if (!p) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null pool");
+  if (!p->exec) {
+    hipStream_t cs = p->capture_stream;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    p->enqueue_step(cs);
+    HIP_TRY(hipStreamEndCapture(cs, &p->graph));
+    HIP_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipGraphLaunch(p->exec, (hipStream_t)stream));
+  return THESEUS_OK;
+}
+
+int theseus_pool_read(theseus_pool *p, int case_id, double *u, void *stream) {
+  if (!p || !u) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null pool or output");
+  if (case_id < 0 || case_id >= (int)p->cases.size() || !p->cases[case_id].active)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "no such active case");
+  const PoolCase &c = p->cases[case_id];
+  const long long nd = (long long)c.ne * p->ndof, off = (long long)c.start * p->ndof;
+  hipStream_t s = (hipStream_t)stream;
+  for (int q = 0; q < p->neq; ++q)
+    HIP_TRY(hipMemcpyAsync(u + q * nd, p->d_U + q * p->Ntot + off, nd * sizeof(double), hipMemcpyDefault, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return THESEUS_OK;
+}
+
+int theseus_pool_release(theseus_pool *p, int case_id, void *stream) {
+  if (!p) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null pool");
+  if (case_id < 0 || case_id >= (int)p->cases.size() || !p->cases[case_id].active)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "no such active case");
+  PoolCase &c = p->cases[case_id];
+  int rc = pool_reset_slots(p, c.start, c.ne, (hipStream_t)stream);
+  if (rc) return rc;
+  for (int e = 0; e < c.ne; ++e) p->slot_used[c.start + e] = 0;
+  c = PoolCase();
   return THESEUS_OK;
 }
 

@@ -1,4 +1,5 @@
-// Kernel instantiations: Ranocha volume + Lax-Friedrichs face (3-D Taylor-Green workload).
+// Kernel instantiations: Ranocha volume + Lax-Friedrichs face (3-D Taylor-Green
+// workload), plus the tile-shape variants the serving pool tunes over.
 #include "registry.hpp"
 
 namespace thx {
@@ -13,5 +14,9 @@ struct Orders {
 };
 Orders<2> orders2;
 Orders<3> orders3;
+// tile-shape variants for the serving pool (3-D, the orders it serves)
+TileRegistrar<3, 2, 5, 2> tiles2;
+TileRegistrar<3, 2, 5, 3> tiles3;
+TileRegistrar<3, 2, 5, 4> tiles4;
 }  // namespace
 }  // namespace thx

@@ -25,14 +25,24 @@ struct ShapeKernels {
   int tpb = 0, epb = 0;
 };
 
+// Tile shapes of the Euler kernel the serving pool's autotuner chooses from:
+// [tile][precision], tile 0 = the default elements per workgroup, tile 1 = twice that.
+struct TileKernels {
+  RHSLaunch euler[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int epb[2] = {0, 0};
+  int tpb[2] = {0, 0};
+};
+
 void register_flux(int dim, int p, int vol, int surf, FluxKernels k);
+void register_tiles(int dim, int p, int vol, int surf, TileKernels k);
+const TileKernels *find_tiles(int dim, int p, int vol, int surf);
 void register_shape(int dim, int p, ShapeKernels k);
 const FluxKernels *find_flux(int dim, int p, int vol, int surf);
 const ShapeKernels *find_shape(int dim, int p);
 
-template <typename R, int DIM, int P, int VOL, int SURF, bool VISC>
+template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX = 0>
 void launch_rhs(const RHSArgs &a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_rhs<R, DIM, P, VOL, SURF, VISC>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+  hipLaunchKernelGGL((k_rhs<R, DIM, P, VOL, SURF, VISC, EPBX>), dim3(nb), dim3(Shape<DIM, P, EPBX>::TPB), 0, s, a);
 }
 template <typename R, int DIM, int P>
 void launch_ns_grad(const RHSArgs &a, int nb, hipStream_t s) {
@@ -52,6 +62,23 @@ struct FluxRegistrar {
     k.euler[1] = &launch_rhs<float, DIM, P, VOL, SURF, false>;
     k.ns[1] = &launch_rhs<float, DIM, P, VOL, SURF, true>;
     register_flux(DIM, P, VOL, SURF, k);
+  }
+};
+
+template <int DIM, int VOL, int SURF, int P>
+struct TileRegistrar {
+  TileRegistrar() {
+    constexpr int E0 = Shape<DIM, P>::EPB, E1 = 2 * Shape<DIM, P>::EPB;
+    TileKernels k;
+    k.euler[0][0] = &launch_rhs<double, DIM, P, VOL, SURF, false>;
+    k.euler[0][1] = &launch_rhs<float, DIM, P, VOL, SURF, false>;
+    k.euler[1][0] = &launch_rhs<double, DIM, P, VOL, SURF, false, E1>;
+    k.euler[1][1] = &launch_rhs<float, DIM, P, VOL, SURF, false, E1>;
+    k.epb[0] = E0;
+    k.epb[1] = E1;
+    k.tpb[0] = Shape<DIM, P>::TPB;
+    k.tpb[1] = Shape<DIM, P, E1>::TPB;
+    register_tiles(DIM, P, VOL, SURF, k);
   }
 };
 

@@ -29,13 +29,15 @@ namespace thx {
 
 constexpr int MAXNP = 8;
 
-template <int DIM, int P>
+// EPBX > 0 overrides the elements per workgroup (a tile shape the serving
+// pool's autotuner can choose).
+template <int DIM, int P, int EPBX = 0>
 struct Shape {
   static constexpr int Np = P + 1;
   static constexpr int NDOF = DIM == 2 ? Np * Np : Np * Np * Np;
   static constexpr int NFP = DIM == 2 ? Np : Np * Np;
   static constexpr int NEQ = DIM + 2;
-  static constexpr int EPB = NDOF >= 64 ? 1 : 64 / NDOF;  // elements per workgroup
+  static constexpr int EPB = EPBX > 0 ? EPBX : (NDOF >= 64 ? 1 : 64 / NDOF);  // elements per workgroup
   static constexpr int TPB = ((EPB * NDOF + 63) / 64) * 64;
   static constexpr int NPOW2 = NDOF <= 8 ? 8 : NDOF <= 16 ? 16 : NDOF <= 32 ? 32 : NDOF <= 64 ? 64 : NDOF <= 128 ? 128 : 256;
 };
@@ -69,6 +71,14 @@ struct RHSArgs {
   GasC<double> gas;
   GasC<float> gas32;  // the same constants rounded to FP32 (mixed precision)
   const double *pref; // mixed precision: background pressure left out of the momentum fluxes (device scalar)
+  // Serving pool: many cases in one element array.  elem_case[e] is the case
+  // owning element slot e (< 0: free slot, skipped); case_dt and case_pref
+  // hold each case's time step and background pressure.  All null for a
+  // single case (dt and pref above apply).
+  const int *elem_case;
+  const double *case_dt;
+  const double *case_pref;
+  int xcd_remap;      // block order (logical_block)
   BasisArgs basis;
 };
 
@@ -107,10 +117,11 @@ __device__ __forceinline__ int face_index(int d, int i, int j, int k) {
 __device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // XCD-aware block order: the 8 XCDs each receive a contiguous range of
-// elements so that face neighbours share an L2.
-__device__ __forceinline__ int logical_block() {
+// elements so that face neighbours share an L2 (hardware dispatches
+// workgroups round-robin over the XCDs).  `remap` = 0 keeps the launch order.
+__device__ __forceinline__ int logical_block(int remap) {
   const int nb = gridDim.x, b = blockIdx.x;
-  if ((nb & 7) == 0) return (b & 7) * (nb >> 3) + (b >> 3);
+  if (remap && (nb & 7) == 0) return (b & 7) * (nb >> 3) + (b >> 3);
   return b;
 }
 
@@ -358,9 +369,9 @@ struct RhsScratch {
 // Navier-Stokes pass when VISC.  R is the type of the flux arithmetic and of
 // the LDS staging (double, or float in the mixed mode); the state, the RK
 // combination and the blending indicator are FP64 in both.
-template <typename R, int DIM, int P, int VOL, int SURF, bool VISC>
-__global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
-  using S = Shape<DIM, P>;
+template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX = 0>
+__global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
+  using S = Shape<DIM, P, EPBX>;
   using SC = RhsScratch<R, DIM, P>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NVD = DIM + 1;                // viscous terms: momentum and energy
@@ -392,14 +403,18 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
   const int node = tid - el * ND;
   const bool lane_ok = el < EPB;
   const int elc = lane_ok ? el : 0;
-  const int ebase_blk = logical_block() * EPB;
+  const int ebase_blk = logical_block(a.xcd_remap) * EPB;
   const int e = ebase_blk + el;
-  const bool valid = lane_ok && e < a.ne;
+  // serving pool: the case owning this element (free slots are skipped)
+  const int ecase = a.elem_case ? a.elem_case[min(ebase_blk + elc, a.ne - 1)] : 0;
+  if (EPB == 1 && ecase < 0) return;  // uniform over the workgroup
+  const bool valid = lane_ok && e < a.ne && ecase >= 0;
   const long long Ntot = a.Ntot;
   const long long Gtot = (long long)a.ne * NFACE;
   const R *Vd = static_cast<const R *>(a.Vd);
   const R *Gf = static_cast<const R *>(a.Gf);
-  const double pref = FP64 ? 0.0 : *a.pref;
+  const double pref = FP64 ? 0.0 : (a.case_pref ? a.case_pref[max(ecase, 0)] : *a.pref);
+  const double dt = a.case_dt ? a.case_dt[max(ecase, 0)] : a.dt;
 
   // ---- global loads, all before the first barrier and unconditional:
   //      neighbour indices first, then the element's own data, then what
@@ -710,8 +725,8 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
       else qo = q64[q];
       double o;
       if (a.mode == 0) o = r;
-      else if (a.mode == 1) o = qo + a.dt * r;
-      else o = a.cx * xb[q] + a.cy * (qo + a.dt * r);
+      else if (a.mode == 1) o = qo + dt * r;
+      else o = a.cx * xb[q] + a.cy * (qo + dt * r);
       a.out[q * Ntot + ebase] = o;
     }
   }
@@ -728,9 +743,9 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs(RHSArgs a) {
 // -0.5 n.Fv lifted with the face weight, and n.Fv at every point of the
 // element's faces for the neighbours' halves (n the canonical area vector).
 // ---------------------------------------------------------------------------
-template <typename R, int DIM, int P>
-__global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
-  using S = Shape<DIM, P>;
+template <typename R, int DIM, int P, int EPBX = 0>
+__global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs a) {
+  using S = Shape<DIM, P, EPBX>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NV = VisComp<DIM>::N;
   constexpr int NVD = DIM + 1;
@@ -756,9 +771,12 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_ns_grad(RHSArgs a) {
   const int node = tid - el * ND;
   const bool lane_ok = el < EPB;
   const int elc = lane_ok ? el : 0;
-  const int ebase_blk = logical_block() * EPB;
+  const int ebase_blk = logical_block(a.xcd_remap) * EPB;
   const int e = ebase_blk + el;
-  const bool valid = lane_ok && e < a.ne;
+  // serving pool: the case owning this element (free slots are skipped)
+  const int ecase = a.elem_case ? a.elem_case[min(ebase_blk + elc, a.ne - 1)] : 0;
+  if (EPB == 1 && ecase < 0) return;  // uniform over the workgroup
+  const bool valid = lane_ok && e < a.ne && ecase >= 0;
   const long long Ntot = a.Ntot;
   const long long Gtot = (long long)a.ne * NFACE;
 
@@ -979,7 +997,8 @@ struct StabArgs {
   const double *U;
   unsigned long long *rates;  // [3] advective, surface, diffusive (bit patterns)
   long long Ntot;
-  int ne;
+  int ne;             // elements estimated: [e0, e0 + ne)
+  int e0;
   int viscous;
   double adv_scale, diff_scale, surf_scale;
   const double *J;
@@ -1002,8 +1021,8 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
   const int tid = threadIdx.x;
   const int el = tid / ND;
   const int node = tid - el * ND;
-  const int e = blockIdx.x * EPB + el;
-  const bool valid = el < EPB && e < a.ne;
+  const int e = a.e0 + blockIdx.x * EPB + el;
+  const bool valid = el < EPB && e < a.e0 + a.ne;
   double adv = 0.0, surf = 0.0, diff = 0.0;
   if (valid) {
     const long long Ntot = a.Ntot;
