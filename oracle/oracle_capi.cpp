@@ -1,6 +1,7 @@
 // TEST INFRASTRUCTURE ONLY.  extern "C" shim over the CPU FP64 restatement
 // (oracle_physics.hpp, oracle_operator.hpp) so that pytest (ctypes) and the
 // bench's cpu_baseline leg can call it.  The product never loads this library.
+#include <algorithm>
 #include <cstdio>
 #include <exception>
 #include <string>
@@ -141,5 +142,82 @@ int oracle_ssprk3_step(void *h, double *U, double dt) {
     return -1;
   }
 }
+
+// ---- tabulated LTE gas (oracle_lte.hpp) --------------------------------
+void *oracle_lte_synthetic(int nx, int ny, double rho_lo, double rho_hi, double T_lo, double T_hi) {
+  try {
+    return new lte::Table(lte::synthetic_table(nx, ny, rho_lo, rho_hi, T_lo, T_hi));
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void *oracle_lte_from_arrays(int nx, int ny, const double *rho, const double *T, const double *e,
+                             const double *planes, const double *inv) {
+  lte::Table *t = new lte::Table();
+  t->nx = nx;
+  t->ny = ny;
+  t->rho.assign(rho, rho + nx);
+  t->T.assign(T, T + ny);
+  t->e.assign(e, e + ny);
+  t->planes.assign(planes, planes + 6 * (size_t)nx * ny);
+  t->inv.assign(inv, inv + (size_t)nx * ny);
+  return t;
+}
+
+void oracle_lte_free(void *t) { delete static_cast<lte::Table *>(t); }
+
+void oracle_lte_arrays(void *h, double *rho, double *T, double *e, double *planes, double *inv) {
+  const lte::Table &t = *static_cast<lte::Table *>(h);
+  std::copy(t.rho.begin(), t.rho.end(), rho);
+  std::copy(t.T.begin(), t.T.end(), T);
+  std::copy(t.e.begin(), t.e.end(), e);
+  std::copy(t.planes.begin(), t.planes.end(), planes);
+  std::copy(t.inv.begin(), t.inv.end(), inv);
+}
+
+// out: pressure, energy, entropy, gibbs, cv, c^2, p_rho, p_T, g_rho, g_T, H, H_rho, H_T
+int oracle_lte_eval(void *h, double rho, double T, double *out) {
+  const lte::Eval s = lte::at(*static_cast<lte::Table *>(h), rho, T);
+  if (!s.valid) return -1;
+  const double v[13] = {s.pressure(), s.energy(), s.entropy(), s.gibbs(), s.cv(), s.sound_speed_squared(),
+                        s.pressure_density_derivative(), s.pressure_temperature_derivative(),
+                        s.gibbs_density_derivative(), s.gibbs_temperature_derivative(), s.H<0, 0>(), s.H<1, 0>(),
+                        s.H<0, 1>()};
+  for (int k = 0; k < 13; ++k) out[k] = v[k];
+  return 0;
+}
+
+int oracle_lte_recover(void *h, double rho, double e, double *T, int *iterations) {
+  const lte::Recovery r = lte::temperature_from_energy(*static_cast<lte::Table *>(h), rho, e);
+  *T = r.T;
+  if (iterations) *iterations = r.iterations;
+  return r.status;
+}
+
+int oracle_lte_means(void *h, double r1, double t1, double r2, double t2, double *out2) {
+  const lte::Table &t = *static_cast<lte::Table *>(h);
+  const lte::Eval a = lte::at(t, r1, t1), b = lte::at(t, r2, t2);
+  if (!a.valid || !b.valid) return -1;
+  const lte::Means m = lte::thermodynamic_means(t, a, b);
+  out2[0] = m.density;
+  out2[1] = m.energy;
+  return 0;
+}
+
+int oracle_lte_flux(void *h, int face, int dim, const double *q1, const double *q2, const double *d1,
+                    const double *d2, double *F) {
+  try {
+    lte::lteec_flux(*static_cast<lte::Table *>(h), face != 0, dim, q1, q2, d1, d2, F);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// The case's gas becomes the table (kept alive by the caller) with the LTEEC flux.
+void oracle_case_set_lte(void *c, void *t) { static_cast<Case *>(c)->lte = static_cast<lte::Table *>(t); }
 
 }  // extern "C"

@@ -29,6 +29,8 @@
 #include <string>
 #include <vector>
 
+#include "oracle_lte.hpp"
+
 #include "../include/theseus_rhs.h"
 #include "oracle_physics.hpp"
 
@@ -160,6 +162,7 @@ struct Case {
   int flow, vol_flux, face_flux, blending;
   double alpha_max, alpha_min, sharpness_fac, modalThreshold;
   Gas gas;
+  const lte::Table *lte = nullptr;  // tabulated LTE gas with the LTEEC flux (volume and face) when set
   Basis basis;
   std::vector<double> J, adj;  // per element
   std::vector<int> nbr;        // [ne][2*dim]
@@ -326,6 +329,22 @@ struct Case {
   }
 };
 
+// Flux and sound speed of the case's gas: the ideal gas (ABI flux ids) or
+// the tabulated LTE gas with LTEEC (LTEECFlux.hpp:93-107).
+inline void CaseVolumeFlux(const Case &c, const double *q1, const double *q2, const double *m1, const double *m2,
+                           double *F) {
+  if (c.lte) lte::lteec_flux(*c.lte, false, c.dim, q1, q2, m1, m2, F);
+  else VolumeFlux(c.vol_flux, c.gas, q1, q2, m1, m2, F);
+}
+inline void CaseFaceFlux(const Case &c, const double *q1, const double *q2, const double *n, double *F) {
+  if (c.lte) lte::lteec_flux(*c.lte, true, c.dim, q1, q2, n, n, F);
+  else FaceFlux(c.face_flux, c.gas, q1, q2, n, F);
+}
+inline double CaseSoundSpeed(const Case &c, const double *q) {
+  if (c.lte) return std::sqrt(lte::evaluate_state(*c.lte, q, c.dim).sound_speed_squared());
+  return c.gas.sound_speed(q);
+}
+
 // ---------------------------------------------------------------------------
 // Volume kernels (E-vector element layout el[q*ndof + node])
 // ---------------------------------------------------------------------------
@@ -351,7 +370,7 @@ inline void VolumePoint(const Case &c, const double *el_u, int e, int point, dou
         sl[q] = el_u[q * dof + lp];
         su[q] = el_u[q * dof + up];
       }
-      VolumeFlux(c.vol_flux, c.gas, sl, su, met, met, flux);
+      CaseVolumeFlux(c, sl, su, met, met, flux);
       const double coef = c.basis.Dhat2d(me, m);
       for (int q = 0; q < neq; ++q) rate[q] += coef * flux[q];
     }
@@ -388,7 +407,7 @@ inline void FVFluxes(const Case &c, const double *el_u, int e, double *el_du) {
         for (int ii = 0; ii < Np - 1; ++ii) {
           const int id2 = id1 + stride;
           gath(id2, s2);
-          FaceFlux(c.face_flux, c.gas, s1, s2, nor, fnum);
+          CaseFaceFlux(c, s1, s2, nor, fnum);
           for (int q = 0; q < neq; ++q) dsub[q] -= fnum[q];
           for (int q = 0; q < neq; ++q) dsub[q] /= (J * qW[ii]);
           for (int q = 0; q < neq; ++q) {
@@ -550,7 +569,7 @@ inline void EulerRHS(const Case &c, const double *U, double *dudt, const double 
       const int nl = c.node_of(d, 1, t), nr = c.node_of(d, 0, t);
       c.gather(U, L, nl, qm);
       c.gather(U, R, nr, qp);
-      FaceFlux(c.face_flux, c.gas, qm, qp, n, F);
+      CaseFaceFlux(c, qm, qp, n, F);
       for (int q = 0; q < neq; ++q) {
         fd[q * N + (size_t)L * c.ndof + nl] += -wm * F[q];
         fd[q * N + (size_t)R * c.ndof + nr] += wp * F[q];
@@ -571,7 +590,7 @@ inline void EulerRHS(const Case &c, const double *U, double *dudt, const double 
     for (int t = 0; t < c.nfp; ++t) {
       const int nd = c.node_of(d, s, t);
       c.gather(U, e, nd, qm);
-      FaceFlux(c.face_flux, c.gas, qm, qm, n, F);
+      CaseFaceFlux(c, qm, qm, n, F);
       for (int q = 0; q < neq; ++q) fd[q * N + (size_t)e * c.ndof + nd] += scale * F[q];
     }
   });
@@ -745,7 +764,7 @@ inline void NSRHS(const Case &c, const double *U, double *dudt, const double *al
           gp[dd][k] = grad[dd][orr];
         }
       }
-      FaceFlux(c.face_flux, c.gas, qm, qp, n, pf);
+      CaseFaceFlux(c, qm, qp, n, pf);
       ComputeViscousFluxKernel(c.gas, qm, gm[0], gm[1], gm[2], vm);
       ComputeViscousFluxKernel(c.gas, qp, gp[0], gp[1], gp[2], vp);
       for (int k = 0; k < neq; ++k)
@@ -796,7 +815,7 @@ inline void EstimateStability(const Case &c, const double *U, double out[3]) {
           vdm += c.gas.velocity(q, pd) * value;
           mn2 += value * value;
         }
-        directional_sum += (std::abs(vdm) + c.gas.sound_speed(q) * std::sqrt(mn2)) * inverse_jacobian;
+        directional_sum += (std::abs(vdm) + CaseSoundSpeed(c, q) * std::sqrt(mn2)) * inverse_jacobian;
         metric_square_sum += mn2 * inverse_jacobian * inverse_jacobian;
       }
       adv = std::max(adv, c.adv_scale * directional_sum);
@@ -826,7 +845,7 @@ inline void EstimateStability(const Case &c, const double *U, double out[3]) {
         vp += c.gas.velocity(qp, k) * n[k];
       }
       const double nm = std::sqrt(n2);
-      const double ws = std::max(std::abs(vm) + c.gas.sound_speed(qm) * nm, std::abs(vp) + c.gas.sound_speed(qp) * nm);
+      const double ws = std::max(std::abs(vm) + CaseSoundSpeed(c, qm) * nm, std::abs(vp) + CaseSoundSpeed(c, qp) * nm);
       surf = std::max(surf, ws * std::max(std::abs(wm), std::abs(wp)) * c.surf_scale);
     }
   });
@@ -841,7 +860,7 @@ inline void EstimateStability(const Case &c, const double *U, double out[3]) {
         n2 += nk * nk;
         vn += c.gas.velocity(qm, k) * nk;
       }
-      const double ws = std::abs(vn) + c.gas.sound_speed(qm) * std::sqrt(n2);
+      const double ws = std::abs(vn) + CaseSoundSpeed(c, qm) * std::sqrt(n2);
       surf = std::max(surf, ws * std::abs(wt) * c.surf_scale);
     }
   });

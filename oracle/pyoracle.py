@@ -167,3 +167,91 @@ class OracleCase:
                 self._lib.oracle_case_destroy(self._h)
         except Exception:
             pass
+
+
+# ---- tabulated LTE gas ------------------------------------------------------
+def _lte_sigs(lib):
+    if getattr(lib, "_lte_ready", False):
+        return lib
+    vp = C.c_void_p
+    lib.oracle_lte_synthetic.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4
+    lib.oracle_lte_synthetic.restype = vp
+    lib.oracle_lte_from_arrays.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_lte_from_arrays.restype = vp
+    lib.oracle_lte_free.argtypes = [vp]
+    lib.oracle_lte_arrays.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_lte_eval.argtypes = [vp, C.c_double, C.c_double, _dp]
+    lib.oracle_lte_eval.restype = C.c_int
+    lib.oracle_lte_recover.argtypes = [vp, C.c_double, C.c_double, _dp, C.POINTER(C.c_int)]
+    lib.oracle_lte_recover.restype = C.c_int
+    lib.oracle_lte_means.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+    lib.oracle_lte_means.restype = C.c_int
+    lib.oracle_lte_flux.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_lte_flux.restype = C.c_int
+    lib.oracle_case_set_lte.argtypes = [vp, vp]
+    lib._lte_ready = True
+    return lib
+
+
+class LteTable:
+    """The oracle's Helmholtz table: the synthetic dissociating gas, or given arrays."""
+
+    EVAL_KEYS = ("pressure", "energy", "entropy", "gibbs", "cv", "c2", "p_rho", "p_T", "g_rho", "g_T", "H", "H_rho",
+                 "H_T")
+
+    def __init__(self, nx=256, ny=256, rho=(1e-4, 10.0), T=(10 ** 2.3, 10 ** 4.2), arrays=None):
+        lib = _lte_sigs(load())
+        self._lib = lib
+        if arrays is None:
+            self._h = lib.oracle_lte_synthetic(nx, ny, rho[0], rho[1], T[0], T[1])
+        else:
+            r, t, e, planes, inv = (arr(a) for a in arrays)
+            nx, ny = r.size, t.size
+            self._h = lib.oracle_lte_from_arrays(nx, ny, _p(r), _p(t), _p(e), _p(planes), _p(inv))
+        if not self._h:
+            raise RuntimeError(lib.oracle_last_error().decode())
+        self.nx, self.ny = nx, ny
+
+    def arrays(self):
+        nx, ny = self.nx, self.ny
+        r, t, e = np.zeros(nx), np.zeros(ny), np.zeros(ny)
+        planes, inv = np.zeros(6 * nx * ny), np.zeros(nx * ny)
+        self._lib.oracle_lte_arrays(self._h, _p(r), _p(t), _p(e), _p(planes), _p(inv))
+        return r, t, e, planes, inv
+
+    def eval(self, rho, T):
+        out = np.zeros(13)
+        if self._lib.oracle_lte_eval(self._h, float(rho), float(T), _p(out)) != 0:
+            return None
+        return dict(zip(self.EVAL_KEYS, out))
+
+    def recover(self, rho, e):
+        T = np.zeros(1)
+        it = C.c_int()
+        st = self._lib.oracle_lte_recover(self._h, float(rho), float(e), _p(T), C.byref(it))
+        return float(T[0]), st, it.value
+
+    def means(self, r1, t1, r2, t2):
+        out = np.zeros(2)
+        if self._lib.oracle_lte_means(self._h, r1, t1, r2, t2, _p(out)) != 0:
+            return None
+        return out
+
+    def flux(self, face, dim, q1, q2, d1, d2=None):
+        q1, q2, d1 = arr(q1), arr(q2), arr(d1)
+        d2 = d1 if d2 is None else arr(d2)
+        F = np.zeros(dim + 2)
+        if self._lib.oracle_lte_flux(self._h, int(face), dim, _p(q1), _p(q2), _p(d1), _p(d2), _p(F)) != 0:
+            raise RuntimeError(self._lib.oracle_last_error().decode())
+        return F
+
+    def attach(self, case: "OracleCase"):
+        """Make the oracle case use this table with the LTEEC flux."""
+        self._lib.oracle_case_set_lte(case._h, self._h)
+        case._lte = self
+
+    def __del__(self):
+        try:
+            self._lib.oracle_lte_free(self._h)
+        except Exception:
+            pass
