@@ -62,6 +62,7 @@ extern "C" {
 #define THESEUS_VOLUME_CHANDRASHEKAR 1 /* ChandrashekarFlux.hpp:20-89 */
 #define THESEUS_VOLUME_RANOCHA 2       /* RanochaFlux.hpp:20-79 */
 #define THESEUS_VOLUME_PIROZZOLI 3     /* PirozzoliFlux.hpp:19-71 */
+#define THESEUS_VOLUME_LTEEC 4         /* LTEECFlux.hpp:11-99, tabulated LTE gas (theseus_rhs_create_lte) */
 
 /* ---- surface (face) fluxes (A5) ---------------------------------------- */
 #define THESEUS_SURFACE_CHANDRASHEKAR 0 /* EC + Scalar_dissipation, ChandrashekarFlux.hpp:91-155 */
@@ -70,6 +71,7 @@ extern "C" {
 #define THESEUS_SURFACE_ROE 3           /* RoeFlux.hpp:50-157 */
 #define THESEUS_SURFACE_HLL 4           /* HLLFlux.hpp:53-116 */
 #define THESEUS_SURFACE_LLF 5           /* LaxFriedrichsFlux.hpp:54-77 */
+#define THESEUS_SURFACE_LTEEC 6         /* LTEEC + Lax dissipation, LTEECFlux.hpp:11-99, DissipationOperator.hpp:15-28 */
 
 /* ---- boundary condition types (bc_cache_utilities.hpp:11-23) ------------ */
 #define THESEUS_BC_SUPERSONIC_OUTFLOW 2 /* F*(q,q,n), bc_kernels.hpp:236-238 */
@@ -137,6 +139,26 @@ typedef struct theseus_stability {
 /* Replaces MakeRHSOperator + RHSOperator::Finalize (SimFactory_impl.hpp:63-488,
  * RHSOperator_impl.hpp:273-314).  Allocates the device-resident cache. */
 THESEUS_API int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out);
+
+/* Six-plane Helmholtz table of a tabulated LTE gas (LTEHelmholtzTable.hpp:13-31,
+ * LTETable.hpp:65-72): grids and planes as the reference's HelmholtzTable::Data
+ * holds them (host pointers, copied by create). */
+typedef struct theseus_lte_table {
+  int nx, ny;
+  const double *rho_grid; /* [nx] strictly increasing, > 0 */
+  const double *T_grid;   /* [ny] strictly increasing, > 0 */
+  const double *e_grid;   /* [ny] strictly increasing: the inverse table's energy axis */
+  const double *planes;   /* [6][ny][nx]: H, H_rho, H_T, H_rhoT, mu, lambda */
+  const double *inverse;  /* [ny][nx]: temperature guess at (rho_i, e_j) */
+} theseus_lte_table;
+
+/* MakeRHSOperator for the LTEHelmholtzGas with the LTEEC flux
+ * (LTEGasModel.hpp:339, LTEECFlux.hpp:93-107): FP64 Euler, volume_flux =
+ * THESEUS_VOLUME_LTEEC, surface_flux = THESEUS_SURFACE_LTEEC.  A state
+ * outside the table (or whose temperature recovery does not converge)
+ * yields non-finite values instead of the reference's trap. */
+THESEUS_API int theseus_rhs_create_lte(const theseus_case_desc *desc, const theseus_lte_table *table,
+                                       theseus_rhs **out);
 
 /* Replaces RHSOperatorBase::~RHSOperatorBase. */
 THESEUS_API int theseus_rhs_destroy(theseus_rhs *h);

@@ -137,3 +137,40 @@ def test_box_mesh_connectivity_and_geometry():
     h = np.array([0.5, 0.5, 0.5])
     np.testing.assert_allclose(m.elem_jac, np.prod(h))
     np.testing.assert_allclose(m.elem_adj.reshape(ne, 3, 3)[0], np.diag(np.prod(h) / h))
+
+
+def _lte_arrays(nx=4, ny=4):
+    rho = np.linspace(0.5, 2.0, nx)
+    T = np.linspace(200.0, 400.0, ny)
+    e = np.linspace(0.0, 1e5, ny)
+    planes = np.ones(6 * nx * ny)
+    inv = np.full(nx * ny, 300.0)
+    return rho, T, e, planes, inv
+
+
+@pytest.mark.parametrize("mutate,code", [
+    ("blending", A.ERR_UNSUPPORTED),
+    ("ideal_flux", A.ERR_INVALID_ARGUMENT),
+    ("bad_grid", A.ERR_INVALID_ARGUMENT),
+    ("ns", A.ERR_UNSUPPORTED),
+])
+def test_create_lte_rejects_invalid_inputs(mutate, code):
+    """theseus_rhs_create_lte validates before any device work."""
+    lib = A.load()
+    rt = RunTime(dim=3, order=3, flow_model=A.FLOW_EULER, volume_flux=A.VOLUME_LTEEC,
+                 surface_flux=A.SURFACE_LTEEC, blending=mutate == "blending")
+    if mutate == "ideal_flux":
+        rt.volume_flux = A.VOLUME_RANOCHA
+    if mutate == "ns":
+        rt.flow_model = A.FLOW_NAVIER_STOKES
+    mesh = box_mesh(3, (2, 2, 2), 0.0, 1.0)
+    d, keep = build_descriptor(rt, mesh)
+    arrays = [np.ascontiguousarray(a) for a in _lte_arrays()]
+    if mutate == "bad_grid":
+        arrays[1][2] = arrays[1][1]  # T grid not strictly increasing
+    t = A.LteTable()
+    t.nx, t.ny = 4, 4
+    t.rho_grid, t.T_grid, t.e_grid, t.planes, t.inverse = (a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrays)
+    h = C.c_void_p()
+    assert lib.theseus_rhs_create_lte(C.byref(d), C.byref(t), C.byref(h)) == code
+    assert not h.value and lib.theseus_last_error().decode()

@@ -219,3 +219,85 @@ def test_lteec_is_entropy_conservative_on_the_synthetic_table(synth):
         rhs = psis[1] - psis[0]
         scale = np.abs(ws[1] - ws[0]) @ np.abs(F) + abs(psis[0]) + abs(psis[1])
         assert abs(lhs - rhs) <= 1e-9 * scale, (k, lhs, rhs)
+
+
+def _lte_case(n, order, table):
+    from theseus_amd.cases import c4_lte_tgv
+    from theseus_amd.operator import build_descriptor
+    from oracle.pyoracle import OracleCase
+
+    rt, mesh, U, _ = c4_lte_tgv(n=n, order=order, table=table)
+    desc, keep = build_descriptor(rt, mesh)
+    c = OracleCase(desc)
+    c._keep = keep
+    ot = LteTable(arrays=(table.rho, table.T, table.e, table.planes.reshape(-1), table.inv.reshape(-1)))
+    ot.attach(c)
+    return rt, mesh, U, c
+
+
+@pytest.fixture(scope="module")
+def small_table():
+    from theseus_amd.lte import synthetic_table
+
+    return synthetic_table(64, 64)
+
+
+def test_product_table_builder_matches_the_oracle_builder():
+    """theseus_amd.lte.synthetic_table (the engine's input) and the oracle's
+    builder agree: grids and forward planes bitwise, inverse guesses to 1e-12."""
+    from theseus_amd.lte import synthetic_table
+
+    t = synthetic_table(48, 40)
+    o = LteTable(48, 40)
+    r, T, e, planes, inv = o.arrays()
+    assert np.array_equal(t.rho, r) and np.array_equal(t.T, T) and np.array_equal(t.e, e)
+    np.testing.assert_allclose(t.planes.reshape(-1), planes, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(t.inv.reshape(-1), inv, rtol=1e-12)
+
+
+def test_lte_operator_free_stream_and_conservation(small_table):
+    """Uniform flow gives dU = 0 to round-off; on a periodic box the
+    w_i J_i-weighted sum of dU vanishes (LTEEC is conservative)."""
+    from theseus_amd.mesh import gll_nodes
+
+    rt, mesh, U, c = _lte_case(2, 3, small_table)
+    N = mesh.num_elements * 64
+    q = U.reshape(5, -1)[:, 1].copy()
+    Uc = np.repeat(q, N)
+    r = c.rhs(Uc)
+    assert np.abs(r).max() < 1e-9 * np.abs(q).max()
+    r = c.rhs(U).reshape(5, -1)
+    _, w = gll_nodes(3)
+    W = np.einsum("i,j,k->kji", w, w, w).reshape(-1)
+    Wall = np.tile(W, mesh.num_elements) * np.repeat(mesh.elem_jac, 64)
+    for k in range(5):
+        assert abs(r[k] @ Wall) <= 1e-11 * (np.abs(r[k]) @ Wall + 1e-300), k
+
+
+def test_lte_operator_conserves_entropy_in_the_volume(small_table):
+    """Entropy stability of the assembled LTEEC operator on a periodic box:
+    with entropy variables w = ((g - |u|^2/2)/T, u/T, -1/T) of the table,
+    the discrete entropy production sum_i w_i J_i (w(U_i) . dU_i) is <= 0
+    (the volume terms conserve entropy, the Lax-Friedrichs faces dissipate)."""
+    from theseus_amd.mesh import gll_nodes
+
+    rt, mesh, U, c = _lte_case(2, 3, small_table)
+    t = LteTable(arrays=(small_table.rho, small_table.T, small_table.e, small_table.planes.reshape(-1),
+                         small_table.inv.reshape(-1)))
+    Uq = U.reshape(5, -1)
+    r = c.rhs(U).reshape(5, -1)
+    _, w = gll_nodes(3)
+    W = np.einsum("i,j,k->kji", w, w, w).reshape(-1)
+    Wall = np.tile(W, mesh.num_elements) * np.repeat(mesh.elem_jac, 64)
+    prod = 0.0
+    for nidx in range(Uq.shape[1]):
+        q = Uq[:, nidx]
+        u = q[1:4] / q[0]
+        e = (q[4] - 0.5 * q[0] * u @ u) / q[0]
+        T, st, _ = t.recover(q[0], e)
+        assert st == 0
+        v = t.eval(q[0], T)
+        wv = np.concatenate([[(v["gibbs"] - 0.5 * u @ u) / T], u / T, [-1 / T]])
+        prod += Wall[nidx] * (wv @ r[:, nidx])
+    scale = np.abs(Wall) @ np.abs(r).sum(axis=0)
+    assert prod <= 1e-10 * scale, (prod, scale)

@@ -25,6 +25,7 @@ VOLUME_CENTRAL = 0
 VOLUME_CHANDRASHEKAR = 1
 VOLUME_RANOCHA = 2
 VOLUME_PIROZZOLI = 3
+VOLUME_LTEEC = 4
 
 SURFACE_CHANDRASHEKAR = 0
 SURFACE_RANOCHA = 1
@@ -32,6 +33,7 @@ SURFACE_PIROZZOLI = 2
 SURFACE_ROE = 3
 SURFACE_HLL = 4
 SURFACE_LLF = 5
+SURFACE_LTEEC = 6
 
 BC_SUPERSONIC_OUTFLOW = 2
 
@@ -40,6 +42,7 @@ PRECISION_MIXED = 1
 
 EXPORTED_SYMBOLS = (
     "theseus_rhs_create",
+    "theseus_rhs_create_lte",
     "theseus_rhs_destroy",
     "theseus_rhs_sizes",
     "theseus_rhs_eval",
@@ -84,6 +87,18 @@ class CaseDesc(C.Structure):
     ]
 
 
+class LteTable(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("rho_grid", C.POINTER(C.c_double)),
+        ("T_grid", C.POINTER(C.c_double)),
+        ("e_grid", C.POINTER(C.c_double)),
+        ("planes", C.POINTER(C.c_double)),
+        ("inverse", C.POINTER(C.c_double)),
+    ]
+
+
 class PoolDesc(C.Structure):
     _fields_ = [
         ("dim", C.c_int),
@@ -125,6 +140,8 @@ def load(path: Path | str | None = None) -> C.CDLL:
     lib = C.CDLL(str(p))
     lib.theseus_rhs_create.argtypes = [C.POINTER(CaseDesc), C.POINTER(C.c_void_p)]
     lib.theseus_rhs_create.restype = C.c_int
+    lib.theseus_rhs_create_lte.argtypes = [C.POINTER(CaseDesc), C.POINTER(LteTable), C.POINTER(C.c_void_p)]
+    lib.theseus_rhs_create_lte.restype = C.c_int
     lib.theseus_rhs_destroy.argtypes = [C.c_void_p]
     lib.theseus_rhs_destroy.restype = C.c_int
     lib.theseus_rhs_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]

@@ -126,3 +126,42 @@ def c3_riemann2d(n: int = 128, order: int = 4):
     rt = RunTime(dim=2, order=order, flow_model=A.FLOW_EULER, volume_flux=A.VOLUME_CENTRAL,
                  surface_flux=A.SURFACE_HLL, blending=True)
     return rt, mesh, four_quadrant_riemann(mesh, order, rt.gamma)
+
+
+def lte_taylor_green(mesh: BoxMesh, order: int, table, rho0: float = 0.1, T0: float = 3500.0, mach: float = 0.3):
+    """Taylor-Green vortex of the tabulated LTE gas: rho = rho0, velocity
+    amplitude U = mach * c(rho0, T0) with c^2 = gamma_eff R T of the synthetic
+    gas, pressure p0 + rho0 U^2/16 (cos 2x + cos 2y)(cos 2z + 2) with
+    p0 = rho0 R T0; T = p/(rho R) (the synthetic gas's pressure H_rho rho^2 =
+    rho R T) and the energy from the table's interpolant."""
+    from .lte import R_AIR, SyntheticGas, _energy_at
+
+    gas = SyntheticGas()
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    x, y, z = X[0], X[1], X[2]
+    gam = 1.0 + R_AIR / gas.cv(T0)
+    U = mach * np.sqrt(gam * R_AIR * T0)
+    u = U * np.sin(x) * np.cos(y) * np.cos(z)
+    v = -U * np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(x)
+    rho = np.full_like(x, rho0)
+    p = rho0 * R_AIR * T0 + rho0 * U * U / 16.0 * (np.cos(2.0 * x) + np.cos(2.0 * y)) * (np.cos(2.0 * z) + 2)
+    T = p / (rho0 * R_AIR)
+    i = int(np.searchsorted(table.rho, rho0, side="right") - 1)
+    e = _energy_at(table, min(i, table.nx - 2), rho0, T)
+    E = rho * e + 0.5 * rho * (u * u + v * v + w * w)
+    return _pack(rho, [rho * u, rho * v, rho * w], E)
+
+
+def c4_lte_tgv(n: int = 24, order: int = 3, table=None):
+    """3-D LTE Taylor-Green vortex, periodic [0,2pi]^3, LTEEC flux, the
+    synthetic dissociating-gas table (256 x 256, log rho in [-4,1], log T in
+    [2.3,4.2]), rho0 = 0.1, T0 = 3500 K, amplitude Mach 0.3."""
+    from .lte import synthetic_table
+
+    table = table if table is not None else synthetic_table()
+    mesh = box_mesh(3, n, 0.0, 2 * np.pi, periodic=True)
+    rt = RunTime(dim=3, order=order, flow_model=A.FLOW_EULER, volume_flux=A.VOLUME_LTEEC,
+                 surface_flux=A.SURFACE_LTEEC, blending=False)
+    return rt, mesh, lte_taylor_green(mesh, order, table), table

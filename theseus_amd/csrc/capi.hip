@@ -45,6 +45,15 @@ const TileKernels *find_tiles(int dim, int p, int vol, int surf) {
   auto it = tile_table().find({dim, p, vol, surf});
   return it == tile_table().end() ? nullptr : &it->second;
 }
+std::map<std::tuple<int, int>, LteKernels> &lte_table() {
+  static std::map<std::tuple<int, int>, LteKernels> t;
+  return t;
+}
+void register_lte(int dim, int p, LteKernels k) { lte_table()[{dim, p}] = k; }
+const LteKernels *find_lte(int dim, int p) {
+  auto it = lte_table().find({dim, p});
+  return it == lte_table().end() ? nullptr : &it->second;
+}
 const FluxKernels *find_flux(int dim, int p, int vol, int surf) {
   auto it = flux_table().find({dim, p, vol, surf});
   return it == flux_table().end() ? nullptr : &it->second;
@@ -234,6 +243,9 @@ struct theseus_rhs {
   long long Ntot;
   const FluxKernels *fk = nullptr;
   const ShapeKernels *sk = nullptr;
+  const LteKernels *lk = nullptr;  // tabulated LTE gas
+  lte::View ltev;                  // its device arrays
+  double *d_lte = nullptr;
   BasisArgs basis;
   GasC<double> gas;
   GasC<float> gas32;
@@ -283,6 +295,7 @@ struct theseus_rhs {
     a.gas32 = gas32;
     a.pref = d_pref;
     a.xcd_remap = 1;
+    a.lte = ltev;
     a.basis = basis;
     return a;
   }
@@ -312,6 +325,8 @@ static int enqueue_rhs(theseus_rhs *h, RHSArgs a, hipStream_t s) {
   if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
     timed(h, 1, s, [&] { h->sk->ns_grad[h->pm](a, nb, s); });
     timed(h, 0, s, [&] { h->fk->ns[h->pm](a, nb, s); });
+  } else if (h->lk) {
+    timed(h, 0, s, [&] { h->lk->rhs(a, nb, s); });
   } else {
     timed(h, 0, s, [&] { h->fk->euler[h->pm](a, nb, s); });
   }
@@ -380,7 +395,7 @@ extern "C" {
 
 const char *theseus_last_error(void) { return g_last_error.c_str(); }
 
-int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
+static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *lt, theseus_rhs **out) {
   if (!desc || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null descriptor or output");
   *out = nullptr;
   const theseus_case_desc &d = *desc;
@@ -396,9 +411,28 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
   if (!(d.gamma > 1.0) || !std::isfinite(d.gamma)) return fail(THESEUS_ERR_INVALID_ARGUMENT, "gamma must be > 1");
   if (d.flow_model == THESEUS_FLOW_NAVIER_STOKES && (!(d.Pr > 0) || !(d.mu >= 0)))
     return fail(THESEUS_ERR_INVALID_ARGUMENT, "Navier-Stokes needs Pr > 0 and mu >= 0");
-  const FluxKernels *fk = find_flux(d.dim, d.order, d.volume_flux, d.surface_flux);
+  const LteKernels *lk = nullptr;
+  if (lt) {
+    if (d.flow_model != THESEUS_FLOW_EULER || d.precision != THESEUS_PRECISION_FP64 || d.blending)
+      return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas is implemented for FP64 Euler without blending");
+    if (d.volume_flux != THESEUS_VOLUME_LTEEC || d.surface_flux != THESEUS_SURFACE_LTEEC)
+      return fail(THESEUS_ERR_INVALID_ARGUMENT, "the tabulated LTE gas uses the LTEEC volume and surface fluxes");
+    if (lt->nx < 2 || lt->ny < 2 || !lt->rho_grid || !lt->T_grid || !lt->e_grid || !lt->planes || !lt->inverse)
+      return fail(THESEUS_ERR_INVALID_ARGUMENT, "LTE table needs nx, ny >= 2 and all arrays");
+    const double *grids[3] = {lt->rho_grid, lt->T_grid, lt->e_grid};
+    const int counts[3] = {lt->nx, lt->ny, lt->ny};
+    for (int g = 0; g < 3; ++g)
+      for (int i = 0; i < counts[g]; ++i)
+        if (!std::isfinite(grids[g][i]) || (i > 0 && !(grids[g][i] > grids[g][i - 1])))
+          return fail(THESEUS_ERR_INVALID_ARGUMENT, "LTE table grids must be finite and strictly increasing");
+    if (!(lt->rho_grid[0] > 0) || !(lt->T_grid[0] > 0))
+      return fail(THESEUS_ERR_INVALID_ARGUMENT, "LTE table density and temperature must be positive");
+    lk = find_lte(d.dim, d.order);
+    if (!lk) return fail(THESEUS_ERR_UNSUPPORTED, "no LTE kernel compiled for this dim/order");
+  }
+  const FluxKernels *fk = lt ? nullptr : find_flux(d.dim, d.order, d.volume_flux, d.surface_flux);
   const ShapeKernels *sk = find_shape(d.dim, d.order);
-  if (!fk || !sk) {
+  if ((!fk && !lk) || !sk) {
     char buf[256];
     std::snprintf(buf, sizeof buf, "no kernel compiled for dim=%d order=%d volume_flux=%d surface_flux=%d", d.dim,
                   d.order, d.volume_flux, d.surface_flux);
@@ -417,6 +451,7 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
   h->d.bc_type = nullptr;
   h->fk = fk;
   h->sk = sk;
+  h->lk = lk;
   h->Np = d.order + 1;
   h->ndof = dim == 2 ? h->Np * h->Np : h->Np * h->Np * h->Np;
   h->nfp = dim == 2 ? h->Np : h->Np * h->Np;
@@ -450,6 +485,28 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
       return rc;
     }
   }
+  if (lt) {
+    // one device block: rho, T, e grids, the six planes, the inverse table
+    const size_t nx = lt->nx, ny = lt->ny;
+    std::vector<double> host;
+    host.reserve(nx + 2 * ny + 7 * nx * ny);
+    host.insert(host.end(), lt->rho_grid, lt->rho_grid + nx);
+    host.insert(host.end(), lt->T_grid, lt->T_grid + ny);
+    host.insert(host.end(), lt->e_grid, lt->e_grid + ny);
+    host.insert(host.end(), lt->planes, lt->planes + 6 * nx * ny);
+    host.insert(host.end(), lt->inverse, lt->inverse + nx * ny);
+    if ((rc = upload(&h->d_lte, host.data(), host.size()))) {
+      theseus_rhs_destroy(h);
+      return rc;
+    }
+    h->ltev.nx = lt->nx;
+    h->ltev.ny = lt->ny;
+    h->ltev.rho = h->d_lte;
+    h->ltev.T = h->d_lte + nx;
+    h->ltev.e = h->d_lte + nx + ny;
+    h->ltev.planes = h->d_lte + nx + 2 * ny;
+    h->ltev.inv = h->d_lte + nx + 2 * ny + 6 * nx * ny;
+  }
   hipError_t e = hipMemset(h->d_alpha, 0, ne * sizeof(double));
   if (e != hipSuccess) {
     theseus_rhs_destroy(h);
@@ -457,6 +514,13 @@ int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) {
   }
   *out = h;
   return THESEUS_OK;
+}
+
+int theseus_rhs_create(const theseus_case_desc *desc, theseus_rhs **out) { return create_impl(desc, nullptr, out); }
+
+int theseus_rhs_create_lte(const theseus_case_desc *desc, const theseus_lte_table *table, theseus_rhs **out) {
+  if (!table) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null LTE table");
+  return create_impl(desc, table, out);
 }
 
 static void release_events(theseus_rhs *h) {
@@ -505,6 +569,7 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_alpha);
   hipFree(h->d_Fv);
   hipFree(h->d_pref);
+  hipFree(h->d_lte);
   hipFree(h->d_stage[0]);
   hipFree(h->d_stage[1]);
   hipFree(h->d_rates);
@@ -575,6 +640,7 @@ int theseus_rhs_estimate_stability(theseus_rhs *h, const double *u, theseus_stab
   a.nbr = h->d_nbr;
   a.gas = h->gas;
   a.w0 = h->basis.w[0];
+  a.lte = h->ltev;
   h->sk->stability(a, h->nblocks(), s);
   HIP_TRY(hipGetLastError());
   unsigned long long r[3];

@@ -1,0 +1,14 @@
+// Kernel instantiations: Euler with the tabulated LTE gas and the LTEEC flux.
+#include "registry.hpp"
+
+namespace thx {
+namespace {
+LteRegistrar<2, 2> l22;
+LteRegistrar<2, 3> l23;
+LteRegistrar<2, 4> l24;
+LteRegistrar<3, 1> l31;
+LteRegistrar<3, 2> l32;
+LteRegistrar<3, 3> l33;
+LteRegistrar<3, 4> l34;
+}  // namespace
+}  // namespace thx

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "rhs_kernels.hpp"
+#include "rhs_lte.hpp"
 
 namespace thx {
 
@@ -33,7 +34,15 @@ struct TileKernels {
   int tpb[2] = {0, 0};
 };
 
+// Euler with the tabulated LTE gas and the LTEEC flux
+struct LteKernels {
+  RHSLaunch rhs = nullptr;
+  int tpb = 0, epb = 0;
+};
+
 void register_flux(int dim, int p, int vol, int surf, FluxKernels k);
+void register_lte(int dim, int p, LteKernels k);
+const LteKernels *find_lte(int dim, int p);
 void register_tiles(int dim, int p, int vol, int surf, TileKernels k);
 const TileKernels *find_tiles(int dim, int p, int vol, int surf);
 void register_shape(int dim, int p, ShapeKernels k);
@@ -79,6 +88,22 @@ struct TileRegistrar {
     k.tpb[0] = Shape<DIM, P>::TPB;
     k.tpb[1] = Shape<DIM, P, E1>::TPB;
     register_tiles(DIM, P, VOL, SURF, k);
+  }
+};
+
+template <int DIM, int P>
+void launch_rhs_lte(const RHSArgs &a, int nb, hipStream_t s) {
+  hipLaunchKernelGGL((k_rhs_lte<DIM, P>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+}
+
+template <int DIM, int P>
+struct LteRegistrar {
+  LteRegistrar() {
+    LteKernels k;
+    k.rhs = &launch_rhs_lte<DIM, P>;
+    k.tpb = Shape<DIM, P>::TPB;
+    k.epb = Shape<DIM, P>::EPB;
+    register_lte(DIM, P, k);
   }
 };
 

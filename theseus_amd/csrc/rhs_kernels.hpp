@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_lte.hpp"
 #include "device_physics.hpp"
 
 namespace thx {
@@ -79,6 +80,7 @@ struct RHSArgs {
   const double *case_dt;
   const double *case_pref;
   int xcd_remap;      // block order (logical_block)
+  lte::View lte;      // tabulated LTE gas (LTEEC kernels)
   BasisArgs basis;
 };
 
@@ -1006,6 +1008,7 @@ struct StabArgs {
   const int *nbr;
   GasC<double> gas;
   double w0;
+  lte::View lte;  // tabulated LTE gas: sound speed from the table when lte.nx > 0
 };
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -1031,7 +1034,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
 #pragma unroll
     for (int c = 0; c < NQ; ++c) s.q[c] = a.U[c * Ntot + ebase];
     derive(g, s);
-    const double c_s = sound_speed(g, s);
+    const double c_s = a.lte.nx > 0 ? sqrt(lte::state_eval<DIM>(a.lte, s.q).c2()) : sound_speed(g, s);
     const double invJ = 1.0 / a.J[e];
     const double *adj_e = a.adj + (size_t)e * DIM * DIM;
     double dsum = 0.0, msum = 0.0;
@@ -1091,7 +1094,8 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
           double vo = 0.0;
 #pragma unroll
           for (int dd = 0; dd < DIM; ++dd) vo += o.v[dd] * n[dd];
-          ws = fmax(ws, fabs(vo) + sound_speed(g, o) * nm);
+          const double c_o = a.lte.nx > 0 ? sqrt(lte::state_eval<DIM>(a.lte, o.q).c2()) : sound_speed(g, o);
+          ws = fmax(ws, fabs(vo) + c_o * nm);
           wmax = fmax(wmax, fabs(1.0 / (a.w0 * a.J[nb])));
         }
         surf = fmax(surf, ws * wmax * a.surf_scale);

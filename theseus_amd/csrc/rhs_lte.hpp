@@ -1,0 +1,253 @@
+// Euler right-hand side of the tabulated LTE gas with the LTEEC flux
+// (LTEECFlux.hpp:93-107; EulerOperator_impl.hpp:28-421 with the
+// LTEHelmholtzGas of LTEGasModel.hpp:339), FP64.  Same element-per-
+// workgroup structure as k_rhs: the element's temperatures are recovered
+// once per node (inverse-table guess + Newton) and staged in LDS with the
+// table cell and its coordinates, so the pair fluxes only evaluate the
+// interpolant; each GLL-line pair is evaluated once; the neighbours' face
+// traces are recovered where the face flux needs them.
+#pragma once
+
+#include "rhs_kernels.hpp"
+
+namespace thx {
+
+template <int DIM, int P>
+__global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
+  using S = Shape<DIM, P>;
+  using SC = RhsScratch<double, DIM, P>;
+  constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
+  constexpr int NFACE = 2 * DIM * NF;
+  constexpr int NFR = (EPB * NFACE + TPB - 1) / TPB;
+  constexpr int NPL = SC::NPL, NPAIR = SC::NPAIR;
+
+  __shared__ double sq[EPB][NQ][ND];
+  __shared__ double sT[EPB][ND];     // recovered temperature
+  __shared__ double sp[EPB][ND];     // pressure
+  __shared__ double sc[EPB][ND];     // sound speed
+  __shared__ int scell[EPB][ND];     // table cell j*nx + i
+  __shared__ double sraw[EPB][SC::DOUBLES];
+  __shared__ double sD2[Np * Np];
+  __shared__ ElemMeta<DIM, EPB> sm;
+  auto sscr = [&](int pe, int row, int q) -> double & { return sraw[pe][row * NQ + q]; };
+
+  const lte::View &tv = a.lte;
+  const int tid = threadIdx.x;
+  const int el = tid / ND;
+  const int node = tid - el * ND;
+  const bool lane_ok = el < EPB;
+  const int elc = lane_ok ? el : 0;
+  const int ebase_blk = logical_block(a.xcd_remap) * EPB;
+  const int e = ebase_blk + el;
+  const bool valid = lane_ok && e < a.ne;
+  const long long Ntot = a.Ntot;
+
+  // ---- loads (unconditional, clamped), as in k_rhs
+  constexpr int NFT = EPB * NFACE;
+  int fnb[NFR];
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int fi = min(tid + r * TPB, NFT - 1);
+    const int pe = fi / NFACE;
+    const int ee = min(ebase_blk + pe, a.ne - 1);
+    fnb[r] = a.nbr[(size_t)ee * 2 * DIM + (fi - pe * NFACE) / NF];
+  }
+  MetaRegs<DIM, EPB, TPB> mr;
+  meta_load_index<DIM, EPB, TPB>(a, ebase_blk, mr);
+  meta_load_own<DIM, EPB, TPB>(a, ebase_blk, mr);
+  const size_t ebase = (size_t)(valid ? e : 0) * ND + node;
+  double qown[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) qown[q] = a.U[q * Ntot + ebase];
+  static_assert(Np * Np <= TPB, "one basis entry per lane");
+  const double bD2 = a.basis.Dhat2[min(tid, Np * Np - 1)];
+  meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
+  double qn[NFR][NQ];
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int fi = min(tid + r * TPB, NFT - 1);
+    const int pe = fi / NFACE;
+    const int fl = fi - pe * NFACE;
+    const int f = fl / NF, t = fl - f * NF;
+    const int d = f >> 1, s = f & 1;
+    const size_t nbase = (size_t)max(fnb[r], 0) * ND + face_node<DIM, Np>(d, 1 - s, t);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
+  }
+  if (tid < Np * Np) sD2[tid] = bD2;
+
+  // ---- stage: recover each node's temperature once
+  meta_store<DIM, EPB, TPB>(mr, sm);
+  if (valid) {
+    const lte::Eval ev = lte::state_eval<DIM>(tv, qown);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sq[el][q][node] = qown[q];
+    sT[el][node] = ev.T;
+    sp[el][node] = ev.pressure();
+    sc[el][node] = sqrt(ev.c2());
+    scell[el][node] = (int)(ev.cell.H - tv.planes);
+  }
+  lds_sync();
+
+  const int i = node % Np;
+  const int j = DIM > 1 ? (node / Np) % Np : 0;
+  const int k = DIM > 2 ? node / (Np * Np) : 0;
+  const int idx[3] = {i, j, k};
+  const double Je = sm.J[elc];
+
+  // a staged node's evaluation (cell from LDS, no search)
+  auto node_eval = [&](int pe, int n) -> lte::Eval {
+    const int off = scell[pe][n];
+    const int ci = off % tv.nx, cj = off / tv.nx;
+    return lte::at_cell(tv, sq[pe][0][n], sT[pe][n], ci, cj);
+  };
+
+  // ---- volume: every GLL-line pair once (DGSEMIntegrator.hpp:124-218)
+  constexpr int NPT = EPB * NPAIR;
+  constexpr int NRND = (DIM * NPT + TPB - 1) / TPB;
+  static_assert(DIM < 3 || (2 * NPT) / TPB > (NPT - 1) / TPB, "slot reuse needs a round boundary");
+  double rate[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) rate[q] = 0.0;
+#pragma unroll 1
+  for (int rr = 0; rr < NRND; ++rr) {
+    const int w = rr * TPB + tid;
+    if (w < DIM * NPT) {
+      const int dir = w / NPT;
+      const int rem = w - dir * NPT;
+      const int pe = rem / NPAIR;
+      const int r = rem - pe * NPAIR;
+      if (ebase_blk + pe < a.ne) {
+        const int sdir = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
+        const int line = r / NPL;
+        int pa, pb;
+        pair_nodes<Np>(r - line * NPL, pa, pb);
+        const int base = face_node<DIM, Np>(dir, 0, line);
+        const int na = base + pa * sdir, nb = base + pb * sdir;
+        double q1[NQ], q2[NQ], met[DIM];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          q1[q] = sq[pe][q][na];
+          q2[q] = sq[pe][q][nb];
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) met[d] = sm.adj[pe][dir * DIM + d];
+        const lte::Eval s1 = node_eval(pe, na), s2 = node_eval(pe, nb);
+        double F[NQ];
+        lte::lteec_flux<DIM, false>(tv, q1, q2, s1, s2, sp[pe][na], sp[pe][nb], 0.0, 0.0, met, F);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) sscr(pe, (dir & 1) * NPAIR + r, q) = F[q];
+      }
+    }
+#pragma unroll
+    for (int dir = 0; dir < DIM; ++dir) {
+      if (((dir + 1) * NPT - 1) / TPB != rr) continue;
+      lds_sync();
+      if (valid) {
+        const int mi = dir == 0 ? i : (dir == 1 ? j : k);
+        const int line = face_index<DIM, Np>(dir, i, j, k);
+#pragma unroll
+        for (int m = 0; m < Np; ++m) {
+          if (m == mi) continue;
+          const int pr = m < mi ? pair_index<Np>(m, mi) : pair_index<Np>(mi, m);
+          const double coef = sD2[mi * Np + m];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr(el, (dir & 1) * NPAIR + line * NPL + pr, q);
+        }
+      }
+      lds_sync();
+    }
+  }
+  double du[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) du[q] = (-1.0 / Je) * rate[q];
+
+  double xb[NQ];
+  {
+    const double *Xp = a.mode == 2 ? a.X : a.U;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) xb[q] = Xp[q * Ntot + ebase];
+  }
+
+  // ---- faces (EulerOperator_impl.hpp:211-392): LTEEC with Lax dissipation
+  {
+    const double w0 = a.basis.w[0];
+#pragma unroll
+    for (int r = 0; r < NFR; ++r) {
+      const int fi = tid + r * TPB;
+      const int pe = fi / NFACE;
+      const int fl = fi - pe * NFACE;
+      if (fi < EPB * NFACE && ebase_blk + pe < a.ne) {
+        const int f = fl / NF, t = fl - f * NF;
+        const int d = f >> 1, s = f & 1;
+        const int on = face_node<DIM, Np>(d, s, t);
+        const bool interior = fnb[r] >= 0;
+        double qo[NQ], qt[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          qo[q] = sq[pe][q][on];
+          qt[q] = interior ? qn[r][q] : qo[q];
+        }
+        const lte::Eval so = node_eval(pe, on);
+        const double po = sp[pe][on], co = sc[pe][on];
+        lte::Eval st = so;
+        double pt = po, ct = co;
+        if (interior) {
+          st = lte::state_eval<DIM>(tv, qt);
+          pt = st.pressure();
+          ct = sqrt(st.c2());
+        }
+        double n[DIM];
+#pragma unroll
+        for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[pe][f][dd];
+        // canonical order (the low-side element first) by selects: one
+        // evaluation per lane whichever side of the element it is on
+        const bool own_low = s != 0;
+        double ql[NQ], qh[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          ql[q] = own_low ? qo[q] : qt[q];
+          qh[q] = own_low ? qt[q] : qo[q];
+        }
+        const lte::Eval sl = own_low ? so : st, sh = own_low ? st : so;
+        double F[NQ];
+        lte::lteec_flux<DIM, true>(tv, ql, qh, sl, sh, own_low ? po : pt, own_low ? pt : po, own_low ? co : ct,
+                                   own_low ? ct : co, n, F);
+        const double wown = 1.0 / (w0 * sm.J[pe]);
+        const double cs = (interior && !s) ? wown : -wown;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) sscr(pe, fl, q) = cs * F[q];
+      }
+    }
+  }
+  lds_sync();
+  double fsum[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) fsum[q] = 0.0;
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const bool hi = idx[d] == Np - 1;
+      const bool on = hi || idx[d] == 0;
+      const int row = (2 * d + (hi ? 1 : 0)) * NF + face_index<DIM, Np>(d, i, j, k);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const double v = sscr(el, row, q);
+        fsum[q] += on ? v : 0.0;
+      }
+    }
+    const double dt = a.dt;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const double r = du[q] + fsum[q];
+      const double qo = sq[el][q][node];
+      double o;
+      if (a.mode == 0) o = r;
+      else if (a.mode == 1) o = qo + dt * r;
+      else o = a.cx * xb[q] + a.cy * (qo + dt * r);
+      a.out[q * Ntot + ebase] = o;
+    }
+  }
+}
+
+}  // namespace thx

@@ -80,13 +80,18 @@ def build_descriptor(rt: RunTime, mesh: BoxMesh):
 class RHSOperator:
     """RHSOperator<PhysicsT> (include/RHSOperator.hpp:119-188) over the GPU engine."""
 
-    def __init__(self, runtime: RunTime, mesh: BoxMesh):
+    def __init__(self, runtime: RunTime, mesh: BoxMesh, lte_table=None):
         self._lib = A.load()
         self.runtime = runtime
         self.mesh = mesh
         desc, keep = build_descriptor(runtime, mesh)
         h = C.c_void_p()
-        A.check(self._lib, self._lib.theseus_rhs_create(C.byref(desc), C.byref(h)))
+        if lte_table is None:
+            A.check(self._lib, self._lib.theseus_rhs_create(C.byref(desc), C.byref(h)))
+        else:
+            lt, lkeep = lte_descriptor(lte_table)
+            A.check(self._lib, self._lib.theseus_rhs_create_lte(C.byref(desc), C.byref(lt), C.byref(h)))
+            del lkeep
         self._h = h
         n = C.c_longlong()
         q = C.c_int()
@@ -176,8 +181,20 @@ class NSOperator(RHSOperator):
     """NSOperator<PhysicsT> (include/NSOperator.hpp:12-71), BR1 viscous terms."""
 
 
-def MakeRHSOperator(runtime: dict | RunTime, mesh: BoxMesh) -> RHSOperator:
-    """SimFactory_impl.hpp:63-488: pick the operator from the runTime strings."""
+def lte_descriptor(table):
+    """theseus_lte_table from a theseus_amd.lte.HelmholtzTable; returns (desc, keepalive)."""
+    arrays = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+              for x in (table.rho, table.T, table.e, table.planes, table.inv)]
+    d = A.LteTable()
+    d.nx, d.ny = int(table.rho.size), int(table.T.size)
+    d.rho_grid, d.T_grid, d.e_grid, d.planes, d.inverse = (a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrays)
+    return d, arrays
+
+
+def MakeRHSOperator(runtime: dict | RunTime, mesh: BoxMesh, lte_table=None) -> RHSOperator:
+    """SimFactory_impl.hpp:63-488: pick the operator from the runTime strings;
+    with `lte_table` (a theseus_amd.lte.HelmholtzTable) the gas is the
+    tabulated LTE gas with the LTEEC flux (LTEGasModel.hpp:339)."""
     rt = runtime if isinstance(runtime, RunTime) else RunTime.from_json(runtime)
     cls = NSOperator if rt.flow_model == A.FLOW_NAVIER_STOKES else EulerOperator
-    return cls(rt, mesh)
+    return cls(rt, mesh, lte_table) if lte_table is not None else cls(rt, mesh)
