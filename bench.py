@@ -91,6 +91,79 @@ def cpu_baseline(rt, mesh, U0, seconds):
     }
 
 
+def serving_leg(num_cases, log):
+    """Served-case stream: 3-D periodic Euler Taylor-Green-type cases
+    (n^3 hexes, n in 8..24, p in {2,3,4}, amplitude in [0.5,5], random
+    phases, 50 SSP-RK3 steps each, mixed precision), continuous batching over
+    one device pool per order under a 64 GiB HBM budget.  A saturated burst
+    (all cases at t = 0) measures the case throughput X; the Poisson stream
+    then arrives at 0.7 X and reports p50/p99 time to solution."""
+    from theseus_amd.serving import Server, make_requests
+
+    burst = make_requests(num=64, rate_per_s=1e9)
+    srv = Server(hbm_budget_bytes=64 << 30, max_cases=256)
+    srv.run(make_requests(num=6, rate_per_s=1e9, n_range=(8, 10)))  # warm the graphs and kernels
+    sat = srv.run(burst)
+    x = sat["cases_done"] / sat["makespan_s"]
+    rate = 0.7 * x
+    log(f"serving: saturated {x:.1f} cases/s, {sat['throughput_dof_rhs_per_s']:.3e} DOF*RHS/s; stream at {rate:.1f}/s")
+    res = srv.run(make_requests(num=num_cases, rate_per_s=rate))
+    info = {p: pool.info() for p, pool in srv.pools.items()}
+    srv.close()
+    log(f"serving: p50 {res['tts_p50_s']*1e3:.1f} ms, p99 {res['tts_p99_s']*1e3:.1f} ms, "
+        f"{res['throughput_dof_rhs_per_s']:.3e} DOF*RHS/s")
+    return {
+        "cases": res["cases_done"],
+        "arrival_rate_per_s": rate,
+        "saturated_cases_per_s": x,
+        "saturated_dof_rhs_per_s": sat["throughput_dof_rhs_per_s"],
+        "tts_p50_ms": res["tts_p50_s"] * 1e3,
+        "tts_p99_ms": res["tts_p99_s"] * 1e3,
+        "throughput_dof_rhs_per_s": res["throughput_dof_rhs_per_s"],
+        "makespan_s": res["makespan_s"],
+        "pools": {str(k): v for k, v in info.items()},
+        "workload": "Euler TGV-type cases, n^3 hexes n in [8,24], p in {2,3,4}, amplitude U(0.5,5), 50 SSP-RK3 "
+                    "steps each, mixed precision, Poisson arrivals, 64 GiB budget",
+    }
+
+
+def lte_leg(log, steps=10):
+    """LTE Taylor-Green vortex, 24^3 hexes, p=3, LTEEC flux, the synthetic
+    256 x 256 Helmholtz table, FP64: DOF*RHS-evals/s over SSP-RK3 steps."""
+    import torch
+
+    from theseus_amd.cases import c4_lte_tgv
+    from theseus_amd.operator import MakeRHSOperator
+
+    rt, mesh, U0, table = c4_lte_tgv(n=24, order=3)
+    op = MakeRHSOperator(rt, mesh, lte_table=table)
+    u = torch.from_numpy(U0).cuda()
+    s = op.EstimateStability(u)
+    dt = 0.5 / s.TotalRate()
+    t = 0.0
+    t = op.Step(u, t, dt)
+    torch.cuda.synchronize()
+    op.enable_kernel_timing(3 * steps)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        t = op.Step(u, t, dt)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    finite = bool(torch.isfinite(u).all())
+    k_ms, _ = op.kernel_timing(0)
+    op.close()
+    dof = U0.size
+    value = dof * 3 * steps / (ms * 1e-3)
+    log(f"lte: {ms/steps:.3f} ms/step, {value:.3e} DOF*RHS/s, kernel {k_ms*1e3:.1f} us")
+    return {"value": value, "unit": "DOF*RHS-evals/s", "ms_per_step": ms / steps, "steps": steps,
+            "kernel_mean_ms": k_ms, "finite": finite, "dof": dof,
+            "workload": "3-D LTE Taylor-Green vortex, 24^3 hexes, p=3, LTEEC, synthetic 256x256 Helmholtz table, "
+                        "rho0 0.1, T0 3500 K, Mach 0.3, FP64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +174,9 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-case stream")
+    ap.add_argument("--no-lte", action="store_true", help="skip the LTE Taylor-Green leg")
+    ap.add_argument("--serve-cases", type=int, default=512)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 10 times (scales the HBM byte counters)")
     args = ap.parse_args()
@@ -251,6 +327,12 @@ def main():
         "kernels": kernels,
         "mixed": mixed,
     }
+    del op, opm
+    torch.cuda.empty_cache()
+    if not args.no_serving:
+        out["serving"] = serving_leg(args.serve_cases, log)
+    if not args.no_lte:
+        out["lte"] = lte_leg(log)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(rt, mesh, U0, args.cpu_seconds)
