@@ -194,8 +194,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design (see DESIGN.md), "
-                                   "not a measured path", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this engine runs on one GPU; sharding across GPUs (element partitions with "
+                                   "a face-trace halo exchange) is not built or measured", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import dataclasses

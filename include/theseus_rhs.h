@@ -188,6 +188,55 @@ THESEUS_API int theseus_rhs_step_ssprk3(theseus_rhs *h, double *u, double t, dou
  * THESEUS_ERR_INVALID_ARGUMENT when blending is off.  Asynchronous. */
 THESEUS_API int theseus_rhs_get_alpha(const theseus_rhs *h, double *alpha, void *stream);
 
+/* Integral measures (dgsem_cache.hpp:16-26): node-quadrature integrals
+ * (weights J w_i w_j w_k, dgsem_cache_utilities.hpp:485-491) of the density,
+ * the kinetic energy density 0.5|m|^2/rho and the total energy density, and
+ * the nodal extremes of pressure, temperature and density.  As in the
+ * reference the maxima start at 0 and the minima at 1e32. */
+typedef struct theseus_integral_measures {
+  double mass, ke, en;
+  double max_press, min_press;
+  double max_temp, min_temp;
+  double max_dens, min_dens;
+} theseus_integral_measures;
+
+/* Replaces RHSOperator<P>::ComputeIntegralMeasures(u, diag)
+ * (RHSOperator_impl.hpp:530-717): per-workgroup partials and a fixed-order
+ * second pass on the device (the result does not depend on scheduling);
+ * nine doubles come back.  The first call whose mass is nonzero becomes the
+ * baseline (RHSOperator_impl.hpp:711-713).  Synchronises `stream`. */
+THESEUS_API int theseus_rhs_integral_measures(theseus_rhs *h, const double *u,
+                                  theseus_integral_measures *out, void *stream);
+/* RHSOperator::GetIntegralMeasuresBaseline (RHSOperator.hpp:92); all zero
+ * before the first theseus_rhs_integral_measures call. */
+THESEUS_API int theseus_rhs_integral_measures_baseline(const theseus_rhs *h, theseus_integral_measures *out);
+
+/* Device-resident clock of a variable-step run (the host loop of
+ * src/Simulation.cpp:1136-1158 and 1240-1301, TimeStepControl.hpp:15-48, kept
+ * on the device so that steps need no host round trip and can be captured in
+ * a graph).  The caller allocates it in device memory and initialises it. */
+typedef struct theseus_clock {
+  double t;             /* time reached */
+  double dt;            /* nominal step: cfl / total rate, at most max_dt_growth
+                           times the previous one; 0 before the first step */
+  double dt_real;       /* length of the last step: dt shortened to land on
+                           t_final; 0 when no step was taken */
+  double t_final;       /* end time; +inf for none */
+  double cfl;           /* target CFL (runTime "cfl") */
+  double max_dt_growth; /* runTime "max_dt_growth" (>= 1; <= 0: no limit) */
+  double rate;          /* total stability rate of the last estimate */
+  long long steps;      /* steps taken */
+} theseus_clock;
+
+/* One variable SSP-RK3 step of u in place with the step chosen on the
+ * device: stability estimate of u, dt = cfl / rate (limited to
+ * max_dt_growth * the previous dt), done when t >= t_final - 1e-8 dt (then no
+ * step is taken and u is left bit for bit as it is), else
+ * dt_real = min(dt, time to t_final), the three stages, t += dt_real.
+ * A total rate that is not positive and finite takes no step (rate records
+ * it).  Asynchronous on `stream`; `clock` is a device pointer. */
+THESEUS_API int theseus_rhs_step_ssprk3_cfl(theseus_rhs *h, double *u, theseus_clock *clock, void *stream);
+
 /* ---- instrumentation (no counterpart in the reference) -------------------
  * Per-kernel HIP-event timing for measurement tools: when enabled, every
  * kernel the handle enqueues is bracketed by hipEvents on its own stream, up
@@ -242,6 +291,10 @@ THESEUS_API int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, 
 THESEUS_API int theseus_pool_step(theseus_pool *p, void *stream);
 /* Copy a case's current state (its L-vector) to `u` (host or device memory); synchronises `stream`. */
 THESEUS_API int theseus_pool_read(theseus_pool *p, int case_id, double *u, void *stream);
+/* Integral measures of one case's current state (its element range only),
+ * as theseus_rhs_integral_measures computes them.  Synchronises `stream`. */
+THESEUS_API int theseus_pool_integral_measures(theseus_pool *p, int case_id, theseus_integral_measures *out,
+                                               void *stream);
 /* Free a case's slots for later admissions. */
 THESEUS_API int theseus_pool_release(theseus_pool *p, int case_id, void *stream);
 

@@ -13,8 +13,8 @@
  * one operator must not overlap.
  *
  * Header-only; link against libtheseus_rhs.so.  MFEM is not required; the
- * adapter a maintainer adds inside the reference wraps
- * this class in an mfem::TimeDependentOperator.
+ * adapter a maintainer adds inside the reference wraps this class in an
+ * mfem::TimeDependentOperator.
  */
 #ifndef THESEUS_RHS_OPERATOR_HPP
 #define THESEUS_RHS_OPERATOR_HPP
@@ -98,6 +98,27 @@ class RHSOperator {
   double Step(double *u, double t, double dt, void *stream = nullptr) {
     check_status(theseus_rhs_step_ssprk3(h_, u, t, dt, stream));
     return t + dt;
+  }
+
+  /* One variable step with dt chosen on the device (the reference's
+   * variable_dt loop, src/Simulation.cpp:1136-1158,1240-1301); `clock` is a
+   * device-resident theseus_clock.  Asynchronous. */
+  void StepCFL(double *u, theseus_clock *clock, void *stream = nullptr) {
+    check_status(theseus_rhs_step_ssprk3_cfl(h_, u, clock, stream));
+  }
+
+  /* RHSOperator<P>::ComputeIntegralMeasures (RHSOperator_impl.hpp:530-717); synchronises `stream`. */
+  theseus_integral_measures ComputeIntegralMeasures(const double *u, void *stream = nullptr) const {
+    theseus_integral_measures m{};
+    check_status(theseus_rhs_integral_measures(h_, u, &m, stream));
+    return m;
+  }
+
+  /* RHSOperator::GetIntegralMeasuresBaseline (RHSOperator.hpp:92). */
+  theseus_integral_measures GetIntegralMeasuresBaseline() const {
+    theseus_integral_measures m{};
+    check_status(theseus_rhs_integral_measures_baseline(h_, &m));
+    return m;
   }
 
   /* Blending coefficient per element (device array of num_elements doubles). */

@@ -133,6 +133,18 @@ int oracle_estimate_stability(void *h, const double *U, double *out3) {
   }
 }
 
+// out9 = {mass, ke, en, max_press, min_press, max_temp, min_temp, max_dens, min_dens}
+int oracle_integral_measures(void *h, const double *U, double *out9) {
+  try {
+    const Measures m = IntegralMeasures(*static_cast<Case *>(h), U);
+    std::memcpy(out9, &m, sizeof m);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
 int oracle_ssprk3_step(void *h, double *U, double dt) {
   try {
     SSPRK3Step(*static_cast<Case *>(h), U, dt);

@@ -870,6 +870,69 @@ inline void EstimateStability(const Case &c, const double *U, double out[3]) {
   out[2] = diff;
 }
 
+// IntegralMeasures (dgsem_cache.hpp:16-26) of one state, as
+// RHSOperator::ComputeIntegralMeasures computes it (RHSOperator_impl.hpp:530-717):
+// node quadrature with weights J * w_i w_j w_k (dgsem_cache_utilities.hpp:485-491;
+// no axisymmetric multiplier), per-element sums in node order then summed over
+// elements in order; maxima start at 0 and minima at 1e32.  Ideal gas:
+// ke = 0.5 |m|^2 / rho, p = (gamma - 1)(rhoE - ke), T = p / (rho R)
+// (EOS.hpp:90-157); tabulated gas: T recovered from e, p from the table
+// (LTEEOS.hpp:74-180).
+struct Measures {
+  double mass, ke, en, max_press, min_press, max_temp, min_temp, max_dens, min_dens;
+};
+
+inline Measures IntegralMeasures(const Case &c, const double *U) {
+  Measures m{0.0, 0.0, 0.0, 0.0, 1e32, 0.0, 1e32, 0.0, 1e32};
+  const int Np = c.Np, dim = c.dim;
+  double q[MAXEQ];
+  for (int e = 0; e < c.ne; ++e) {
+    double mass = 0.0, ke_i = 0.0, en = 0.0;
+    double min_d = 1e32, max_d = 0.0, min_t = 1e32, max_t = 0.0, min_p = 1e32, max_p = 0.0;
+    for (int node = 0; node < c.ndof; ++node) {
+      c.gather(U, e, node, q);
+      const int i = node % Np, j = dim > 1 ? (node / Np) % Np : 0, k = dim > 2 ? node / (Np * Np) : 0;
+      double w = c.basis.w[i];
+      if (dim > 1) w *= c.basis.w[j];
+      if (dim > 2) w *= c.basis.w[k];
+      const double rho = q[0];
+      double m2 = 0.0;
+      for (int d = 0; d < dim; ++d) m2 += q[1 + d] * q[1 + d];
+      const double ke = 0.5 * m2 / rho;
+      const double rhoE = q[dim + 1];
+      double press, temper;
+      if (c.lte) {
+        const lte::Eval s = lte::evaluate_state(*c.lte, q, dim);
+        press = s.pressure();
+        temper = s.T;
+      } else {
+        press = c.gas.gammaM1 * (rhoE - ke);
+        temper = press / (rho * c.gas.R_gas);
+      }
+      const double measure = c.J[e] * w;
+      mass += rho * measure;
+      ke_i += ke * measure;
+      en += rhoE * measure;
+      min_t = std::min(min_t, temper);
+      max_t = std::max(max_t, temper);
+      min_d = std::min(min_d, rho);
+      max_d = std::max(max_d, rho);
+      min_p = std::min(min_p, press);
+      max_p = std::max(max_p, press);
+    }
+    m.mass += mass;
+    m.ke += ke_i;
+    m.en += en;
+    m.min_press = std::min(m.min_press, min_p);
+    m.max_press = std::max(m.max_press, max_p);
+    m.min_temp = std::min(m.min_temp, min_t);
+    m.max_temp = std::max(m.max_temp, max_t);
+    m.min_dens = std::min(m.min_dens, min_d);
+    m.max_dens = std::max(m.max_dens, max_d);
+  }
+  return m;
+}
+
 // mfem::RK3SSPSolver::Step (Shu-Osher form).
 inline void SSPRK3Step(const Case &c, double *x, double dt) {
   const size_t M = c.ntot() * c.neq;

@@ -50,6 +50,8 @@ def load():
     lib.oracle_blending.argtypes = [C.c_void_p, _dp, _dp, _dp]
     lib.oracle_estimate_stability.argtypes = [C.c_void_p, _dp, _dp]
     lib.oracle_estimate_stability.restype = C.c_int
+    lib.oracle_integral_measures.argtypes = [C.c_void_p, _dp, _dp]
+    lib.oracle_integral_measures.restype = C.c_int
     lib.oracle_ssprk3_step.argtypes = [C.c_void_p, _dp, C.c_double]
     lib.oracle_ssprk3_step.restype = C.c_int
     _lib = lib
@@ -153,6 +155,51 @@ class OracleCase:
         if rc:
             raise RuntimeError(self._lib.oracle_last_error().decode())
         return out
+
+    MEASURES = ("mass", "ke", "en", "max_press", "min_press", "max_temp", "min_temp", "max_dens", "min_dens")
+
+    def integral_measures(self, U) -> dict:
+        """RHSOperator::ComputeIntegralMeasures (RHSOperator_impl.hpp:530-717)."""
+        U = arr(U)
+        out = np.zeros(9)
+        rc = self._lib.oracle_integral_measures(self._h, _p(U), _p(out))
+        if rc:
+            raise RuntimeError(self._lib.oracle_last_error().decode())
+        return dict(zip(self.MEASURES, out.tolist()))
+
+    def adaptive_steps(self, U, cfl, max_dt_growth, t_final, nsteps, t=0.0):
+        """The reference's variable-step loop (src/Simulation.cpp:1136-1158,
+        1240-1301; TimeStepControl.hpp:15-48): dt = cfl / TotalRate on the
+        initial state, then per step dt_real = min(dt, time to t_final),
+        step, t += dt_real, and dt = min(cfl / TotalRate, max_dt_growth * dt)
+        on the new state; the loop ends once t >= t_final - 1e-8 dt.
+        Returns (U, t, dt, dt_real, steps)."""
+        eps = np.finfo(float).eps
+
+        def total(r):
+            return max(r[0], r[1]) + r[2]
+
+        U = arr(U).copy()
+        rate = total(self.stability(U))
+        if not rate > 0.0:
+            raise RuntimeError("The stability rate must be positive for variable timestepping.")
+        dt = cfl / rate
+        dt_real = 0.0
+        steps = 0
+        done = t >= t_final - 1e-8 * dt
+        while not done and steps < nsteps:
+            tol = 16.0 * eps * max(1.0, abs(t))
+            delta = t_final - t
+            dt_real = min(dt, delta) if (np.isfinite(t_final) and delta > tol) else dt
+            U = self.step(U, dt_real)
+            t += dt_real
+            steps += 1
+            rate = total(self.stability(U))
+            if not rate > 0.0:
+                raise RuntimeError("The stability rate must be positive for variable timestepping.")
+            dt = min(cfl / rate, max_dt_growth * dt)
+            done = t >= t_final - 1e-8 * dt
+        return U, t, dt, dt_real, steps
 
     def step(self, U, dt):
         U = arr(U).copy()

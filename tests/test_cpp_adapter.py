@@ -51,6 +51,13 @@ int main() {
     std::printf("invalid_argument: %s\n", e.what());
     seen |= 4;
   }
+  // the measure and device-clock members compile against the C-ABI (no device work here)
+  auto m1 = &theseus::RHSOperator::ComputeIntegralMeasures;
+  auto m2 = &theseus::RHSOperator::GetIntegralMeasuresBaseline;
+  auto m3 = &theseus::RHSOperator::StepCFL;
+  (void)m1; (void)m2; (void)m3;
+  static_assert(sizeof(theseus_clock) == 64, "clock layout");
+  static_assert(sizeof(theseus_integral_measures) == 72, "measures layout");
   theseus::StabilityEstimate s;
   s.advective_rate = 2.0; s.surface_rate = 3.0; s.diffusive_rate = 0.5;
   if (s.TotalRate() == 3.5) seen |= 8;

@@ -49,6 +49,9 @@ EXPORTED_SYMBOLS = (
     "theseus_rhs_estimate_stability",
     "theseus_rhs_step_ssprk3",
     "theseus_rhs_get_alpha",
+    "theseus_rhs_integral_measures",
+    "theseus_rhs_integral_measures_baseline",
+    "theseus_rhs_step_ssprk3_cfl",
     "theseus_rhs_enable_kernel_timing",
     "theseus_rhs_kernel_timing",
     "theseus_last_error",
@@ -58,6 +61,7 @@ EXPORTED_SYMBOLS = (
     "theseus_pool_admit",
     "theseus_pool_step",
     "theseus_pool_read",
+    "theseus_pool_integral_measures",
     "theseus_pool_release",
 )
 
@@ -126,6 +130,21 @@ class Stability(C.Structure):
     _fields_ = [("advective_rate", C.c_double), ("surface_rate", C.c_double), ("diffusive_rate", C.c_double)]
 
 
+class IntegralMeasures(C.Structure):
+    """theseus_integral_measures (dgsem_cache.hpp:16-26)."""
+    _fields_ = [(n, C.c_double) for n in ("mass", "ke", "en", "max_press", "min_press", "max_temp", "min_temp",
+                                          "max_dens", "min_dens")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Clock(C.Structure):
+    """theseus_clock: the device-resident variable-step clock (64 bytes)."""
+    _fields_ = [("t", C.c_double), ("dt", C.c_double), ("dt_real", C.c_double), ("t_final", C.c_double),
+                ("cfl", C.c_double), ("max_dt_growth", C.c_double), ("rate", C.c_double), ("steps", C.c_longlong)]
+
+
 _lib = None
 
 
@@ -158,6 +177,12 @@ def load(path: Path | str | None = None) -> C.CDLL:
     lib.theseus_rhs_enable_kernel_timing.restype = C.c_int
     lib.theseus_rhs_kernel_timing.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.theseus_rhs_kernel_timing.restype = C.c_int
+    lib.theseus_rhs_integral_measures.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(IntegralMeasures), C.c_void_p]
+    lib.theseus_rhs_integral_measures.restype = C.c_int
+    lib.theseus_rhs_integral_measures_baseline.argtypes = [C.c_void_p, C.POINTER(IntegralMeasures)]
+    lib.theseus_rhs_integral_measures_baseline.restype = C.c_int
+    lib.theseus_rhs_step_ssprk3_cfl.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.theseus_rhs_step_ssprk3_cfl.restype = C.c_int
     lib.theseus_last_error.argtypes = []
     lib.theseus_last_error.restype = C.c_char_p
     vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
@@ -168,8 +193,9 @@ def load(path: Path | str | None = None) -> C.CDLL:
     lib.theseus_pool_step.argtypes = [vp, vp]
     lib.theseus_pool_read.argtypes = [vp, C.c_int, vp, vp]
     lib.theseus_pool_release.argtypes = [vp, C.c_int, vp]
+    lib.theseus_pool_integral_measures.argtypes = [vp, C.c_int, C.POINTER(IntegralMeasures), vp]
     for name in ("theseus_pool_create", "theseus_pool_destroy", "theseus_pool_info", "theseus_pool_admit",
-                 "theseus_pool_step", "theseus_pool_read", "theseus_pool_release"):
+                 "theseus_pool_step", "theseus_pool_read", "theseus_pool_release", "theseus_pool_integral_measures"):
         getattr(lib, name).restype = C.c_int
     if path is None:
         _lib = lib

@@ -203,6 +203,28 @@ static double diffusion_scale(int order) {
 
 // Clears the stability accumulators with a kernel (no memset node, so a
 // captured graph replays it correctly).
+// One workgroup of 256: thread t combines partials t, t+256, ... in order,
+// then a fixed binary tree.
+__global__ void __launch_bounds__(256) k_measures_final(const double *partial, int nb, double *out) {
+  __shared__ double red[256][NMEAS];
+  const int tid = threadIdx.x;
+  double m[NMEAS];
+  meas_identity(m);
+  for (int b = tid; b < nb; b += 256) meas_combine(m, partial + (size_t)b * NMEAS);
+#pragma unroll
+  for (int c = 0; c < NMEAS; ++c) red[tid][c] = m[c];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      meas_combine(m, red[tid + st]);
+#pragma unroll
+      for (int c = 0; c < NMEAS; ++c) red[tid][c] = m[c];
+    }
+    __syncthreads();
+  }
+  if (tid < NMEAS) out[tid] = red[0][tid];
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0ull;
@@ -259,6 +281,8 @@ struct theseus_rhs {
   bool pref_set = false;
   double *d_stage[2] = {nullptr, nullptr};
   unsigned long long *d_rates = nullptr;
+  double *d_meas = nullptr;           // integral measures: [nblocks][9] partials, then the 9 results
+  theseus_integral_measures diag0{};  // baseline: the first measures with nonzero mass
   bool has_boundary = false;
   // per-kernel event timing (instrumentation)
   int timing_capacity = 0;
@@ -573,6 +597,7 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_stage[0]);
   hipFree(h->d_stage[1]);
   hipFree(h->d_rates);
+  hipFree(h->d_meas);
   delete h;
   return THESEUS_OK;
 }
@@ -591,22 +616,21 @@ int theseus_rhs_eval(theseus_rhs *h, const double *u, double *dudt, void *stream
   return enqueue_rhs(h, a, (hipStream_t)stream);
 }
 
-int theseus_rhs_step_ssprk3(theseus_rhs *h, double *u, double t, double dt, void *stream) {
-  (void)t;  // the operator is autonomous (no time-dependent sources or boundary data)
-  if (!h || !u) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle or vector");
-  if (!std::isfinite(dt)) return fail(THESEUS_ERR_INVALID_ARGUMENT, "dt must be finite");
-  hipStream_t s = (hipStream_t)stream;
+// The three Shu-Osher stages; dt_dev (device, may be null) overrides dt.
+static int enqueue_ssprk3(theseus_rhs *h, double *u, double dt, const double *dt_dev, hipStream_t s) {
   double *A = h->d_stage[0], *B = h->d_stage[1];
   int rc;
   // stage 1: A = u + dt L(u)
   RHSArgs a = h->args(u, A);
   a.mode = 1;
   a.dt = dt;
+  a.case_dt = dt_dev;
   if ((rc = enqueue_rhs(h, a, s))) return rc;
   // stage 2: B = 3/4 u + 1/4 (A + dt L(A))
   a = h->args(A, B);
   a.mode = 2;
   a.dt = dt;
+  a.case_dt = dt_dev;
   a.X = u;
   a.cx = 3. / 4;
   a.cy = 1. / 4;
@@ -615,15 +639,15 @@ int theseus_rhs_step_ssprk3(theseus_rhs *h, double *u, double t, double dt, void
   a = h->args(B, u);
   a.mode = 2;
   a.dt = dt;
+  a.case_dt = dt_dev;
   a.X = u;
   a.cx = 1. / 3;
   a.cy = 2. / 3;
   return enqueue_rhs(h, a, s);
 }
 
-int theseus_rhs_estimate_stability(theseus_rhs *h, const double *u, theseus_stability *out, void *stream) {
-  if (!h || !u || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle, vector or output");
-  hipStream_t s = (hipStream_t)stream;
+// Zeroes the rate slots and launches the stability kernel (no host copy).
+static void enqueue_stability(theseus_rhs *h, const double *u, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, h->d_rates, 3);
   StabArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -642,6 +666,104 @@ int theseus_rhs_estimate_stability(theseus_rhs *h, const double *u, theseus_stab
   a.w0 = h->basis.w[0];
   a.lte = h->ltev;
   h->sk->stability(a, h->nblocks(), s);
+}
+
+// The variable-step decision of src/Simulation.cpp:1136-1158,1240-1301 and
+// TimeStepControl.hpp:15-48 on the device, from the rates just estimated.
+__global__ void k_clock(theseus_clock *c, const unsigned long long *rates) {
+  if (threadIdx.x != 0) return;
+  const double adv = __longlong_as_double((long long)rates[0]);
+  const double surf = __longlong_as_double((long long)rates[1]);
+  const double diff = __longlong_as_double((long long)rates[2]);
+  const double rate = fmax(adv, surf) + diff;  // StabilityEstimate::TotalRate
+  c->rate = rate;
+  if (!(rate > 0.0) || !isfinite(rate)) {
+    c->dt_real = 0.0;
+    return;
+  }
+  const double est = c->cfl / rate;
+  double dt = est;
+  if (c->dt > 0.0 && c->max_dt_growth > 0.0) dt = fmin(est, c->max_dt_growth * c->dt);
+  const double t = c->t;
+  double dt_real = 0.0;
+  if (!(t >= c->t_final - 1e-8 * dt)) {
+    const double tol = 16.0 * 2.220446049250313e-16 * fmax(1.0, fabs(t));  // TimeEventTolerance
+    const double delta = c->t_final - t;
+    dt_real = (isfinite(c->t_final) && delta > tol) ? fmin(dt, delta) : dt;
+  }
+  c->dt = dt;
+  c->dt_real = dt_real;
+  c->t = t + dt_real;
+  if (dt_real > 0.0) c->steps += 1;
+}
+
+int theseus_rhs_step_ssprk3(theseus_rhs *h, double *u, double t, double dt, void *stream) {
+  (void)t;  // the operator is autonomous (no time-dependent sources or boundary data)
+  if (!h || !u) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle or vector");
+  if (!std::isfinite(dt)) return fail(THESEUS_ERR_INVALID_ARGUMENT, "dt must be finite");
+  return enqueue_ssprk3(h, u, dt, nullptr, (hipStream_t)stream);
+}
+
+int theseus_rhs_step_ssprk3_cfl(theseus_rhs *h, double *u, theseus_clock *clock, void *stream) {
+  if (!h || !u || !clock) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle, vector or clock");
+  hipStream_t s = (hipStream_t)stream;
+  enqueue_stability(h, u, s);
+  hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, clock, h->d_rates);
+  HIP_TRY(hipGetLastError());
+  return enqueue_ssprk3(h, u, 0.0, &clock->dt_real, s);
+}
+
+// Integral measures of elements [e0, e0 + ne) of U; partial holds the
+// per-workgroup partials followed by the nine results.
+static int measures_impl(const ShapeKernels *sk, const double *U, long long Ntot, int e0, int ne, const double *J,
+                         const GasC<double> &gas, const BasisArgs &basis, const lte::View &lv, double *partial,
+                         theseus_integral_measures *out, hipStream_t s) {
+  MeasArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.U = U;
+  a.partial = partial;
+  a.Ntot = Ntot;
+  a.e0 = e0;
+  a.ne = ne;
+  a.J = J;
+  a.gas = gas;
+  std::memcpy(a.w, basis.w, sizeof(a.w));
+  a.lte = lv;
+  const int nb = (ne + sk->epb - 1) / sk->epb;
+  sk->measures(a, nb, s);
+  hipLaunchKernelGGL(k_measures_final, dim3(1), dim3(256), 0, s, partial, nb, partial + (size_t)nb * NMEAS);
+  HIP_TRY(hipGetLastError());
+  double r[NMEAS];
+  HIP_TRY(hipMemcpyAsync(r, partial + (size_t)nb * NMEAS, sizeof r, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  static_assert(sizeof(theseus_integral_measures) == sizeof r, "nine doubles");
+  std::memcpy(out, r, sizeof r);
+  return THESEUS_OK;
+}
+
+int theseus_rhs_integral_measures(theseus_rhs *h, const double *u, theseus_integral_measures *out, void *stream) {
+  if (!h || !u || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle, vector or output");
+  if (!h->d_meas) {
+    const int rc = upload<double>(&h->d_meas, nullptr, (size_t)(h->nblocks() + 1) * NMEAS);
+    if (rc) return rc;
+  }
+  const int rc = measures_impl(h->sk, u, h->Ntot, 0, h->d.num_elements, h->d_J, h->gas, h->basis, h->ltev, h->d_meas,
+                               out, (hipStream_t)stream);
+  if (rc) return rc;
+  if (h->diag0.mass == 0.0) h->diag0 = *out;
+  return THESEUS_OK;
+}
+
+int theseus_rhs_integral_measures_baseline(const theseus_rhs *h, theseus_integral_measures *out) {
+  if (!h || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle or output");
+  *out = h->diag0;
+  return THESEUS_OK;
+}
+
+int theseus_rhs_estimate_stability(theseus_rhs *h, const double *u, theseus_stability *out, void *stream) {
+  if (!h || !u || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle, vector or output");
+  hipStream_t s = (hipStream_t)stream;
+  enqueue_stability(h, u, s);
   HIP_TRY(hipGetLastError());
   unsigned long long r[3];
   HIP_TRY(hipMemcpyAsync(r, h->d_rates, sizeof r, hipMemcpyDeviceToHost, s));
@@ -704,6 +826,7 @@ struct theseus_pool {
   int *d_nbr = nullptr, *d_elem_case = nullptr;
   double *d_case_dt = nullptr, *d_case_pref = nullptr;
   unsigned long long *d_rates = nullptr;
+  double *d_meas = nullptr;  // integral measures: partials of the largest case, then the results
   std::vector<PoolCase> cases;
   std::vector<unsigned char> slot_used;  // per element slot
   hipStream_t capture_stream = nullptr;
@@ -1028,6 +1151,7 @@ int theseus_pool_destroy(theseus_pool *p) {
   hipFree(p->d_case_dt);
   hipFree(p->d_case_pref);
   hipFree(p->d_rates);
+  hipFree(p->d_meas);
   hipFree(p->d_U);
   hipFree(p->d_stage[0]);
   hipFree(p->d_stage[1]);
@@ -1159,6 +1283,19 @@ int theseus_pool_read(theseus_pool *p, int case_id, double *u, void *stream) {
     HIP_TRY(hipMemcpyAsync(u + q * nd, p->d_U + q * p->Ntot + off, nd * sizeof(double), hipMemcpyDefault, s));
   HIP_TRY(hipStreamSynchronize(s));
   return THESEUS_OK;
+}
+
+int theseus_pool_integral_measures(theseus_pool *p, int case_id, theseus_integral_measures *out, void *stream) {
+  if (!p || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null pool or output");
+  if (case_id < 0 || case_id >= (int)p->cases.size() || !p->cases[case_id].active)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "no such active case");
+  if (!p->d_meas) {
+    const int rc = upload<double>(&p->d_meas, nullptr, (size_t)((p->cap + p->sk->epb - 1) / p->sk->epb + 1) * NMEAS);
+    if (rc) return rc;
+  }
+  const PoolCase &c = p->cases[case_id];
+  return measures_impl(p->sk, p->d_U, p->Ntot, c.start, c.ne, p->d_J, p->gas, p->basis, lte::View{}, p->d_meas, out,
+                       (hipStream_t)stream);
 }
 
 int theseus_pool_release(theseus_pool *p, int case_id, void *stream) {

@@ -13,6 +13,14 @@
 
 #include <hip/hip_runtime.h>
 
+// No multiply-add contraction in the table arithmetic: the Hermite sums and
+// the thermodynamic combinations built on them (c^2, the divided differences)
+// cancel strongly, and fusing changes their rounding enough to move the
+// right-hand side of a smooth state by ~2e-12 relative (an uncontracted CPU
+// build and a contracted one of the same code differ by that much).  Restored
+// to the default at the end of this header.
+#pragma clang fp contract(off)
+
 namespace thx {
 namespace lte {
 
@@ -329,3 +337,5 @@ __device__ __forceinline__ Eval state_eval(const View &v, const double *q) {
 
 }  // namespace lte
 }  // namespace thx
+
+#pragma clang fp contract(fast)

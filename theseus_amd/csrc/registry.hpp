@@ -13,6 +13,7 @@ namespace thx {
 
 using RHSLaunch = void (*)(const RHSArgs &, int nblocks, hipStream_t);
 using StabLaunch = void (*)(const StabArgs &, int nblocks, hipStream_t);
+using MeasLaunch = void (*)(const MeasArgs &, int nblocks, hipStream_t);
 
 // [0]: FP64 arithmetic; [1]: FP32 flux arithmetic (mixed precision)
 struct FluxKernels {
@@ -23,6 +24,7 @@ struct FluxKernels {
 struct ShapeKernels {
   RHSLaunch ns_grad[2] = {nullptr, nullptr};  // Navier-Stokes pass 1
   StabLaunch stability = nullptr;
+  MeasLaunch measures = nullptr;
   int tpb = 0, epb = 0;
 };
 
@@ -60,6 +62,11 @@ void launch_ns_grad(const RHSArgs &a, int nb, hipStream_t s) {
 template <int DIM, int P>
 void launch_stability(const StabArgs &a, int nb, hipStream_t s) {
   hipLaunchKernelGGL((k_stability<DIM, P>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+}
+
+template <int DIM, int P>
+void launch_measures(const MeasArgs &a, int nb, hipStream_t s) {
+  hipLaunchKernelGGL((k_measures<DIM, P>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
 }
 
 template <int DIM, int VOL, int SURF, int P>
@@ -114,6 +121,7 @@ struct ShapeRegistrar {
     k.ns_grad[0] = &launch_ns_grad<double, DIM, P>;
     k.ns_grad[1] = &launch_ns_grad<float, DIM, P>;
     k.stability = &launch_stability<DIM, P>;
+    k.measures = &launch_measures<DIM, P>;
     k.tpb = Shape<DIM, P>::TPB;
     k.epb = Shape<DIM, P>::EPB;
     register_shape(DIM, P, k);

@@ -399,6 +399,9 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
     else return spd_[pe];
   };
 
+  // variable-step clock: a stage of a step of length 0 (the end time was
+  // reached) does nothing, so the state is left exactly as it is
+  if (a.mode != 0 && a.case_dt && !a.elem_case && a.case_dt[0] == 0.0) return;
   const GasC<R> &g = gas_of<R>(a);
   const int tid = threadIdx.x;
   const int el = tid / ND;
@@ -1108,6 +1111,117 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
     atomicMax(&a.rates[0], (unsigned long long)__double_as_longlong(adv));
     atomicMax(&a.rates[1], (unsigned long long)__double_as_longlong(surf));
     atomicMax(&a.rates[2], (unsigned long long)__double_as_longlong(diff));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Integral measures (RHSOperator_impl.hpp:530-717): node-quadrature integrals
+// of rho, 0.5|m|^2/rho and rhoE (weights J w_i w_j w_k) and the extremes of
+// pressure, temperature and density.  Each workgroup writes its partials with
+// plain stores; k_measures_final combines them in a fixed order, so the
+// result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr int NMEAS = 9;  // mass, ke, en, max_p, min_p, max_T, min_T, max_rho, min_rho
+
+struct MeasArgs {
+  const double *U;
+  double *partial;  // [gridDim.x][NMEAS]
+  long long Ntot;
+  int ne, e0;       // elements measured: [e0, e0 + ne)
+  const double *J;
+  GasC<double> gas;
+  double w[MAXNP];  // GLL weights on [0,1]
+  lte::View lte;
+};
+
+__device__ __forceinline__ void meas_identity(double *m) {
+  m[0] = m[1] = m[2] = 0.0;
+  m[3] = m[5] = m[7] = 0.0;  // maxima start at 0
+  m[4] = m[6] = m[8] = 1e32;  // minima start at 1e32
+}
+
+__device__ __forceinline__ void meas_combine(double *m, const double *o) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) m[k] += o[k];
+#pragma unroll
+  for (int k = 3; k < NMEAS; k += 2) {
+    m[k] = fmax(m[k], o[k]);
+    m[k + 1] = fmin(m[k + 1], o[k + 1]);
+  }
+}
+
+__device__ __forceinline__ void meas_wave(double *m) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o[NMEAS];
+#pragma unroll
+    for (int k = 0; k < NMEAS; ++k) o[k] = __shfl_xor(m[k], off, 64);
+    meas_combine(m, o);
+  }
+}
+
+template <int DIM, int P>
+__global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_measures(MeasArgs a) {
+  using S = Shape<DIM, P>;
+  constexpr int Np = S::Np, ND = S::NDOF, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
+  constexpr int NW = (TPB + 63) / 64;
+  __shared__ double red[NW][NMEAS];
+  const int tid = threadIdx.x;
+  const int el = tid / ND;
+  const int node = tid - el * ND;
+  const int e = a.e0 + blockIdx.x * EPB + el;
+  double m[NMEAS];
+  meas_identity(m);
+  if (el < EPB && e < a.e0 + a.ne) {
+    double q[NQ];
+    const size_t ebase = (size_t)e * ND + node;
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) q[c] = a.U[c * a.Ntot + ebase];
+    const int i = node % Np;
+    const int j = DIM > 1 ? (node / Np) % Np : 0;
+    const int k = DIM > 2 ? node / (Np * Np) : 0;
+    double w = a.w[i];
+    if (DIM > 1) w *= a.w[j];
+    if (DIM > 2) w *= a.w[k];
+    const double rho = q[0];
+    double m2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) m2 += q[1 + d] * q[1 + d];
+    const double ke = 0.5 * m2 / rho;
+    const double rhoE = q[DIM + 1];
+    double press, temper;
+    if (a.lte.nx > 0) {
+      const lte::Eval st = lte::state_eval<DIM>(a.lte, q);
+      press = st.pressure();
+      temper = st.T;
+    } else {
+      press = a.gas.gammaM1 * (rhoE - ke);
+      temper = press / (rho * a.gas.R_gas);
+    }
+    const double measure = a.J[e] * w;
+    m[0] = rho * measure;
+    m[1] = ke * measure;
+    m[2] = rhoE * measure;
+    m[3] = fmax(0.0, press);
+    m[4] = fmin(1e32, press);
+    m[5] = fmax(0.0, temper);
+    m[6] = fmin(1e32, temper);
+    m[7] = fmax(0.0, rho);
+    m[8] = fmin(1e32, rho);
+  }
+  meas_wave(m);
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int c = 0; c < NMEAS; ++c) red[tid >> 6][c] = m[c];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double r[NMEAS];
+#pragma unroll
+    for (int c = 0; c < NMEAS; ++c) r[c] = red[0][c];
+    for (int wv = 1; wv < NW; ++wv) meas_combine(r, red[wv]);
+#pragma unroll
+    for (int c = 0; c < NMEAS; ++c) a.partial[(size_t)blockIdx.x * NMEAS + c] = r[c];
   }
 }
 

@@ -31,6 +31,8 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   __shared__ ElemMeta<DIM, EPB> sm;
   auto sscr = [&](int pe, int row, int q) -> double & { return sraw[pe][row * NQ + q]; };
 
+  // a stage of a variable step of length 0 (end time reached) does nothing
+  if (a.mode != 0 && a.case_dt && a.case_dt[0] == 0.0) return;
   const lte::View &tv = a.lte;
   const int tid = threadIdx.x;
   const int el = tid / ND;
@@ -236,7 +238,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
         fsum[q] += on ? v : 0.0;
       }
     }
-    const double dt = a.dt;
+    const double dt = a.case_dt ? a.case_dt[0] : a.dt;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const double r = du[q] + fsum[q];

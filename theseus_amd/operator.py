@@ -37,6 +37,52 @@ class StabilityEstimate:
         return step * self.TotalRate()
 
 
+@dataclass
+class IntegralMeasures:
+    """Theseus::IntegralMeasures (dgsem_cache.hpp:16-26)."""
+
+    mass: float = 0.0
+    ke: float = 0.0
+    en: float = 0.0
+    max_press: float = 0.0
+    min_press: float = 0.0
+    max_temp: float = 0.0
+    min_temp: float = 0.0
+    max_dens: float = 0.0
+    min_dens: float = 0.0
+
+    @classmethod
+    def from_c(cls, m: "A.IntegralMeasures") -> "IntegralMeasures":
+        return cls(**m.as_dict())
+
+
+class DeviceClock:
+    """A theseus_clock in device memory: the state of a variable-step run
+    (time, nominal and actual step, end time, CFL target, growth limit, last
+    rate, steps taken) that StepCFL advances without a host round trip."""
+
+    FIELDS = ("t", "dt", "dt_real", "t_final", "cfl", "max_dt_growth", "rate", "steps")
+
+    def __init__(self, cfl: float, max_dt_growth: float = 1.2, t_final: float = float("inf"), t: float = 0.0,
+                 device="cuda"):
+        import torch
+
+        c = A.Clock(t=float(t), dt=0.0, dt_real=0.0, t_final=float(t_final), cfl=float(cfl),
+                    max_dt_growth=float(max_dt_growth), rate=0.0, steps=0)
+        raw = np.frombuffer(bytes(c), dtype=np.uint8).copy()
+        self.buf = torch.from_numpy(raw).to(device)
+
+    @property
+    def ptr(self) -> int:
+        return self.buf.data_ptr()
+
+    def read(self) -> dict:
+        """Synchronous copy of the clock to the host."""
+        raw = self.buf.cpu().numpy().tobytes()
+        c = A.Clock.from_buffer_copy(raw)
+        return {f: getattr(c, f) for f in self.FIELDS}
+
+
 def _stream_ptr(stream):
     import torch
 
@@ -140,6 +186,28 @@ class RHSOperator:
         A.check(self._lib, self._lib.theseus_rhs_step_ssprk3(self._h, C.c_void_p(u.data_ptr()), float(t),
                                                              float(dt), _stream_ptr(stream)))
         return t + dt
+
+    def StepCFL(self, u, clock: DeviceClock, stream=None) -> None:
+        """One SSP-RK3 step with the step length chosen on the device from the
+        stability estimate of u (the reference's variable-dt loop,
+        src/Simulation.cpp:1136-1158,1240-1301); asynchronous, no host copy."""
+        self._check(u, "u")
+        A.check(self._lib, self._lib.theseus_rhs_step_ssprk3_cfl(self._h, C.c_void_p(u.data_ptr()),
+                                                                 C.c_void_p(clock.ptr), _stream_ptr(stream)))
+
+    def ComputeIntegralMeasures(self, u, stream=None) -> IntegralMeasures:
+        """RHSOperator::ComputeIntegralMeasures (RHSOperator_impl.hpp:530-717)."""
+        self._check(u, "u")
+        m = A.IntegralMeasures()
+        A.check(self._lib, self._lib.theseus_rhs_integral_measures(self._h, C.c_void_p(u.data_ptr()), C.byref(m),
+                                                                   _stream_ptr(stream)))
+        return IntegralMeasures.from_c(m)
+
+    def GetIntegralMeasuresBaseline(self) -> IntegralMeasures:
+        """RHSOperator::GetIntegralMeasuresBaseline (RHSOperator.hpp:92)."""
+        m = A.IntegralMeasures()
+        A.check(self._lib, self._lib.theseus_rhs_integral_measures_baseline(self._h, C.byref(m)))
+        return IntegralMeasures.from_c(m)
 
     def Alpha(self, stream=None):
         """Per-element blending coefficient of the last evaluation (device tensor)."""

@@ -81,6 +81,13 @@ class Pool:
         ptr = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
         A.check(self._lib, self._lib.theseus_pool_read(self._h, int(case_id), C.c_void_p(ptr), _stream(stream)))
 
+    def integral_measures(self, case_id: int, stream=None) -> dict:
+        """Integral measures of one case's current state (RHSOperator_impl.hpp:530-717)."""
+        m = A.IntegralMeasures()
+        A.check(self._lib, self._lib.theseus_pool_integral_measures(self._h, int(case_id), C.byref(m),
+                                                                    _stream(stream)))
+        return m.as_dict()
+
     def release(self, case_id: int, stream=None) -> None:
         A.check(self._lib, self._lib.theseus_pool_release(self._h, int(case_id), _stream(stream)))
 
