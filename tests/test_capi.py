@@ -101,8 +101,24 @@ def test_numerical_flux_aliases(s, fam):
 def test_numerical_flux_errors():
     with pytest.raises(ValueError):
         numerical_flux_name("upwind")
+    assert numerical_flux_name("LTEECFlux") == "lteec"
+
+
+def test_lte_gas_model_selection():
+    """SimFactory_impl.hpp:75-95: the Helmholtz LTE gas is gas_model
+    "lte_helmholtz", or "lte" with the LTEEC flux; the legacy LTE gas with
+    other fluxes is not implemented; LTEEC needs the LTE gas."""
+    rt = RunTime.from_json({"dim": 3, "order": 3, "gas_model": "lte", "numerical_flux": "lteec"},
+                           {"SUBCELL_FV_BLENDING": False})
+    assert (rt.volume_flux, rt.surface_flux, rt.blending) == (A.VOLUME_LTEEC, A.SURFACE_LTEEC, False)
     with pytest.raises(NotImplementedError):
-        numerical_flux_name("lteec")
+        RunTime.from_json({"dim": 2, "order": 3, "gas_model": "lte", "numerical_flux": "lfr"})
+    with pytest.raises(NotImplementedError):
+        RunTime.from_json({"dim": 2, "order": 3, "gas_model": "lte_helmholtz", "numerical_flux": "hll"})
+    with pytest.raises(ValueError):
+        RunTime.from_json({"dim": 2, "order": 3, "numerical_flux": "lteec"})
+    assert RunTime.from_json({"dim": 2, "order": 3}, {"SUBCELL_FV_BLENDING": False}).blending is False
+    assert RunTime.from_json({"dim": 2, "order": 3}).blending is True
 
 
 def test_flow_model_strings():

@@ -32,6 +32,52 @@ def taylor_green(mesh: BoxMesh, order: int, gamma: float = 1.4, Ma: float = 0.1)
     return _pack(den, [den * u, den * v, den * w], E)
 
 
+def taylor_green_2d(mesh: BoxMesh, order: int, gamma: float = 1.4, Ma: float = 0.1) -> np.ndarray:
+    """TaylorGreenVortex2DIC (StateInit.hpp:218-246)."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    x, y = X[0], X[1]
+    p0 = 1.0 / (gamma * Ma * Ma)
+    den = np.ones_like(x)
+    u = np.sin(x) * np.cos(y)
+    v = -np.cos(x) * np.sin(y)
+    p = p0 + 0.25 * (np.cos(2.0 * x) + np.cos(2.0 * y))
+    E = p / (gamma - 1.0) + 0.5 * den * (u * u + v * v)
+    return _pack(den, [den * u, den * v], E)
+
+
+def compressible_taylor_green(mesh: BoxMesh, order: int, Ma: float = 0.1) -> np.ndarray:
+    """CompressibleTaylorGreenVortexIC (StateInit.hpp:176-205): p0 = 1e5,
+    T0 = 273.15, R = 287.05, gamma = 1.4, U0 = Ma sqrt(gamma R T0), rho = p/(R T0)."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    x, y, z = X[0], X[1], X[2]
+    p0, T0, R, gamma = 1.0e5, 273.15, 287.05, 1.4
+    rho0 = p0 / (R * T0)
+    U0 = Ma * np.sqrt(gamma * R * T0)
+    u = U0 * np.sin(x) * np.cos(y) * np.cos(z)
+    v = -U0 * np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(x)
+    p = p0 + (rho0 * U0 * U0) / 16.0 * (np.cos(2.0 * x) + np.cos(2.0 * y)) * (np.cos(2.0 * z) + 2)
+    rho = p / (R * T0)
+    E = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return _pack(rho, [rho * u, rho * v, rho * w], E)
+
+
+def kelvin_helmholtz(mesh: BoxMesh, order: int, gammaM1Inverse: float = 2.5) -> np.ndarray:
+    """KelvinHelmholtzInstabilyIC (StateInit.hpp:662-695)."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    x, y = X[0], X[1]
+    B = np.tanh(15.0 * y + 7.5) - np.tanh(15.0 * y - 7.5)
+    rho = 0.5 + 0.75 * B
+    u = 0.5 * (B - 1.0)
+    v = 0.1 * np.sin(2.0 * np.pi * x)
+    p = np.ones_like(x)
+    E = p * gammaM1Inverse + rho * 0.5 * (u * u + v * v)
+    return _pack(rho, [rho * u, rho * v], E)
+
+
 def shu_vortex(mesh: BoxMesh, order: int, gamma: float = 1.4, strength: float = 5.0,
                center=(0.0, 0.0), u_inf=(1.0, 1.0)) -> np.ndarray:
     """Textbook isentropic vortex (Shu): rho_inf = p_inf = 1, u_inf = (1,1)."""

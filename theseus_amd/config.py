@@ -26,12 +26,14 @@ PAIRINGS = {
     "hll": (A.VOLUME_CENTRAL, A.SURFACE_HLL),
     "llf": (A.VOLUME_CENTRAL, A.SURFACE_LLF),
     "roe": (A.VOLUME_CENTRAL, A.SURFACE_ROE),
+    "lteec": (A.VOLUME_LTEEC, A.SURFACE_LTEEC),
 }
 VOLUME_NAMES = {
     "central": A.VOLUME_CENTRAL,
     "chandrashekar": A.VOLUME_CHANDRASHEKAR,
     "ranocha": A.VOLUME_RANOCHA,
     "pirozzoli": A.VOLUME_PIROZZOLI,
+    "lteec": A.VOLUME_LTEEC,
 }
 SURFACE_NAMES = {
     "chandrashekar": A.SURFACE_CHANDRASHEKAR,
@@ -40,6 +42,7 @@ SURFACE_NAMES = {
     "roe": A.SURFACE_ROE,
     "hll": A.SURFACE_HLL,
     "llf": A.SURFACE_LLF,
+    "lteec": A.SURFACE_LTEEC,
 }
 
 
@@ -48,7 +51,7 @@ def numerical_flux_name(s: str | None) -> str:
     reference's aliases and test order (SimFactory_impl.hpp:78-118,253-334)."""
     f = (s or "").lower()
     if f in ("lteec", "lteecflux"):
-        raise NotImplementedError("the LTEEC flux (LTE Helmholtz gas) is not implemented in this engine yet")
+        return "lteec"
     if f == "" or f == "chandrashekar" or f.startswith("chan"):
         return "chandrashekar"
     if f == "ranocha":
@@ -91,11 +94,22 @@ class RunTime:
     max_dt_growth: float = 1.2
 
     @classmethod
-    def from_json(cls, runtime: dict) -> "RunTime":
+    def from_json(cls, runtime: dict, compile_time: dict | None = None) -> "RunTime":
+        """`runtime` is a case file's "runTime" object; `compile_time` its
+        "compileTime" object (SUBCELL_FV_BLENDING sets the blending default)."""
         gas = (runtime.get("gas_model") or "").lower()
-        if gas not in ("", "cpg", "ideal", "ideal_gas"):
-            raise NotImplementedError(f"gas_model {gas!r} is not implemented (calorically perfect gas only)")
         fam = numerical_flux_name(runtime.get("numerical_flux"))
+        # SimFactory_impl.hpp:75-95: the Helmholtz LTE gas is gas_model
+        # "lte_helmholtz", or "lte" together with the LTEEC flux
+        helmholtz = gas == "lte_helmholtz" or (gas == "lte" and fam == "lteec")
+        if gas not in ("", "cpg", "ideal", "ideal_gas") and not helmholtz:
+            raise NotImplementedError(f"gas_model {gas!r} with numerical_flux {runtime.get('numerical_flux')!r} is "
+                                      "not implemented (calorically perfect gas, or the Helmholtz LTE gas with "
+                                      "the LTEEC flux)")
+        if helmholtz and fam != "lteec":
+            raise NotImplementedError("the Helmholtz LTE gas is implemented with the LTEEC flux only")
+        if fam == "lteec" and not helmholtz:
+            raise ValueError("the LTEEC flux needs gas_model 'lte' or 'lte_helmholtz'")
         vol, surf = PAIRINGS[fam]
         if "volume_flux" in runtime:
             vol = VOLUME_NAMES[str(runtime["volume_flux"]).lower()]
@@ -118,7 +132,7 @@ class RunTime:
             R_gas=float(runtime.get("R_gas", 287.05)),
             mu=float(runtime.get("mu", 0.02)),
             alpha_max=float(runtime.get("alpha_max", 0.5)),
-            blending=bool(runtime.get("blending", True)),
+            blending=bool(runtime.get("blending", (compile_time or {}).get("SUBCELL_FV_BLENDING", True))),
             precision=A.PRECISION_MIXED if prec == "mixed" else A.PRECISION_FP64,
             cfl=float(runtime.get("cfl", 1.0)),
             dt=float(runtime["dt"]) if "dt" in runtime else None,

@@ -44,6 +44,12 @@ def log_grid(n: int, lo: float, hi: float) -> np.ndarray:
     return np.array([lo * 10.0 ** (i * step) for i in range(n)])
 
 
+def uniform_grid(n: int, lo: float, hi: float) -> np.ndarray:
+    """LTETable.hpp:85-93"""
+    step = (hi - lo) / (n - 1)
+    return np.array([lo + i * step for i in range(n)])
+
+
 class SyntheticGas:
     def __init__(self, R=R_AIR, Tc=4000.0, w=1000.0, g0=1.4, g1=0.2):
         self.R, self.Tc, self.w, self.g0, self.g1 = R, Tc, w, g0, g1
@@ -123,15 +129,15 @@ def _energy_at(t: HelmholtzTable, i: int, rho: float, T: np.ndarray) -> np.ndarr
 
 
 def synthetic_table(nx: int = 256, ny: int = 256, rho=(1e-4, 10.0), T=(10 ** 2.3, 10 ** 4.2),
-                    gas: SyntheticGas | None = None) -> HelmholtzTable:
+                    gas: SyntheticGas | None = None, rho_dist: str = "log", T_dist: str = "log") -> HelmholtzTable:
     """The LTE Taylor-Green workload's table: log-spaced rho and T grids, the
     forward planes from the synthetic gas, and the inverse table built as the
     reference builds it (energy axis uniform over the padded range of
     H - T H_T, 60 bisection halvings of the interpolated energy per density
     column, endpoint temperatures outside a column's range)."""
     gas = gas or SyntheticGas()
-    rg = log_grid(nx, *rho)
-    Tg = log_grid(ny, *T)
+    rg = (log_grid if rho_dist == "log" else uniform_grid)(nx, *rho)
+    Tg = (log_grid if T_dist == "log" else uniform_grid)(ny, *T)
     sT = entropy_integral(gas, Tg)
     e_of_T = np.array([gas.energy(t, Tg[0]) for t in Tg])
     s = sT[:, None] - gas.R * np.log(rg)[None, :]
