@@ -249,17 +249,20 @@ def main():
     opm = MakeRHSOperator(rtm, mesh)
     msm, (m2_ms, _), (m1_ms, _), u32 = timed_run(opm)
     value_m = dof * 3 * args.steps / (msm * 1e-3)
-    u0 = torch.from_numpy(U0).cuda()
-    r64 = torch.empty_like(u0)
-    r32 = torch.empty_like(u0)
-    op.Mult(u0, r64)
-    opm.Mult(u0, r32)
+    # the RHS of both paths on the evolved FP64 state: at t = 0 the density
+    # RHS of the Taylor-Green vortex vanishes (divergence-free velocity,
+    # uniform density), so a relative error there measures nothing
+    r64 = torch.empty_like(u64)
+    r32 = torch.empty_like(u64)
+    op.Mult(u64, r64)
+    opm.Mult(u64, r32)
     torch.cuda.synchronize()
     mixed = {
         "value": value_m,
         "ms_per_step": msm / args.steps,
         "speedup_vs_fp64": value_m / value,
         "rhs_rel_l2_vs_fp64_per_field": per_field(r32, r64),
+        "rhs_state": f"FP64 state after {args.warmup + args.steps} steps",
         "state_rel_l2_vs_fp64_per_field": per_field(u32, u64),
         "state_after_steps": args.warmup + args.steps,
         "kernels_mean_ms": {"ns_pass2_volume_face": m2_ms, "ns_pass1_br1_gradient": m1_ms},

@@ -64,4 +64,7 @@ def test_loaded_tgv2d_navier_stokes_rhs_parity(oracle):
     op.close()
     got = du.cpu().numpy()
     err = rel_l2(got, ref)
-    assert err <= 1e-12, (err, per_field_rel(got, ref, 4))
+    # Ma 0.08 on a 32^2 mesh: as for the full-size 3-D Taylor-Green case, the
+    # RHS is fixed by FP64 only to ~7e-12 (the oracle's own builds with and
+    # without fused multiply-add differ by 7.0e-12 here)
+    assert err <= 2e-11, (err, per_field_rel(got, ref, 4))

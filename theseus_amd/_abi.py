@@ -148,6 +148,19 @@ class Clock(C.Structure):
 _lib = None
 
 
+def _init_torch_device() -> None:
+    """When torch is present and sees a GPU, initialise its device before
+    the library makes its first HIP call.  In one process the two share the
+    HIP runtime, and torch's device initialisation has been seen to report
+    no GPU when the library had already created a stream of its own."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if getattr(torch.version, "hip", None) and torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def load(path: Path | str | None = None) -> C.CDLL:
     """Load libtheseus_rhs.so (raises FileNotFoundError when it is not built)."""
     global _lib
@@ -156,6 +169,7 @@ def load(path: Path | str | None = None) -> C.CDLL:
     p = Path(path) if path else LIB_PATH
     if not p.exists():
         raise FileNotFoundError(f"{p} is not built; run `python -m theseus_amd.build` (or __graft_entry__.build())")
+    _init_torch_device()
     lib = C.CDLL(str(p))
     lib.theseus_rhs_create.argtypes = [C.POINTER(CaseDesc), C.POINTER(C.c_void_p)]
     lib.theseus_rhs_create.restype = C.c_int

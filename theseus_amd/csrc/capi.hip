@@ -820,6 +820,7 @@ struct theseus_pool {
   GasC<double> gas;
   GasC<float> gas32;
   int pm = 0, tile = 0, xcd = 1;
+  int active = 0;  // slots a step launches over: the occupied slots' high-water mark, rounded up
   double adv_scale = 0, diff_scale = 0, surf_scale = 0;
   double *d_J = nullptr, *d_adj = nullptr, *d_alpha = nullptr, *d_Fv = nullptr, *d_U = nullptr;
   double *d_stage[2] = {nullptr, nullptr};
@@ -834,7 +835,7 @@ struct theseus_pool {
   hipGraphExec_t exec = nullptr;
 
   int epb() const { return (tk && cd.flow_model == THESEUS_FLOW_EULER) ? tk->epb[tile] : sk->epb; }
-  int nblocks() const { return (cap + epb() - 1) / epb(); }
+  int nblocks() const { return (active + epb() - 1) / epb(); }
 
   RHSArgs args(const double *U, double *out) const {
     RHSArgs a;
@@ -849,7 +850,7 @@ struct theseus_pool {
     }
     a.alpha = d_alpha;
     a.Ntot = Ntot;
-    a.ne = cap;
+    a.ne = active;  // every neighbour index of an occupied slot is below the high-water mark
     a.blending = cd.blending;
     a.alpha_max = cd.alpha_max;
     a.alpha_min = cd.alpha_min;
@@ -928,6 +929,30 @@ static int pool_reset_slots(theseus_pool *p, int start, int n, hipStream_t s) {
   return THESEUS_OK;
 }
 
+// The step's launch range follows the occupied slots: their high-water mark
+// rounded up to 1024 slots (first-fit keeps it low).  A new range drops the
+// captured graph, which the next step captures again; `s` is synchronised
+// first so that no launch of the old graph is still in flight.
+static int pool_update_active(theseus_pool *p, hipStream_t s) {
+  int hwm = 0;
+  for (const PoolCase &c : p->cases)
+    if (c.active) hwm = std::max(hwm, c.start + c.ne);
+  const int gran = 1024;
+  const int next = std::min(p->cap, (hwm + gran - 1) / gran * gran);
+  if (next == p->active) return THESEUS_OK;
+  HIP_TRY(hipStreamSynchronize(s));
+  if (p->exec) {
+    HIP_TRY(hipGraphExecDestroy(p->exec));
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    HIP_TRY(hipGraphDestroy(p->graph));
+    p->graph = nullptr;
+  }
+  p->active = next;
+  return THESEUS_OK;
+}
+
 static int pool_find_range(const theseus_pool *p, int n) {
   int run = 0;
   for (int e = 0; e < p->cap; ++e) {
@@ -993,6 +1018,7 @@ static int pool_autotune(theseus_pool *p, hipStream_t s) {
       if (!tune_xcd && x != p->xcd) continue;
       p->tile = t;
       p->xcd = x;
+      p->active = ne;
       RHSArgs a = p->args(p->d_U, p->d_stage[0]);
       for (int it = 0; it < 3; ++it) p->launch(a, s);
       hipEventRecord(e0, s);
@@ -1013,6 +1039,7 @@ static int pool_autotune(theseus_pool *p, hipStream_t s) {
   HIP_TRY(hipGetLastError());
   p->tile = best_tile;
   p->xcd = best_xcd;
+  p->active = 0;
   return pool_reset_slots(p, 0, ne, s);
 }
 
@@ -1254,6 +1281,10 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
   p->cases[cid].start = start;
   p->cases[cid].ne = ne;
   p->cases[cid].active = true;
+  {
+    const int rc = pool_update_active(p, s);
+    if (rc) return rc;
+  }
   if (case_id) *case_id = cid;
   if (dt_out) *dt_out = dt;
   return THESEUS_OK;
@@ -1261,6 +1292,7 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
 
 int theseus_pool_step(theseus_pool *p, void *stream) {
   if (!p) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null pool");
+  if (p->active == 0) return THESEUS_OK;  // no case admitted
   if (!p->exec) {
     hipStream_t cs = p->capture_stream;
     HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
@@ -1307,7 +1339,7 @@ int theseus_pool_release(theseus_pool *p, int case_id, void *stream) {
   if (rc) return rc;
   for (int e = 0; e < c.ne; ++e) p->slot_used[c.start + e] = 0;
   c = PoolCase();
-  return THESEUS_OK;
+  return pool_update_active(p, (hipStream_t)stream);
 }
 
 }  // extern "C"

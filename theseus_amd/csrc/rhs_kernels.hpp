@@ -171,27 +171,46 @@ __device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, 
     for (int l = 0; l < Np; ++l) acc += sVi[k * Np + l] * A[lane_ok ? base + l * Np * Np : 0];
     mode = acc;
   }
-  lds_sync();  // the sweeps' buffers are reused for the reduction below
-  double *red = buf;  // 3 rows of NP2
   const double mode2 = mode * mode;
   const bool keep1 = i <= P - 1 && j <= P - 1 && k <= P - 1;
   const bool keep2 = i <= P - 2 && j <= P - 2 && k <= P - 2;
-  if (lane_ok) {
-    red[node] = valid ? mode2 : 0.0;
-    red[NP2 + node] = valid && keep1 ? mode2 : 0.0;
-    red[2 * NP2 + node] = valid && keep2 ? mode2 : 0.0;
-    for (int t = node + ND; t < NP2; t += ND) red[t] = red[NP2 + t] = red[2 * NP2 + t] = 0.0;
-  }
-  lds_sync();
-  for (int stride = NP2 / 2; stride > 0; stride /= 2) {
-    if (lane_ok && node < stride) {
-      red[node] += red[node + stride];
-      red[NP2 + node] += red[NP2 + node + stride];
-      red[2 * NP2 + node] += red[2 * NP2 + node + stride];
+  double mm, m1m1, m2m2;
+  if constexpr (ND <= 64 && (ND & (ND - 1)) == 0) {
+    // An element's ND lanes lie in one wave: a butterfly over them pairs the
+    // operands exactly as the tree below does (own + partner at offsets
+    // ND/2, ..., 1; IEEE addition is commutative), so every lane ends with
+    // the tree's sum, bit for bit, without LDS round trips or barriers.
+    mm = valid ? mode2 : 0.0;
+    m1m1 = valid && keep1 ? mode2 : 0.0;
+    m2m2 = valid && keep2 ? mode2 : 0.0;
+#pragma unroll
+    for (int off = ND / 2; off > 0; off >>= 1) {
+      mm += __shfl_xor(mm, off, 64);
+      m1m1 += __shfl_xor(m1m1, off, 64);
+      m2m2 += __shfl_xor(m2m2, off, 64);
+    }
+  } else {
+    lds_sync();  // the sweeps' buffers are reused for the reduction below
+    double *red = buf;  // 3 rows of NP2
+    if (lane_ok) {
+      red[node] = valid ? mode2 : 0.0;
+      red[NP2 + node] = valid && keep1 ? mode2 : 0.0;
+      red[2 * NP2 + node] = valid && keep2 ? mode2 : 0.0;
+      for (int t = node + ND; t < NP2; t += ND) red[t] = red[NP2 + t] = red[2 * NP2 + t] = 0.0;
     }
     lds_sync();
+    for (int stride = NP2 / 2; stride > 0; stride /= 2) {
+      if (lane_ok && node < stride) {
+        red[node] += red[node + stride];
+        red[NP2 + node] += red[NP2 + node + stride];
+        red[2 * NP2 + node] += red[2 * NP2 + node + stride];
+      }
+      lds_sync();
+    }
+    mm = red[0];
+    m1m1 = red[NP2];
+    m2m2 = red[2 * NP2];
   }
-  const double mm = red[0], m1m1 = red[NP2], m2m2 = red[2 * NP2];
   const double eps = 1.0e-30;
   double val = 0.0;
   if (mm > eps) {
@@ -235,8 +254,19 @@ __device__ __forceinline__ int vis_index(int q, int dd) {
   return q == DIM + 1 ? VisComp<DIM>::en(dd) : VisComp<DIM>::tau(q - 1, dd);
 }
 
+// Staged node arrays leave one slot free after every x-line of Np nodes
+// (index n + n / Np): the nodes that a round of pair loads touches then fall
+// into distinct LDS banks.
+constexpr bool kLdsPad = true;
+template <int Np, int ND>
+struct StagedIndex {
+  static constexpr int size = kLdsPad ? ND + ND / Np : ND;
+  __device__ static __forceinline__ int at(int n) { return kLdsPad ? n + n / Np : n; }
+};
+
 // A node's point sample from the staged state: velocity is m * (1/rho),
 // the same product derive() forms, so staged and fresh samples agree bitwise.
+// `n` is a staged index (StagedIndex::at of the node).
 template <typename R, int DIM, int ND, int NQ>
 __device__ __forceinline__ void load_pt(const R (*sq)[ND], const R *sp, const R *sir, const R *spd, int n,
                                         Pt<R, DIM> &o) {
@@ -382,10 +412,12 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
   constexpr int NPL = SC::NPL, NPAIR = SC::NPAIR, NI = SC::NI;
   constexpr bool FP64 = sizeof(R) == sizeof(double);
 
-  __shared__ R sq[EPB][NQ][ND];
-  __shared__ R sp[EPB][ND];
-  __shared__ R sir[EPB][ND];
-  __shared__ R spd_[FP64 ? 1 : EPB][FP64 ? 1 : ND];  // p - p_ref (mixed precision)
+  using SX = StagedIndex<Np, ND>;
+  constexpr int NDS = SX::size;
+  __shared__ R sq[EPB][NQ][NDS];
+  __shared__ R sp[EPB][NDS];
+  __shared__ R sir[EPB][NDS];
+  __shared__ R spd_[FP64 ? 1 : EPB][FP64 ? 1 : NDS];  // p - p_ref (mixed precision)
   __shared__ double sraw[EPB][SC::DOUBLES];  // flux scratch (as R) or indicator sweeps (FP64)
   __shared__ double srp[EPB][ND];            // pressure * density, FP64, for the indicator
   __shared__ R sD2[Np * Np];
@@ -488,12 +520,12 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       me.q[q] = R(qown[q]);
-      sq[el][q][node] = me.q[q];
+      sq[el][q][SX::at(node)] = me.q[q];
     }
     derive(g, me);
-    sir[el][node] = me.ir;
+    sir[el][SX::at(node)] = me.ir;
     if constexpr (FP64) {
-      sp[el][node] = me.p;
+      sp[el][SX::at(node)] = me.p;
       if (a.blending) srp[el][node] = me.p * me.q[0];
     } else {
       // pressure from the FP64 state: p rounded once, p - p_ref formed before rounding
@@ -501,8 +533,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) m64.q[q] = qown[q];
       derive(a.gas, m64);
-      sp[el][node] = R(m64.p);
-      spd_[el][node] = R(m64.p - pref);
+      sp[el][SX::at(node)] = R(m64.p);
+      spd_[el][SX::at(node)] = R(m64.p - pref);
       if (a.blending) srp[el][node] = m64.p * m64.q[0];
     }
   }
@@ -550,8 +582,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
         pair_nodes<Np>(r - line * NPL, pa, pb);
         const int base = face_node<DIM, Np>(dir, 0, line);
         Pt<R, DIM> lo, hi;
-        load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), base + pa * sdir, lo);
-        load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), base + pb * sdir, hi);
+        load_pt<R, DIM, NDS, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), SX::at(base + pa * sdir), lo);
+        load_pt<R, DIM, NDS, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), SX::at(base + pb * sdir), hi);
         R met[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) met[d] = R(sm.adj[pe][dir * DIM + d]);
@@ -601,8 +633,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
           const int L = ii / (Np - 1), pos = ii - L * (Np - 1);
           const int ln = face_node<DIM, Np>(dir, 0, L) + pos * stride[dir];
           Pt<R, DIM> sl, sr;
-          load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), ln, sl);
-          load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), ln + stride[dir], sr);
+          load_pt<R, DIM, NDS, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), SX::at(ln), sl);
+          load_pt<R, DIM, NDS, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), SX::at(ln + stride[dir]), sr);
           R nor[DIM];
 #pragma unroll
           for (int d = 0; d < DIM; ++d) nor[d] = R(sm.adj[pe][dir * DIM + d]);
@@ -664,7 +696,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
         const int f = fl / NF, t = fl - f * NF;
         const int d = f >> 1, s = f & 1;
         Pt<R, DIM> own;
-        load_pt<R, DIM, ND, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), face_node<DIM, Np>(d, s, t), own);
+        load_pt<R, DIM, NDS, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), SX::at(face_node<DIM, Np>(d, s, t)), own);
         const R wown = R(1.0 / (w0 * sm.J[pe]));
         R n[DIM];
 #pragma unroll
@@ -726,7 +758,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
     for (int q = 0; q < NQ; ++q) {
       const double r = FP64 ? double(du[q] + fsum[q]) : double(du[q]) + double(fsum[q]);
       double qo;
-      if constexpr (FP64) qo = sq[el][q][node];
+      if constexpr (FP64) qo = sq[el][q][SX::at(node)];
       else qo = q64[q];
       double o;
       if (a.mode == 0) o = r;

@@ -225,11 +225,14 @@ class Server:
 
     def __init__(self, orders=(2, 3, 4), hbm_budget_bytes: int = 64 << 30, max_cases: int = 256,
                  precision=A.PRECISION_MIXED, cfl: float = 0.5, keep_results: bool = False, tile: int = -1,
-                 xcd_remap: int = -1):
+                 xcd_remap: int = -1, max_batch_steps: int = 8):
         import torch
 
         self.torch = torch
         self.cfl = cfl
+        # steps enqueued per pool between host synchronisations (never past a
+        # case's last step): fewer round trips, admissions wait at most this long
+        self.max_batch_steps = max(1, int(max_batch_steps))
         self.keep_results = keep_results
         budget = int(hbm_budget_bytes) // len(orders)
         self.pools = {p: Pool(euler_runtime(p, precision), budget, max_cases, tile, xcd_remap) for p in orders}
@@ -272,17 +275,22 @@ class Server:
                 if waiting:
                     time.sleep(max(0.0, min(waiting[0].arrival_s - (time.perf_counter() - t0), 0.01)))
                 continue
+            batch = {}
             for p, cases in active.items():
                 if cases:
-                    self.pools[p].step(self.streams[p])
+                    k = min(self.max_batch_steps, min(rec[0].steps - rec[1] for rec in cases.values()))
+                    for _ in range(k):
+                        self.pools[p].step(self.streams[p])
+                    batch[p] = k
             for p, cases in active.items():
                 if not cases:
                     continue
                 self.streams[p].synchronize()
                 finished = []
+                k = batch[p]
                 for cid, rec in cases.items():
-                    rec[1] += 1
-                    dof_rhs += 3 * rec[0].dofs
+                    rec[1] += k
+                    dof_rhs += 3 * k * rec[0].dofs
                     if rec[1] >= rec[0].steps:
                         finished.append(cid)
                 for cid in finished:
