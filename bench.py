@@ -91,7 +91,7 @@ def cpu_baseline(rt, mesh, U0, seconds):
     }
 
 
-def serving_leg(num_cases, log):
+def serving_leg(num_cases, log, fp64=False):
     """Served-case stream: 3-D periodic Euler Taylor-Green-type cases
     (n^3 hexes, n in 8..24, p in {2,3,4}, amplitude in [0.5,5], random
     phases, 50 SSP-RK3 steps each, mixed precision), continuous batching over
@@ -110,6 +110,15 @@ def serving_leg(num_cases, log):
     res = srv.run(make_requests(num=num_cases, rate_per_s=rate))
     info = {p: pool.info() for p, pool in srv.pools.items()}
     srv.close()
+    fp64_rate = None
+    if fp64:
+        from theseus_amd import _abi as A
+
+        s64 = Server(hbm_budget_bytes=64 << 30, max_cases=256, precision=A.PRECISION_FP64)
+        s64.run(make_requests(num=6, rate_per_s=1e9, n_range=(8, 10)))
+        fp64_rate = s64.run(burst)["throughput_dof_rhs_per_s"]
+        s64.close()
+        log(f"serving: FP64 saturated {fp64_rate:.3e} DOF*RHS/s")
     log(f"serving: p50 {res['tts_p50_s']*1e3:.1f} ms, p99 {res['tts_p99_s']*1e3:.1f} ms, "
         f"{res['throughput_dof_rhs_per_s']:.3e} DOF*RHS/s")
     return {
@@ -120,6 +129,7 @@ def serving_leg(num_cases, log):
         "tts_p50_ms": res["tts_p50_s"] * 1e3,
         "tts_p99_ms": res["tts_p99_s"] * 1e3,
         "throughput_dof_rhs_per_s": res["throughput_dof_rhs_per_s"],
+        "fp64_throughput_dof_rhs_per_s": fp64_rate,
         "makespan_s": res["makespan_s"],
         "pools": {str(k): v for k, v in info.items()},
         "workload": "Euler TGV-type cases, n^3 hexes n in [8,24], p in {2,3,4}, amplitude U(0.5,5), 50 SSP-RK3 "
@@ -164,6 +174,122 @@ def lte_leg(log, steps=10):
                         "rho0 0.1, T0 3500 K, Mach 0.3, FP64"}
 
 
+# Whole-RHS bytes per DOF (node x equation) of the workload table's
+# roofline fraction: the state read and the RHS written (Euler, 16), plus the
+# gradient written and read (Navier-Stokes BR1, 72); half in the mixed mode.
+TABLE_B_PER_DOF = {("euler", 0): 16.0, ("euler", 1): 8.0, ("ns", 0): 72.0, ("ns", 1): 36.0}
+
+
+def _gpu_rate(rt, mesh, U0, steps, warmup=5, lte_table=None):
+    """DOF*RHS-evals/s over `steps` SSP-RK3 steps at CFL 0.5 (device-resident)."""
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+
+    op = MakeRHSOperator(rt, mesh, lte_table=lte_table)
+    u = torch.from_numpy(U0).cuda()
+    dt = 0.5 / op.EstimateStability(u).TotalRate()
+    t = 0.0
+    for _ in range(warmup):
+        t = op.Step(u, t, dt)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        t = op.Step(u, t, dt)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    ok = bool(torch.isfinite(u).all())
+    op.close()
+    return U0.size * 3 * steps / (ms * 1e-3), ok
+
+
+def _cpu_rate(rt, mesh, U0, seconds, lte_table=None):
+    """The CPU oracle's SSP-RK3 steps on the host cores (bounded sample)."""
+    from oracle import pyoracle
+    from theseus_amd.operator import build_descriptor
+
+    if not pyoracle.LIB.exists():
+        pyoracle.build()
+    lib = pyoracle.load()
+    desc, keep = build_descriptor(rt, mesh)
+    case = pyoracle.OracleCase(desc)
+    tab = None
+    if lte_table is not None:
+        t = lte_table
+        tab = pyoracle.LteTable(arrays=(t.rho, t.T, t.e, t.planes.reshape(-1), t.inv.reshape(-1)))
+        tab.attach(case)
+    r = case.stability(U0)
+    dt = 0.5 / (max(r[0], r[1]) + r[2])
+    U = case.step(U0, dt)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds or n == 0:
+        U = case.step(U, dt)
+        n += 1
+    el = time.perf_counter() - t0
+    return U0.size * 3 * n / el, int(lib.oracle_num_threads()), n
+
+
+def config_table(log, c2_row, lte_row, serving_row, cpu_seconds=8.0):
+    """The workload table (C1-C5): GPU FP64 and mixed DOF*RHS-evals/s over
+    SSP-RK3 steps, the CPU oracle on the host cores (bounded samples) and the
+    whole-RHS roofline fraction value * B/DOF / 8 TB/s."""
+    import dataclasses
+
+    from theseus_amd import _abi as A
+    from theseus_amd.cases import c1_vortex2d, c3_riemann2d, c4_lte_tgv
+    from theseus_amd.serving import euler_runtime, tgv_case
+
+    rows = []
+
+    def frac(v, kind, prec):
+        return v * TABLE_B_PER_DOF[(kind, prec)] / (PEAK_HBM_GBPS * 1e9)
+
+    for name, make, steps in (("C1 2-D Euler vortex 16^2, p=3, blending", c1_vortex2d, 300),
+                              ("C3 2-D Riemann 128^2, p=4, blending, transmissive", c3_riemann2d, 60)):
+        rt, mesh, U0 = make()
+        g64, ok64 = _gpu_rate(rt, mesh, U0, steps)
+        gmx, okmx = _gpu_rate(dataclasses.replace(rt, precision=A.PRECISION_MIXED), mesh, U0, steps)
+        cpu, cores, n = _cpu_rate(rt, mesh, U0, cpu_seconds)
+        rows.append({"config": name, "dof": U0.size, "gpu_fp64": g64, "gpu_mixed": gmx, "cpu_fp64": cpu,
+                     "cpu_cores": cores, "cpu_steps": n, "roofline_frac_fp64": frac(g64, "euler", 0),
+                     "roofline_frac_mixed": frac(gmx, "euler", 1), "finite": ok64 and okmx})
+        log(f"table: {name}: GPU {g64:.3e} / mixed {gmx:.3e}, CPU {cpu:.3e} ({cores} cores)")
+    from theseus_amd.cases import c2_tgv3d
+
+    rt, mesh, U0 = c2_tgv3d(n=16)
+    cpu, cores, n = _cpu_rate(rt, mesh, U0, cpu_seconds)
+    rows.append({"config": "C2 3-D NS TGV 32^3, p=3", "dof": 10485760, "gpu_fp64": c2_row["value"],
+                 "gpu_mixed": c2_row["mixed"]["value"], "cpu_fp64": cpu, "cpu_cores": cores, "cpu_steps": n,
+                 "cpu_sample": "16^3 p=3 (same rate per DOF: element-local work)",
+                 "roofline_frac_fp64": frac(c2_row["value"], "ns", 0),
+                 "roofline_frac_mixed": frac(c2_row["mixed"]["value"], "ns", 1)})
+    if lte_row is not None:
+        rt, mesh, U0, table = c4_lte_tgv(n=8, order=3)
+        cpu, cores, n = _cpu_rate(rt, mesh, U0, cpu_seconds, lte_table=table)
+        rows.append({"config": "C4 3-D LTE TGV 24^3, p=3, LTEEC", "dof": lte_row["dof"], "gpu_fp64": lte_row["value"],
+                     "gpu_mixed": None, "cpu_fp64": cpu, "cpu_cores": cores, "cpu_steps": n,
+                     "cpu_sample": "8^3 p=3 (same rate per DOF)",
+                     "roofline_frac_fp64": frac(lte_row["value"], "euler", 0), "roofline_frac_mixed": None})
+        log(f"table: C4: GPU {lte_row['value']:.3e}, CPU {cpu:.3e}")
+    if serving_row is not None:
+        rt = euler_runtime(3, A.PRECISION_FP64)
+        mesh, U0 = tgv_case(rt, 16, 2.0, (0.1, 0.2, 0.3))
+        cpu, cores, n = _cpu_rate(rt, mesh, U0, cpu_seconds)
+        rows.append({"config": "C5 serving, 512 cases, 50 steps", "gpu_fp64": serving_row.get("fp64_throughput_dof_rhs_per_s"),
+                     "gpu_mixed": serving_row["saturated_dof_rhs_per_s"], "cpu_fp64": cpu, "cpu_cores": cores,
+                     "cpu_steps": n, "cpu_sample": "one 16^3 p=3 served case",
+                     "tts_p50_ms": serving_row["tts_p50_ms"], "tts_p99_ms": serving_row["tts_p99_ms"],
+                     "roofline_frac_fp64": frac(serving_row["fp64_throughput_dof_rhs_per_s"], "euler", 0)
+                     if serving_row.get("fp64_throughput_dof_rhs_per_s") else None,
+                     "roofline_frac_mixed": frac(serving_row["saturated_dof_rhs_per_s"], "euler", 1),
+                     "note": "GPU columns: saturated burst of 64 cases (all arrivals at t=0)"})
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +303,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the served-case stream")
     ap.add_argument("--no-lte", action="store_true", help="skip the LTE Taylor-Green leg")
     ap.add_argument("--serve-cases", type=int, default=512)
+    ap.add_argument("--table", action="store_true",
+                    help="also measure every workload (C1-C5: GPU FP64 / mixed, CPU oracle samples)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 10 times (scales the HBM byte counters)")
     args = ap.parse_args()
@@ -333,12 +461,14 @@ def main():
     del op, opm
     torch.cuda.empty_cache()
     if not args.no_serving:
-        out["serving"] = serving_leg(args.serve_cases, log)
+        out["serving"] = serving_leg(args.serve_cases, log, fp64=args.table)
     if not args.no_lte:
         out["lte"] = lte_leg(log)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(rt, mesh, U0, args.cpu_seconds)
+    if args.table:
+        out["table"] = config_table(log, out, out.get("lte"), out.get("serving"))
     print(json.dumps(out))
 
 
