@@ -73,8 +73,29 @@ extern "C" {
 #define THESEUS_SURFACE_LLF 5           /* LaxFriedrichsFlux.hpp:54-77 */
 #define THESEUS_SURFACE_LTEEC 6         /* LTEEC + Lax dissipation, LTEECFlux.hpp:11-99, DissipationOperator.hpp:15-28 */
 
-/* ---- boundary condition types (bc_cache_utilities.hpp:11-23) ------------ */
-#define THESEUS_BC_SUPERSONIC_OUTFLOW 2 /* F*(q,q,n), bc_kernels.hpp:236-238 */
+/* ---- boundary condition types (BCType, bc_cache_utilities.hpp:11-23) ----
+ * Inviscid fluxes as ApplyBoundaryConditionInviscid (bc_kernels.hpp:219-287),
+ * Navier-Stokes fluxes as ApplyViscousBoundaryCondition (:289-375) and the
+ * BR1 gradient's boundary jumps as ComputeBdrFaceGradFlux (:32-135).  Each
+ * boundary index b owns THESEUS_BC_DATA_STRIDE doubles of bc_data
+ * (bc_data[b * THESEUS_BC_DATA_STRIDE + k]), the reference's bc_vector_data
+ * payload of that boundary (Simulation.cpp:575-820):
+ *   SUPERSONIC_INFLOW, PRESCRIBED_STATE: the exterior conservative state
+ *     [rho, rho*u_0..rho*u_{dim-1}, rho*E];
+ *   NO_SLIP_ADIABATIC: wall velocity [dim], then the wall heat flux;
+ *   NO_SLIP_ISOTHERMAL: wall velocity [dim], then the wall temperature (> 0);
+ *   SLIP_WALL, SUPERSONIC_OUTFLOW, SYMMETRY: none.
+ * In the Euler operator the no-slip types give a zero flux, as the
+ * reference's inviscid switch does (its default branch). */
+#define THESEUS_BC_SLIP_WALL 0          /* (0, p* n, 0), p* from the 1-D wall Riemann problem, Flow.hpp:55-170 */
+#define THESEUS_BC_SUPERSONIC_INFLOW 1  /* F*(q, q_bc, n) */
+#define THESEUS_BC_SUPERSONIC_OUTFLOW 2 /* F*(q, q, n), bc_kernels.hpp:236-238 */
+#define THESEUS_BC_PRESCRIBED_STATE 3   /* F*(q, q_bc, n); NS: - n.Fv(q, grad q) */
+#define THESEUS_BC_SYMMETRY 4           /* F*(q, q mirrored about the unit normal, n) */
+#define THESEUS_BC_AXIS 5               /* axisymmetric builds only: rejected (THESEUS_ERR_UNSUPPORTED) */
+#define THESEUS_BC_NO_SLIP_ADIABATIC 6  /* NS wall, bc_kernels.hpp:154-184 */
+#define THESEUS_BC_NO_SLIP_ISOTHERMAL 7 /* NS wall, bc_kernels.hpp:186-216 */
+#define THESEUS_BC_DATA_STRIDE 8
 
 /* ---- precision modes ---------------------------------------------------- */
 #define THESEUS_PRECISION_FP64 0
@@ -125,6 +146,7 @@ typedef struct theseus_case_desc {
 
   int num_bc;
   const int *bc_type;          /* [num_bc] THESEUS_BC_* */
+  const double *bc_data;       /* [num_bc][THESEUS_BC_DATA_STRIDE]; may be NULL when no boundary needs data */
 } theseus_case_desc;
 
 typedef struct theseus_rhs theseus_rhs; /* opaque */
@@ -282,9 +304,11 @@ THESEUS_API int theseus_pool_destroy(theseus_pool *p);
 THESEUS_API int theseus_pool_info(const theseus_pool *p, int *capacity_elements, int *free_elements,
                                   int *largest_free_range, int *active_cases, int *tile, int *xcd_remap);
 /* Admit one case (physics and kernel fields must match the pool's; its
- * geometry/connectivity arrays are host pointers as for theseus_rhs_create),
- * with initial state u0 (its L-vector, host or device memory) and
- * dt = cfl / total stability rate of u0.  Synchronises `stream`. */
+ * geometry/connectivity arrays and boundary table (at most
+ * THESEUS_POOL_MAX_BC boundaries) are host pointers as for
+ * theseus_rhs_create), with initial state u0 (its L-vector, host or device
+ * memory) and dt = cfl / total stability rate of u0.  Synchronises `stream`. */
+#define THESEUS_POOL_MAX_BC 16
 THESEUS_API int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double *u0, double cfl,
                                    int *case_id, double *dt, void *stream);
 /* One SSP-RK3 step of every admitted case (graph replay on `stream`). */

@@ -77,6 +77,34 @@ void oracle_viscous_flux(int dim, double gamma, double Pr, double R, double mu, 
     for (int d = 0; d < MAXDIM; ++d) F[e * MAXDIM + d] = f[e][d];
 }
 
+// Boundary conditions (bc_kernels.hpp, Flow.hpp); gas constants as
+// oracle_viscous_flux; `face_id` selects the surface flux F*.
+double oracle_slipwall_pstar(int dim, double gamma, const double *q, const double *n) {
+  Gas g(dim, gamma, 0.72, 287.05, 0.02, 2.0 / 3.0);
+  return SlipWallPStarNormal(g, q, n);
+}
+
+void oracle_bc_inviscid(int face_id, int type, int dim, double gamma, const double *data, const double *q,
+                        const double *n, double *F) {
+  Gas g(dim, gamma, 0.72, 287.05, 0.02, 2.0 / 3.0);
+  auto face = [&](const double *a, const double *b, const double *nn, double *out) { FaceFlux(face_id, g, a, b, nn, out); };
+  BoundaryInviscidFlux(g, face, type, data, q, n, F);
+}
+
+void oracle_bc_viscous(int face_id, int type, int dim, double gamma, double Pr, double R, double mu, double mu_bulk,
+                       const double *data, const double *q, const double *dx, const double *dy, const double *dz,
+                       const double *n, double *F) {
+  Gas g(dim, gamma, Pr, R, mu, mu_bulk);
+  auto face = [&](const double *a, const double *b, const double *nn, double *out) { FaceFlux(face_id, g, a, b, nn, out); };
+  const double *grads[3] = {dx, dy, dz};
+  BoundaryViscousFlux(g, face, type, data, q, grads, n, F);
+}
+
+void oracle_bc_grad_jump(int type, int dim, double gamma, double R, const double *data, const double *S, double *J) {
+  Gas g(dim, gamma, 0.72, R, 0.02, 2.0 / 3.0);
+  BoundaryGradJump(g, type, data, S, J);
+}
+
 // x, w: Np; D, Dhat, Dhat2: Np*Np stored [i*Np + l]
 void oracle_basis(int p, double *x, double *w, double *D, double *Dhat, double *Dhat2) {
   Basis b(p);

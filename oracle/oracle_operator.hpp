@@ -33,6 +33,7 @@
 
 #include "../include/theseus_rhs.h"
 #include "oracle_physics.hpp"
+#include "oracle_bc.hpp"
 
 namespace oracle {
 
@@ -167,6 +168,7 @@ struct Case {
   std::vector<double> J, adj;  // per element
   std::vector<int> nbr;        // [ne][2*dim]
   std::vector<int> bc_type;
+  std::vector<double> bc_data;  // [num_bc][BC_STRIDE]
   // Persson-Peraire cache (dgsem_cache_utilities.hpp:867-943)
   std::vector<double> modal, keep_M1, keep_M2;
   double adv_scale, diff_scale, surf_scale;
@@ -233,6 +235,8 @@ struct Case {
     adj.assign(d.elem_adj, d.elem_adj + (size_t)ne * dim * dim);
     nbr.assign(d.face_neighbor, d.face_neighbor + (size_t)ne * 2 * dim);
     if (d.num_bc > 0) bc_type.assign(d.bc_type, d.bc_type + d.num_bc);
+    bc_data.assign((size_t)std::max(d.num_bc, 0) * BC_STRIDE, 0.0);
+    if (d.num_bc > 0 && d.bc_data) bc_data.assign(d.bc_data, d.bc_data + (size_t)d.num_bc * BC_STRIDE);
     adv_scale = ReferenceAdvectionSpectralScale(p);
     diff_scale = ReferenceBR1DiffusionSpectralScale(p);
     surf_scale = adv_scale / (1.0 / basis.w[0]);
@@ -580,9 +584,12 @@ inline void EulerRHS(const Case &c, const double *U, double *dudt, const double 
   // boundary faces (EulerOperator_impl.hpp:292-392)
   bool any_bnd = false;
   std::fill(fd.begin(), fd.end(), 0.0);
+  // (ApplyBoundaryConditionInviscid, bc_kernels.hpp:219-287)
+  auto face = [&](const double *a, const double *b, const double *n, double *out) { CaseFaceFlux(c, a, b, n, out); };
   ForEachBoundaryFace(c, [&](int e, int d, int s, int bc) {
     any_bnd = true;
-    if (c.bc_type.at(bc) != THESEUS_BC_SUPERSONIC_OUTFLOW) throw std::runtime_error("unsupported BC in oracle");
+    const int type = c.bc_type.at(bc);
+    const double *data = &c.bc_data[(size_t)bc * BC_STRIDE];
     double n[3];
     const double *a = c.adj_row(e, d);
     for (int k = 0; k < dim; ++k) n[k] = s ? a[k] : -a[k];
@@ -590,7 +597,7 @@ inline void EulerRHS(const Case &c, const double *U, double *dudt, const double 
     for (int t = 0; t < c.nfp; ++t) {
       const int nd = c.node_of(d, s, t);
       c.gather(U, e, nd, qm);
-      CaseFaceFlux(c, qm, qm, n, F);
+      BoundaryInviscidFlux(c.gas, face, type, data, qm, n, F);
       for (int q = 0; q < neq; ++q) fd[q * N + (size_t)e * c.ndof + nd] += scale * F[q];
     }
   });
@@ -599,15 +606,17 @@ inline void EulerRHS(const Case &c, const double *U, double *dudt, const double 
 }
 
 // ---------------------------------------------------------------------------
-// BR1 Navier-Stokes operator (NSOperator_impl.hpp:10-1105), periodic/interior faces only.
+// BR1 Navier-Stokes operator (NSOperator_impl.hpp:10-1105).
 // ---------------------------------------------------------------------------
 inline void NSRHS(const Case &c, const double *U, double *dudt, const double *alpha) {
   const size_t N = c.ntot();
   const int neq = c.neq, dim = c.dim, dof = c.ndof, Np = c.Np;
   const double w0 = c.basis.w[0];
-  bool has_bnd = false;
-  ForEachBoundaryFace(c, [&](int, int, int, int) { has_bnd = true; });
-  if (has_bnd) throw std::runtime_error("oracle: Navier-Stokes boundary conditions are not restated");
+  auto face = [&](const double *a, const double *b, const double *n, double *out) { CaseFaceFlux(c, a, b, n, out); };
+  auto outward = [&](int e, int d, int s, double *n) {
+    const double *a = c.adj_row(e, d);
+    for (int k = 0; k < dim; ++k) n[k] = s ? a[k] : -a[k];
+  };
 
   // ComputeEntropyState (NSOperator_impl.hpp:10-78)
   std::vector<double> S(N * neq);
@@ -679,6 +688,28 @@ inline void NSRHS(const Case &c, const double *U, double *dudt, const double *al
     for (int dd = 0; dd < dim; ++dd)
 #pragma omp parallel for schedule(static)
       for (long v = 0; v < (long)(N * neq); ++v) grad[dd][v] += gi[dd][v];
+  }
+  {  // GradOperator_BoundaryFaces (NSOperator_impl.hpp:286-403): += w (S_bc - S) n
+    std::vector<std::vector<double>> gb(dim, std::vector<double>(N * neq, 0.0));
+    bool any = false;
+    ForEachBoundaryFace(c, [&](int e, int d, int s, int bc) {
+      any = true;
+      const int type = c.bc_type.at(bc);
+      const double *data = &c.bc_data[(size_t)bc * BC_STRIDE];
+      double n[3], Sv[MAXEQ], jump[MAXEQ];
+      outward(e, d, s, n);
+      const double wt = 1.0 / (w0 * c.J[e]);
+      for (int t = 0; t < c.nfp; ++t) {
+        const int nd = c.node_of(d, s, t);
+        for (int qq = 0; qq < neq; ++qq) Sv[qq] = S[qq * N + (size_t)e * dof + nd];
+        BoundaryGradJump(c.gas, type, data, Sv, jump);
+        for (int dd = 0; dd < dim; ++dd)
+          for (int qq = 0; qq < neq; ++qq) gb[dd][qq * N + (size_t)e * dof + nd] += wt * (jump[qq] * n[dd]);
+      }
+    });
+    if (any)
+      for (int dd = 0; dd < dim; ++dd)
+        for (size_t v = 0; v < N * neq; ++v) grad[dd][v] += gb[dd][v];
   }
   // ComputeGradPrimFromGradEntropy (NSOperator_impl.hpp:80-159)
   for (int dd = 0; dd < dim; ++dd)
@@ -780,6 +811,31 @@ inline void NSRHS(const Case &c, const double *U, double *dudt, const double *al
   });
 #pragma omp parallel for schedule(static)
   for (long v = 0; v < (long)(N * neq); ++v) dudt[v] += fd[v];
+  // MultCNS_BoundaryFaces (NSOperator_impl.hpp:717-840): -w * ApplyViscousBoundaryCondition
+  bool any = false;
+  std::fill(fd.begin(), fd.end(), 0.0);
+  ForEachBoundaryFace(c, [&](int e, int d, int s, int bc) {
+    any = true;
+    const int type = c.bc_type.at(bc);
+    const double *data = &c.bc_data[(size_t)bc * BC_STRIDE];
+    double n[3];
+    outward(e, d, s, n);
+    const double scale = -1.0 / (w0 * c.J[e]);
+    for (int t = 0; t < c.nfp; ++t) {
+      const int nd = c.node_of(d, s, t);
+      double q[MAXEQ], gq[3][MAXEQ] = {{0}}, F[MAXEQ];
+      for (int k = 0; k < neq; ++k) {
+        const size_t o = k * N + (size_t)e * dof + nd;
+        q[k] = U[o];
+        for (int dd = 0; dd < dim; ++dd) gq[dd][k] = grad[dd][o];
+      }
+      const double *grads[3] = {gq[0], gq[1], gq[2]};
+      BoundaryViscousFlux(c.gas, face, type, data, q, grads, n, F);
+      for (int k = 0; k < neq; ++k) fd[k * N + (size_t)e * dof + nd] += scale * F[k];
+    }
+  });
+  if (any)
+    for (size_t v = 0; v < N * neq; ++v) dudt[v] += fd[v];
 }
 
 inline void RHS(const Case &c, const double *U, double *dudt, std::vector<double> *alpha_out = nullptr) {
@@ -795,7 +851,7 @@ inline void RHS(const Case &c, const double *U, double *dudt, std::vector<double
   if (alpha_out) *alpha_out = alpha;
 }
 
-// RHSOperator_impl.hpp:10-270 (no prescribed-state boundaries are restated)
+// RHSOperator_impl.hpp:10-270
 inline void EstimateStability(const Case &c, const double *U, double out[3]) {
   const size_t N = c.ntot();
   const int dim = c.dim, neq = c.neq;
@@ -849,9 +905,22 @@ inline void EstimateStability(const Case &c, const double *U, double out[3]) {
       surf = std::max(surf, ws * std::max(std::abs(wm), std::abs(wp)) * c.surf_scale);
     }
   });
-  ForEachBoundaryFace(c, [&](int e, int d, int s, int) {
+  // boundary points (RHSOperator_impl.hpp:176-267): a prescribed or inflow
+  // state adds its own wave speed and, when viscous, its diffusivity with the
+  // element's metric sum (dgsem_cache_utilities.hpp:430-446)
+  ForEachBoundaryFace(c, [&](int e, int d, int s, int bc) {
     const double *a = c.adj_row(e, d);
     const double wt = 1.0 / (w0 * c.J[e]);
+    const int type = c.bc_type.at(bc);
+    const double *ext = &c.bc_data[(size_t)bc * BC_STRIDE];
+    const bool has_ext = type == BC_PRESCRIBED_STATE || type == BC_SUPERSONIC_INFLOW;
+    double metric_sum = 0.0;
+    const double ij = 1.0 / c.J[e];
+    for (int rd = 0; rd < dim; ++rd) {
+      double r2 = 0.0;
+      for (int pd = 0; pd < dim; ++pd) r2 += c.adj_row(e, rd)[pd] * c.adj_row(e, rd)[pd];
+      metric_sum += r2 * ij * ij;
+    }
     for (int t = 0; t < c.nfp; ++t) {
       c.gather(U, e, c.node_of(d, s, t), qm);
       double n2 = 0.0, vn = 0.0;
@@ -860,7 +929,20 @@ inline void EstimateStability(const Case &c, const double *U, double out[3]) {
         n2 += nk * nk;
         vn += c.gas.velocity(qm, k) * nk;
       }
-      const double ws = std::abs(vn) + CaseSoundSpeed(c, qm) * std::sqrt(n2);
+      double ws = std::abs(vn) + CaseSoundSpeed(c, qm) * std::sqrt(n2);
+      if (has_ext) {
+        if (viscous) {
+          const double density = ext[0];
+          const double shear = c.gas.viscosity(ext);
+          const double longitudinal = (2.0 - c.gas.bulk_viscosity(ext)) * shear;
+          const double eff = std::max(std::max(shear, longitudinal) / density,
+                                      c.gas.thermal_conductivity(ext) * c.gas.gamma / (density * c.gas.cp));
+          diff = std::max(diff, c.diff_scale * eff * metric_sum);
+        }
+        double ve = 0.0;
+        for (int k = 0; k < dim; ++k) ve += c.gas.velocity(ext, k) * (s ? a[k] : -a[k]);
+        ws = std::max(ws, std::abs(ve) + CaseSoundSpeed(c, ext) * std::sqrt(n2));
+      }
       surf = std::max(surf, ws * std::abs(wt) * c.surf_scale);
     }
   });

@@ -42,6 +42,11 @@ def load():
     lib.oracle_grad_entropy_to_grad_prim.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp]
     lib.oracle_viscous_flux.argtypes = [C.c_int] + [C.c_double] * 5 + [_dp] * 5
     lib.oracle_basis.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_slipwall_pstar.argtypes = [C.c_int, C.c_double, _dp, _dp]
+    lib.oracle_slipwall_pstar.restype = C.c_double
+    lib.oracle_bc_inviscid.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+    lib.oracle_bc_viscous.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_double] * 5 + [_dp] * 7
+    lib.oracle_bc_grad_jump.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
     lib.oracle_case_create.argtypes = [C.c_void_p]
     lib.oracle_case_create.restype = C.c_void_p
     lib.oracle_case_destroy.argtypes = [C.c_void_p]
@@ -111,6 +116,41 @@ def viscous_flux(dim, gamma, Pr, R, mu, mu_bulk, q, dx, dy, dz):
     F = np.zeros(15)
     load().oracle_viscous_flux(dim, gamma, Pr, R, mu, mu_bulk, _p(q), _p(dx), _p(dy), _p(dz), _p(F))
     return F.reshape(5, 3)[: dim + 2, :dim]
+
+
+def _bc_data(data):
+    d = np.zeros(8)
+    if data is not None:
+        v = np.asarray(data, dtype=np.float64).ravel()
+        d[: v.size] = v
+    return d
+
+
+def slipwall_pstar(dim, gamma, q, n):
+    q, n = arr(q), arr(n)
+    return load().oracle_slipwall_pstar(dim, gamma, _p(q), _p(n))
+
+
+def bc_inviscid(face_id, bc_type, dim, gamma, q, n, data=None):
+    q, n, d = arr(q), arr(n), _bc_data(data)
+    F = np.zeros(dim + 2)
+    load().oracle_bc_inviscid(face_id, bc_type, dim, gamma, _p(d), _p(q), _p(n), _p(F))
+    return F
+
+
+def bc_viscous(face_id, bc_type, dim, gamma, Pr, R, mu, mu_bulk, q, dx, dy, dz, n, data=None):
+    q, dx, dy, dz, n, d = arr(q), arr(dx), arr(dy), arr(dz), arr(n), _bc_data(data)
+    F = np.zeros(dim + 2)
+    load().oracle_bc_viscous(face_id, bc_type, dim, gamma, Pr, R, mu, mu_bulk, _p(d), _p(q), _p(dx), _p(dy), _p(dz),
+                             _p(n), _p(F))
+    return F
+
+
+def bc_grad_jump(bc_type, dim, gamma, R, S, data=None):
+    S, d = arr(S), _bc_data(data)
+    J = np.zeros(dim + 2)
+    load().oracle_bc_grad_jump(bc_type, dim, gamma, R, _p(d), _p(S), _p(J))
+    return J
 
 
 def basis(p):

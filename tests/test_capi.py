@@ -72,14 +72,51 @@ def test_create_rejects_inconsistent_connectivity():
     assert "symmetric" in lib.theseus_last_error().decode()
 
 
-def test_navier_stokes_boundaries_are_rejected():
+_GOOD_STATE = (1.2, 0.3, -0.1, 2.5 / 0.4 + 0.5 * (0.09 + 0.01) / 1.2)
+
+
+@pytest.mark.parametrize("side,code,msg", [
+    ((A.BC_AXIS, ()), A.ERR_UNSUPPORTED, "axisymmetric"),
+    ((9, ()), A.ERR_INVALID_ARGUMENT, "unknown boundary type"),
+    ((A.BC_PRESCRIBED_STATE, (1.0, 0.0, 0.0, -1.0)), A.ERR_INVALID_ARGUMENT, "positive density and pressure"),
+    ((A.BC_SUPERSONIC_INFLOW, (-1.0, 0.0, 0.0, 3.0)), A.ERR_INVALID_ARGUMENT, "positive density and pressure"),
+    ((A.BC_NO_SLIP_ISOTHERMAL, (0.0, 0.0, 0.0)), A.ERR_INVALID_ARGUMENT, "temperature must be positive"),
+    ((A.BC_NO_SLIP_ADIABATIC, (0.0, float("nan"), 0.0)), A.ERR_INVALID_ARGUMENT, "finite"),
+])
+def test_create_validates_boundary_conditions(side, code, msg):
+    """The reference's configuration checks (Simulation.cpp:555-820) at create time."""
     lib = A.load()
     rt = RunTime(dim=2, order=3, flow_model=A.FLOW_NAVIER_STOKES, volume_flux=A.VOLUME_RANOCHA,
                  surface_flux=A.SURFACE_LLF)
-    mesh = box_mesh(2, (3, 2), 0.0, 1.0, periodic=False)
+    sides = [A.BC_SLIP_WALL] * 4
+    sides[2] = side
+    mesh = box_mesh(2, (3, 2), 0.0, 1.0, periodic=(True, False), side_bc=sides)
     d, keep = build_descriptor(rt, mesh)
     h = C.c_void_p()
-    assert lib.theseus_rhs_create(C.byref(d), C.byref(h)) == A.ERR_UNSUPPORTED
+    assert lib.theseus_rhs_create(C.byref(d), C.byref(h)) == code and not h.value
+    assert msg in lib.theseus_last_error().decode()
+
+
+def test_boundary_payload_is_required_and_navier_stokes_boundaries_pass_validation():
+    lib = A.load()
+    rt = RunTime(dim=2, order=3, flow_model=A.FLOW_NAVIER_STOKES, volume_flux=A.VOLUME_RANOCHA,
+                 surface_flux=A.SURFACE_LLF)
+    sides = [A.BC_SLIP_WALL, A.BC_SUPERSONIC_OUTFLOW, (A.BC_PRESCRIBED_STATE, _GOOD_STATE),
+             (A.BC_NO_SLIP_ISOTHERMAL, (0.1, 0.0, 300.0))]
+    mesh = box_mesh(2, (3, 2), 0.0, 1.0, periodic=False, side_bc=sides)
+    assert list(mesh.bc_type) == [0, 2, 3, 7] and mesh.bc_data.shape == (4, A.BC_DATA_STRIDE)
+    assert (mesh.face_neighbor[:, 2] == -3).sum() == 3  # y-low side of the bottom row: boundary index 2
+    d, keep = build_descriptor(rt, mesh)
+    d.bc_data = C.POINTER(C.c_double)()
+    h = C.c_void_p()
+    assert lib.theseus_rhs_create(C.byref(d), C.byref(h)) == A.ERR_INVALID_ARGUMENT
+    assert "needs bc_data" in lib.theseus_last_error().decode()
+    d, keep = build_descriptor(rt, mesh)
+    rc = lib.theseus_rhs_create(C.byref(d), C.byref(h))
+    # without a GPU the device allocation is what fails; validation passed
+    assert rc not in (A.ERR_INVALID_ARGUMENT, A.ERR_UNSUPPORTED), lib.theseus_last_error().decode()
+    if h.value:
+        lib.theseus_rhs_destroy(h)
 
 
 def test_null_arguments():

@@ -1,7 +1,7 @@
 """Edge cases and full sizes on the GPU against the CPU FP64 oracle: the
 smallest and largest compiled orders (p = 1 and p = 5, 2-D and 3-D, Euler
 and Navier-Stokes, blending on and off), an element that is its own
-neighbour across a periodic direction, partially filled 2-D workgroups; one
+neighbour across a periodic direction, partially filled 2-D and 3-D workgroups; one
 full-size RHS of the 2-D Riemann workload (128^2 quads, p = 4, blending,
 transmissive boundaries) to 1e-12 and of the 3-D Navier-Stokes Taylor-Green
 workload (32^3 hexes, p = 3) to the bound its low-Mach conditioning allows
@@ -60,6 +60,15 @@ def test_partially_filled_2d_workgroups(oracle):
     _say("case edge ragged 2-D blocks")
     rt, mesh = make_case(2, 1, (7, 3), "roe", blending=True)
     _parity(oracle, rt, mesh, smooth_state(mesh, 1, seed=23, amp=0.3))
+
+
+@pytest.mark.parametrize("flow,blending", [(A.FLOW_NAVIER_STOKES, False), (A.FLOW_EULER, True)])
+def test_partially_filled_3d_p3_workgroups(oracle, flow, blending):
+    """3-D p = 3 packs two elements per workgroup; 3 x 1 x 3 = 9 elements
+    leave the last workgroup half empty."""
+    _say(f"case edge ragged 3-D p=3 blocks flow={flow} blending={blending}")
+    rt, mesh = make_case(3, 3, (3, 1, 3), "ranocha_llf", flow=flow, blending=blending)
+    _parity(oracle, rt, mesh, smooth_state(mesh, 3, seed=24, amp=0.3))
 
 
 # Low-Mach Navier-Stokes at full resolution: the momentum RHS is the

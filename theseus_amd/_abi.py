@@ -35,7 +35,16 @@ SURFACE_HLL = 4
 SURFACE_LLF = 5
 SURFACE_LTEEC = 6
 
+BC_SLIP_WALL = 0
+BC_SUPERSONIC_INFLOW = 1
 BC_SUPERSONIC_OUTFLOW = 2
+BC_PRESCRIBED_STATE = 3
+BC_SYMMETRY = 4
+BC_AXIS = 5
+BC_NO_SLIP_ADIABATIC = 6
+BC_NO_SLIP_ISOTHERMAL = 7
+BC_DATA_STRIDE = 8
+POOL_MAX_BC = 16
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -88,6 +97,7 @@ class CaseDesc(C.Structure):
         ("face_neighbor", C.POINTER(C.c_int)),
         ("num_bc", C.c_int),
         ("bc_type", C.POINTER(C.c_int)),
+        ("bc_data", C.POINTER(C.c_double)),
     ]
 
 

@@ -275,6 +275,8 @@ struct theseus_rhs {
   double adv_scale, diff_scale, surf_scale;
   double *d_J = nullptr, *d_adj = nullptr;
   int *d_nbr = nullptr;
+  int *d_bc_type = nullptr;     // [max(num_bc, 1)]
+  double *d_bc_data = nullptr;  // [max(num_bc, 1)][THESEUS_BC_DATA_STRIDE]
   double *d_alpha = nullptr;
   double *d_Fv = nullptr;       // NS: viscous volume divergence, then n.Fv at face points
   double *d_pref = nullptr;     // mixed mode: background pressure (one double)
@@ -315,6 +317,8 @@ struct theseus_rhs {
     a.J = d_J;
     a.adj = d_adj;
     a.nbr = d_nbr;
+    a.bc_type = d_bc_type;
+    a.bc_data = d_bc_data;
     a.gas = gas;
     a.gas32 = gas32;
     a.pref = d_pref;
@@ -366,6 +370,48 @@ static int upload(T **dst, const T *src, size_t n) {
   return THESEUS_OK;
 }
 
+// One boundary's type and payload, as the reference's configuration checks
+// them (Simulation.cpp:555-820: positive wall temperature, a physical
+// exterior state).  Axis boundaries need the axisymmetric formulation.
+// The boundary table on the device (at least one entry, zeros where the
+// descriptor has no payload).
+static int upload_bc(int **type, double **data, const theseus_case_desc &d) {
+  const int nb = std::max(d.num_bc, 1);
+  std::vector<int> t(nb, THESEUS_BC_SUPERSONIC_OUTFLOW);
+  std::vector<double> v((size_t)nb * THESEUS_BC_DATA_STRIDE, 0.0);
+  for (int b = 0; b < d.num_bc; ++b) t[b] = d.bc_type[b];
+  if (d.bc_data && d.num_bc > 0) std::copy(d.bc_data, d.bc_data + (size_t)d.num_bc * THESEUS_BC_DATA_STRIDE, v.begin());
+  int rc = upload(type, t.data(), t.size());
+  if (rc) return rc;
+  return upload(data, v.data(), v.size());
+}
+
+static int validate_bc(const theseus_case_desc &d, int b) {
+  const int t = d.bc_type[b];
+  const std::string who = "boundary " + std::to_string(b) + ": ";
+  if (t == THESEUS_BC_AXIS)
+    return fail(THESEUS_ERR_UNSUPPORTED, who + "axis boundaries need the axisymmetric formulation, which is not built");
+  if (t < THESEUS_BC_SLIP_WALL || t > THESEUS_BC_NO_SLIP_ISOTHERMAL)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, who + "unknown boundary type " + std::to_string(t));
+  const bool needs_data = t == THESEUS_BC_SUPERSONIC_INFLOW || t == THESEUS_BC_PRESCRIBED_STATE ||
+                          t == THESEUS_BC_NO_SLIP_ADIABATIC || t == THESEUS_BC_NO_SLIP_ISOTHERMAL;
+  if (!needs_data) return THESEUS_OK;
+  if (!d.bc_data) return fail(THESEUS_ERR_INVALID_ARGUMENT, who + "this boundary type needs bc_data");
+  const double *v = d.bc_data + (size_t)b * THESEUS_BC_DATA_STRIDE;
+  const int nv = (t == THESEUS_BC_SUPERSONIC_INFLOW || t == THESEUS_BC_PRESCRIBED_STATE) ? d.dim + 2 : d.dim + 1;
+  for (int k = 0; k < nv; ++k)
+    if (!std::isfinite(v[k])) return fail(THESEUS_ERR_INVALID_ARGUMENT, who + "payload must be finite");
+  if (t == THESEUS_BC_SUPERSONIC_INFLOW || t == THESEUS_BC_PRESCRIBED_STATE) {
+    double m2 = 0.0;
+    for (int k = 0; k < d.dim; ++k) m2 += v[1 + k] * v[1 + k];
+    const double p = (d.gamma - 1.0) * (v[d.dim + 1] - 0.5 * m2 / v[0]);
+    if (!(v[0] > 0) || !(p > 0)) return fail(THESEUS_ERR_INVALID_ARGUMENT, who + "exterior state needs positive density and pressure");
+  }
+  if (t == THESEUS_BC_NO_SLIP_ISOTHERMAL && !(v[d.dim] > 0))
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, who + "wall temperature must be positive");
+  return THESEUS_OK;
+}
+
 static int validate_mesh(const theseus_case_desc &d, bool &has_boundary) {
   const int ne = d.num_elements, dim = d.dim;
   has_boundary = false;
@@ -382,12 +428,13 @@ static int validate_mesh(const theseus_case_desc &d, bool &has_boundary) {
       has_boundary = true;
       const int bc = -(nb + 1);
       if (bc >= d.num_bc || !d.bc_type) return fail(THESEUS_ERR_INVALID_ARGUMENT, "boundary index out of range");
-      if (d.bc_type[bc] != THESEUS_BC_SUPERSONIC_OUTFLOW)
-        return fail(THESEUS_ERR_UNSUPPORTED, "only supersonic-outflow (transmissive) boundaries are implemented");
     }
   }
-  if (has_boundary && d.flow_model == THESEUS_FLOW_NAVIER_STOKES)
-    return fail(THESEUS_ERR_UNSUPPORTED, "Navier-Stokes boundary conditions are not implemented (periodic only)");
+  if (d.num_bc < 0) return fail(THESEUS_ERR_INVALID_ARGUMENT, "num_bc must not be negative");
+  for (int b = 0; b < d.num_bc; ++b) {
+    const int rc = validate_bc(d, b);
+    if (rc) return rc;
+  }
   for (int e = 0; e < ne; ++e)
     if (!(d.elem_jac[e] > 0) || !std::isfinite(d.elem_jac[e]))
       return fail(THESEUS_ERR_INVALID_ARGUMENT, "element Jacobian must be positive and finite");
@@ -465,6 +512,10 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
   bool has_boundary = false;
   int rc0 = validate_mesh(d, has_boundary);
   if (rc0) return rc0;
+  if (lt)
+    for (int b = 0; b < d.num_bc; ++b)
+      if (d.bc_type[b] != THESEUS_BC_SUPERSONIC_OUTFLOW)
+        return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas supports supersonic-outflow boundaries only");
   const int ne = d.num_elements, dim = d.dim;
 
   theseus_rhs *h = new theseus_rhs();
@@ -473,6 +524,7 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
   h->d.elem_adj = nullptr;
   h->d.face_neighbor = nullptr;
   h->d.bc_type = nullptr;
+  h->d.bc_data = nullptr;
   h->fk = fk;
   h->sk = sk;
   h->lk = lk;
@@ -496,7 +548,7 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
   const size_t M = (size_t)h->Ntot * h->neq;
   if ((rc = upload(&h->d_J, d.elem_jac, ne)) || (rc = upload(&h->d_adj, d.elem_adj, (size_t)ne * dim * dim)) ||
       (rc = upload(&h->d_nbr, d.face_neighbor, (size_t)ne * 2 * dim)) ||
-      (rc = upload<double>(&h->d_alpha, nullptr, ne)) ||
+      (rc = upload_bc(&h->d_bc_type, &h->d_bc_data, d)) || (rc = upload<double>(&h->d_alpha, nullptr, ne)) ||
       (rc = upload<unsigned long long>(&h->d_rates, nullptr, 4)) || (rc = upload<double>(&h->d_pref, nullptr, 1)) ||
       (rc = upload<double>(&h->d_stage[0], nullptr, M)) || (rc = upload<double>(&h->d_stage[1], nullptr, M))) {
     theseus_rhs_destroy(h);
@@ -590,6 +642,8 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_J);
   hipFree(h->d_adj);
   hipFree(h->d_nbr);
+  hipFree(h->d_bc_type);
+  hipFree(h->d_bc_data);
   hipFree(h->d_alpha);
   hipFree(h->d_Fv);
   hipFree(h->d_pref);
@@ -662,6 +716,8 @@ static void enqueue_stability(theseus_rhs *h, const double *u, hipStream_t s) {
   a.J = h->d_J;
   a.adj = h->d_adj;
   a.nbr = h->d_nbr;
+  a.bc_type = h->d_bc_type;
+  a.bc_data = h->d_bc_data;
   a.gas = h->gas;
   a.w0 = h->basis.w[0];
   a.lte = h->ltev;
@@ -825,6 +881,9 @@ struct theseus_pool {
   double *d_J = nullptr, *d_adj = nullptr, *d_alpha = nullptr, *d_Fv = nullptr, *d_U = nullptr;
   double *d_stage[2] = {nullptr, nullptr};
   int *d_nbr = nullptr, *d_elem_case = nullptr;
+  // boundary table: case c's boundary b is entry c * THESEUS_POOL_MAX_BC + b
+  int *d_bc_type = nullptr;
+  double *d_bc_data = nullptr;
   double *d_case_dt = nullptr, *d_case_pref = nullptr;
   unsigned long long *d_rates = nullptr;
   double *d_meas = nullptr;  // integral measures: partials of the largest case, then the results
@@ -859,6 +918,8 @@ struct theseus_pool {
     a.J = d_J;
     a.adj = d_adj;
     a.nbr = d_nbr;
+    a.bc_type = d_bc_type;
+    a.bc_data = d_bc_data;
     a.gas = gas;
     a.gas32 = gas32;
     a.elem_case = d_elem_case;
@@ -1123,6 +1184,8 @@ int theseus_pool_create(const theseus_pool_desc *desc, theseus_pool **out) {
   if ((rc = upload<double>(&p->d_J, nullptr, p->cap)) || (rc = upload<double>(&p->d_adj, nullptr, (size_t)p->cap * dim * dim)) ||
       (rc = upload<int>(&p->d_nbr, nullptr, (size_t)p->cap * 2 * dim)) ||
       (rc = upload<int>(&p->d_elem_case, nullptr, p->cap)) || (rc = upload<double>(&p->d_alpha, nullptr, p->cap)) ||
+      (rc = upload<int>(&p->d_bc_type, nullptr, (size_t)d.max_cases * THESEUS_POOL_MAX_BC)) ||
+      (rc = upload<double>(&p->d_bc_data, nullptr, (size_t)d.max_cases * THESEUS_POOL_MAX_BC * THESEUS_BC_DATA_STRIDE)) ||
       (rc = upload<double>(&p->d_case_dt, nullptr, d.max_cases)) ||
       (rc = upload<double>(&p->d_case_pref, nullptr, d.max_cases)) ||
       (rc = upload<unsigned long long>(&p->d_rates, nullptr, 4)) || (rc = upload<double>(&p->d_U, nullptr, M)) ||
@@ -1173,6 +1236,8 @@ int theseus_pool_destroy(theseus_pool *p) {
   hipFree(p->d_J);
   hipFree(p->d_adj);
   hipFree(p->d_nbr);
+  hipFree(p->d_bc_type);
+  hipFree(p->d_bc_data);
   hipFree(p->d_elem_case);
   hipFree(p->d_alpha);
   hipFree(p->d_case_dt);
@@ -1223,6 +1288,8 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
   bool has_boundary = false;
   int rc = validate_mesh(d, has_boundary);
   if (rc) return rc;
+  if (d.num_bc > THESEUS_POOL_MAX_BC)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "a pooled case has at most " + std::to_string(THESEUS_POOL_MAX_BC) + " boundaries");
   int cid = -1;
   for (int i = 0; i < (int)p->cases.size(); ++i)
     if (!p->cases[i].active) {
@@ -1234,9 +1301,21 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
   const int start = pool_find_range(p, ne);
   if (start < 0) return fail(THESEUS_ERR_CAPACITY, "no free range of " + std::to_string(ne) + " elements");
   hipStream_t s = (hipStream_t)stream;
-  // geometry and connectivity at the slot offset (boundary codes kept)
+  // geometry and connectivity at the slot offset; boundary codes point into
+  // the case's rows of the pool's boundary table
   std::vector<int> nbr((size_t)ne * 2 * dim), owner(ne, cid);
-  for (size_t f = 0; f < nbr.size(); ++f) nbr[f] = d.face_neighbor[f] >= 0 ? d.face_neighbor[f] + start : d.face_neighbor[f];
+  const int boff = cid * THESEUS_POOL_MAX_BC;
+  for (size_t f = 0; f < nbr.size(); ++f) {
+    const int v = d.face_neighbor[f];
+    nbr[f] = v >= 0 ? v + start : -(boff - (v + 1) + 1);
+  }
+  if (d.num_bc > 0) {
+    std::vector<double> bdat((size_t)d.num_bc * THESEUS_BC_DATA_STRIDE, 0.0);
+    if (d.bc_data) std::copy(d.bc_data, d.bc_data + bdat.size(), bdat.begin());
+    HIP_TRY(hipMemcpyAsync(p->d_bc_type + boff, d.bc_type, d.num_bc * sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(p->d_bc_data + (size_t)boff * THESEUS_BC_DATA_STRIDE, bdat.data(), bdat.size() * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+  }
   HIP_TRY(hipMemcpyAsync(p->d_J + start, d.elem_jac, ne * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(p->d_adj + (size_t)start * dim * dim, d.elem_adj, (size_t)ne * dim * dim * sizeof(double),
                          hipMemcpyHostToDevice, s));
@@ -1260,6 +1339,8 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
   a.J = p->d_J;
   a.adj = p->d_adj;
   a.nbr = p->d_nbr;
+  a.bc_type = p->d_bc_type;
+  a.bc_data = p->d_bc_data;
   a.gas = p->gas;
   a.w0 = p->basis.w[0];
   p->sk->stability(a, (ne + p->sk->epb - 1) / p->sk->epb, s);

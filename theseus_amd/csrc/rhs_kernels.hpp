@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bc_device.hpp"
 #include "device_lte.hpp"
 #include "device_physics.hpp"
 
@@ -38,7 +39,10 @@ struct Shape {
   static constexpr int NDOF = DIM == 2 ? Np * Np : Np * Np * Np;
   static constexpr int NFP = DIM == 2 ? Np : Np * Np;
   static constexpr int NEQ = DIM + 2;
-  static constexpr int EPB = EPBX > 0 ? EPBX : (NDOF >= 64 ? 1 : 64 / NDOF);  // elements per workgroup
+  // elements per workgroup: one wave's worth of nodes, and two elements of 64
+  // nodes (3-D p = 3), whose pair and face lists (288 and 96 items) then
+  // end on a wave boundary every other element instead of every element
+  static constexpr int EPB = EPBX > 0 ? EPBX : (NDOF == 64 ? 2 : NDOF > 64 ? 1 : 64 / NDOF);
   static constexpr int TPB = ((EPB * NDOF + 63) / 64) * 64;
   static constexpr int NPOW2 = NDOF <= 8 ? 8 : NDOF <= 16 ? 16 : NDOF <= 32 ? 32 : NDOF <= 64 ? 64 : NDOF <= 128 ? 128 : 256;
 };
@@ -68,7 +72,9 @@ struct RHSArgs {
   double alpha_max, alpha_min, sharp, thresh;
   const double *J;      // [ne]
   const double *adj;    // [ne][DIM][DIM]
-  const int *nbr;       // [ne][2*DIM]
+  const int *nbr;       // [ne][2*DIM]; -(b+1) on a face of boundary b
+  const int *bc_type;   // [nbc] boundary type (BC_*)
+  const double *bc_data;  // [nbc][BC_STRIDE] boundary payload
   GasC<double> gas;
   GasC<float> gas32;  // the same constants rounded to FP32 (mixed precision)
   const double *pref; // mixed precision: background pressure left out of the momentum fluxes (device scalar)
@@ -497,7 +503,10 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
     const int f = fl / NF, t = fl - f * NF;
     const int d = f >> 1, s = f & 1;
     const int nbs = max(fnb[r], 0);
-    const size_t nbase = (size_t)nbs * ND + face_node<DIM, Np>(d, 1 - s, t);
+    // a boundary point loads the element's own trace (FP64), the input of
+    // its exterior state and wall pressure
+    const size_t nbase = fnb[r] >= 0 ? (size_t)nbs * ND + face_node<DIM, Np>(d, 1 - s, t)
+                                     : (size_t)min(ebase_blk + pe, a.ne - 1) * ND + face_node<DIM, Np>(d, s, t);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
     if constexpr (VISC) {
@@ -703,27 +712,50 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
         for (int dd = 0; dd < DIM; ++dd) n[dd] = R(sm.nrm[pe][f][dd]);
         // One flux evaluation per face point, operands put in canonical
         // (low side first) order by selects, so that lanes on the two sides
-        // of the element do not diverge.  A boundary point (supersonic
-        // outflow, bc_kernels.hpp:236-238) evaluates F*(q, q, n_out).
+        // of the element do not diverge.  A boundary point evaluates
+        // F*(q, q_ext, n_out) (ApplyBoundaryConditionInviscid,
+        // bc_kernels.hpp:219-287) with the interior trace first, or the
+        // wall flux (0, p* n_out, 0).
         const bool interior = fnb[r] >= 0;
-        Pt<R, DIM> oth;  // the neighbour's trace (loaded at the top)
+        double qx[NQ];  // FP64 exterior state: the neighbour's trace, or the boundary's
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) oth.q[q] = interior ? R(qn[r][q]) : own.q[q];
+        for (int q = 0; q < NQ; ++q) qx[q] = qn[r][q];
+        int bct = BC_SUPERSONIC_OUTFLOW;
+        const double *bcd = nullptr;
+        if (!interior) {
+          const int b = -(fnb[r] + 1);
+          bct = a.bc_type[b];
+          bcd = a.bc_data + (size_t)b * BC_STRIDE;
+          bc_exterior<DIM>(bct, bcd, sm.nrm[pe][f], qx);
+        }
+        Pt<R, DIM> oth;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) oth.q[q] = R(qx[q]);
         derive(g, oth);
         if constexpr (!FP64) {
-          // pressure from the FP64 trace, as for the staged nodes
+          // pressure from the FP64 state, as for the staged nodes
           Pt<double, DIM> o64;
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) o64.q[q] = qn[r][q];
+          for (int q = 0; q < NQ; ++q) o64.q[q] = qx[q];
           derive(a.gas, o64);
-          oth.p = interior ? R(o64.p) : own.p;
-          oth.pd = interior ? R(o64.p - pref) : own.pd;
+          oth.p = R(o64.p);
+          oth.pd = R(o64.p - pref);
         }
         Pt<R, DIM> pl, pr;
-        pt_select(s != 0, own, oth, pl);
-        pt_select(s != 0, oth, own, pr);
+        const bool own_left = s != 0 || !interior;
+        pt_select(own_left, own, oth, pl);
+        pt_select(own_left, oth, own, pr);
         R F[NQ];
         surface_flux<SURF, R, DIM>(g, pl, pr, n, F);
+        if (!interior && (bc_wall_pressure(bct, VISC) || bc_zero_flux(bct, VISC))) {
+          // SlipWallInviscidFluxKernel (bc_kernels.hpp:138-152) and the
+          // walls' pressure term (:181-183, :212-214), from the FP64 trace
+          const double pw = bc_zero_flux(bct, VISC) ? 0.0 : slipwall_pstar<DIM>(a.gas, qn[r], sm.nrm[pe][f]) - pref;
+          F[0] = R(0);
+#pragma unroll
+          for (int dd = 0; dd < DIM; ++dd) F[1 + dd] = bc_zero_flux(bct, VISC) ? R(0) : R(pw * sm.nrm[pe][f][dd]);
+          F[DIM + 1] = R(0);
+        }
         if constexpr (VISC) {
           // F* - 0.5 n.(Fv_L + Fv_R)   (DGSEMIntegrator.hpp:261-316); the own
           // side's -0.5 n.Fv was folded into Vd by the gradient kernel
@@ -796,6 +828,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
   __shared__ R sD[Np * Np];
   __shared__ R sDh[Np * Np];
   __shared__ ElemMeta<DIM, EPB> sm;
+  __shared__ int sfnb[EPB][2 * DIM];  // face neighbours (boundary codes < 0)
   R (*sS)[NQ][ND] = reinterpret_cast<R (*)[NQ][ND]>(sbuf);
   R (*sjump)[NFACE][NQ] = reinterpret_cast<R (*)[NFACE][NQ]>(sbuf + EPB * NQ * ND);
   R (*sG)[DIM][NVD][ND] = reinterpret_cast<R (*)[DIM][NVD][ND]>(sbuf);
@@ -827,6 +860,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
     const int pe = fi / NFACE;
     const int ee = min(ebase_blk + pe, a.ne - 1);
     fnb[r] = a.nbr[(size_t)ee * 2 * DIM + (fi - pe * NFACE) / NF];
+    const int fu = tid + r * TPB;  // unclamped: one writer per face
+    if (fu < NFT && (fu - pe * NFACE) % NF == 0) sfnb[pe][(fu - pe * NFACE) / NF] = fnb[r];
   }
   MetaRegs<DIM, EPB, TPB> mr;
   meta_load_index<DIM, EPB, TPB>(a, ebase_blk, mr);
@@ -870,24 +905,35 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
   lds_sync();
 
   // canonical interior-face jumps 0.5*(S_R - S_L) (AssembleGradInteriorFacePointKernel);
-  // boundary faces do not occur (Navier-Stokes boundaries are rejected at create time)
+  // at a boundary point the jump S_bc - S along the outward normal
+  // (ComputeBdrFaceGradFlux, bc_kernels.hpp:32-135)
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
     const int fi = tid + r * TPB;
     const int pe = fi / NFACE;
     const int fl = fi - pe * NFACE;
-    if (fi < NFT && ebase_blk + pe < a.ne && fnb[r] >= 0) {
+    if (fi < NFT && ebase_blk + pe < a.ne) {
       const int f = fl / NF, t = fl - f * NF;
       const int d = f >> 1, s = f & 1;
       const int on = face_node<DIM, Np>(d, s, t);
-      R qr[NQ], Sn[NQ];
+      if (fnb[r] >= 0) {
+        R qr[NQ], Sn[NQ];
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) qr[c] = R(qn[r][c]);
-      entropy_state<R, DIM>(g, qr, Sn);
+        for (int c = 0; c < NQ; ++c) qr[c] = R(qn[r][c]);
+        entropy_state<R, DIM>(g, qr, Sn);
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) {
-        const R so = sS[pe][c][on];
-        sjump[pe][fl][c] = s ? R(0.5) * (Sn[c] - so) : R(0.5) * (so - Sn[c]);
+        for (int c = 0; c < NQ; ++c) {
+          const R so = sS[pe][c][on];
+          sjump[pe][fl][c] = s ? R(0.5) * (Sn[c] - so) : R(0.5) * (so - Sn[c]);
+        }
+      } else {
+        const int b = -(fnb[r] + 1);
+        R So[NQ], jb[NQ];
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) So[c] = sS[pe][c][on];
+        bc_grad_jump<R, DIM>(g, a.bc_type[b], a.bc_data + (size_t)b * BC_STRIDE, So, jb);
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) sjump[pe][fl][c] = jb[c];
       }
     }
   }
@@ -973,14 +1019,28 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
         R n[DIM];
 #pragma unroll
         for (int dd = 0; dd < DIM; ++dd) n[dd] = R(sm.nrm[el][f][dd]);
+        R vn[NVD];
 #pragma unroll
         for (int c = 0; c < NVD; ++c) {
           R v = n[0] * vf[vis_index<DIM>(1 + c, 0)];
 #pragma unroll
           for (int dd = 1; dd < DIM; ++dd) v += n[dd] * vf[vis_index<DIM>(1 + c, dd)];
           Gf_out[c * Gtot + gi] = v;
+          vn[c] = v;
+        }
+        const int fc = sfnb[el][f];
+        if (fc >= 0) {
           // face term scale: -w on the element's high face, +w on its low face
-          hv[c] += (hi ? wown : -wown) * (R(0.5) * v);
+#pragma unroll
+          for (int c = 0; c < NVD; ++c) hv[c] += (hi ? wown : -wown) * (R(0.5) * vn[c]);
+        } else {
+          // the boundary's viscous share, -w times the viscous part of its
+          // flux (n is the outward area vector here)
+          const int b = -(fc + 1);
+          R bs[NVD];
+          bc_viscous_share<R, DIM>(a.bc_type[b], a.bc_data + (size_t)b * BC_STRIDE, vn, th.u, n, bs);
+#pragma unroll
+          for (int c = 0; c < NVD; ++c) hv[c] += wown * bs[c];
         }
       }
     }
@@ -1041,6 +1101,8 @@ struct StabArgs {
   const double *J;
   const double *adj;
   const int *nbr;
+  const int *bc_type;     // [nbc]
+  const double *bc_data;  // [nbc][BC_STRIDE]
   GasC<double> gas;
   double w0;
   lte::View lte;  // tabulated LTE gas: sound speed from the table when lte.nx > 0
@@ -1132,6 +1194,29 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
           const double c_o = a.lte.nx > 0 ? sqrt(lte::state_eval<DIM>(a.lte, o.q).c2()) : sound_speed(g, o);
           ws = fmax(ws, fabs(vo) + c_o * nm);
           wmax = fmax(wmax, fabs(1.0 / (a.w0 * a.J[nb])));
+        } else {
+          // an inflow or prescribed exterior state adds its wave speed and,
+          // when viscous, its diffusivity with the element's metric sum
+          // (RHSOperator_impl.hpp:176-267)
+          const int b = -(nb + 1);
+          const int bt = a.bc_type[b];
+          if (bt == BC_SUPERSONIC_INFLOW || bt == BC_PRESCRIBED_STATE) {
+            Pt<double, DIM> o;
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) o.q[c] = a.bc_data[(size_t)b * BC_STRIDE + c];
+            derive(g, o);
+            double vo = 0.0;
+#pragma unroll
+            for (int dd = 0; dd < DIM; ++dd) vo += o.v[dd] * n[dd];
+            ws = fmax(ws, fabs(vo) + sound_speed(g, o) * nm);
+            if (a.viscous) {
+              const double rho_o = o.q[0];
+              const double longitudinal = (2.0 - g.mu_bulk) * g.mu;
+              const double md = fmax(g.mu, longitudinal) / rho_o;
+              const double td = g.mu * g.cp * g.PrInverse * g.gamma / (rho_o * g.cp);
+              diff = fmax(diff, a.diff_scale * fmax(md, td) * msum);
+            }
+          }
         }
         surf = fmax(surf, ws * wmax * a.surf_scale);
       }

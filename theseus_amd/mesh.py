@@ -15,6 +15,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+BC_DATA_STRIDE = 8  # doubles of payload per boundary index (THESEUS_BC_DATA_STRIDE)
+
 
 @dataclass
 class BoxMesh:
@@ -27,6 +29,7 @@ class BoxMesh:
     elem_adj: np.ndarray = field(repr=False)
     face_neighbor: np.ndarray = field(repr=False)
     bc_type: np.ndarray = field(repr=False)
+    bc_data: np.ndarray = field(default=None, repr=False)  # (nbc, BC_DATA_STRIDE) payload, None = zeros
 
     @property
     def num_elements(self) -> int:
@@ -116,11 +119,16 @@ def refine_breaks(base, levels: int) -> np.ndarray:
     return b
 
 
-def box_mesh(dim: int, n, lower, upper, periodic=True, bc: int = 2) -> BoxMesh:
+def box_mesh(dim: int, n, lower, upper, periodic=True, bc: int = 2, side_bc=None) -> BoxMesh:
     """Cartesian box of n[0] x n[1] (x n[2]) affine elements.
 
     periodic: bool or per-direction tuple.  Non-periodic faces get boundary
     index 0 whose type is `bc` (default: supersonic outflow = transmissive).
+    side_bc: optional sequence of 2*dim entries, one per box side in the
+    order (x low, x high, y low, y high, z low, z high), each a boundary
+    type or a (type, data) pair with up to BC_DATA_STRIDE payload values;
+    every non-periodic side then gets a boundary index of its own (numbered
+    in that order) and the entries of periodic sides are ignored.
     """
     n = tuple(int(v) for v in (n if np.iterable(n) else [n] * dim))
     lower = tuple(float(v) for v in (lower if np.iterable(lower) else [lower] * dim))
@@ -138,6 +146,7 @@ def box_mesh(dim: int, n, lower, upper, periodic=True, bc: int = 2) -> BoxMesh:
     strides = np.cumprod((1,) + n[:-1])
     nbr = np.empty((ne, 2 * dim), dtype=np.int32)
     any_bnd = False
+    types, payload = [], []
     for d in range(dim):
         for s, step in ((0, -1), (1, +1)):
             j = idx[d] + step
@@ -146,11 +155,25 @@ def box_mesh(dim: int, n, lower, upper, periodic=True, bc: int = 2) -> BoxMesh:
                 nbr[:, 2 * d + s] = np.arange(ne) + (jj - idx[d]) * strides[d]
             else:
                 inside = (j >= 0) & (j < n[d])
-                val = np.where(inside, np.arange(ne) + step * strides[d], -1)
+                code = -1
+                if side_bc is not None:
+                    entry = side_bc[2 * d + s]
+                    t, data = (entry, ()) if np.isscalar(entry) else (entry[0], entry[1])
+                    row = np.zeros(BC_DATA_STRIDE)
+                    row[: len(data)] = data
+                    types.append(int(t))
+                    payload.append(row)
+                    code = -len(types)
+                val = np.where(inside, np.arange(ne) + step * strides[d], code)
                 any_bnd |= bool((~inside).any())
                 nbr[:, 2 * d + s] = val
-    bc_type = np.array([bc] if any_bnd else [], dtype=np.int32)
-    return BoxMesh(dim, n, lower, upper, per, jac, adj.reshape(ne, dim * dim).copy(), nbr, bc_type)
+    if side_bc is not None:
+        bc_type = np.array(types, dtype=np.int32)
+        bc_data = np.array(payload).reshape(-1, BC_DATA_STRIDE)
+    else:
+        bc_type = np.array([bc] if any_bnd else [], dtype=np.int32)
+        bc_data = None
+    return BoxMesh(dim, n, lower, upper, per, jac, adj.reshape(ne, dim * dim).copy(), nbr, bc_type, bc_data)
 
 
 def gll_nodes(p: int):

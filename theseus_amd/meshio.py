@@ -34,6 +34,8 @@ from pathlib import Path
 
 import numpy as np
 
+from .mesh import BC_DATA_STRIDE
+
 # Gmsh 2.2 / MFEM corner order -> lexicographic corner order (x fastest)
 _QUAD_TO_LEX = (0, 1, 3, 2)               # ccw (0,0),(1,0),(1,1),(0,1)
 _HEX_TO_LEX = (0, 1, 3, 2, 4, 5, 7, 6)    # bottom ccw, then top ccw
@@ -52,6 +54,7 @@ class AffineMesh:
     bc_attributes: tuple = ()      # file attribute (physical tag) of each boundary index
     elem_attr: np.ndarray = field(default=None, repr=False)
     vertex_ids: np.ndarray = field(default=None, repr=False)  # (ne, 2^dim) after periodic identification
+    bc_data: np.ndarray = field(default=None, repr=False)     # (nbc, BC_DATA_STRIDE) boundary payload
 
     @property
     def num_elements(self) -> int:
@@ -247,12 +250,18 @@ def _finish(dim, vids, coords_lex, boundary, bc_types, attrs, rtol=1e-9):
                         bc_attr.append(0)
                     a = 0
                 nbr[e, 2 * d + s] = -(bc_attr.index(a) + 1)
-    types = np.array([int(bc_types.get(a, 2)) if isinstance(bc_types, dict) else int(bc_types) for a in bc_attr],
-                     dtype=np.int32)
+    # bc_types: one type for every boundary, or attribute -> type or (type, payload)
+    entries = [bc_types.get(a, 2) if isinstance(bc_types, dict) else bc_types for a in bc_attr]
+    types = np.array([int(v if np.isscalar(v) else v[0]) for v in entries], dtype=np.int32)
+    data = np.zeros((len(entries), BC_DATA_STRIDE))
+    for i, v in enumerate(entries):
+        if not np.isscalar(v):
+            data[i, : len(v[1])] = v[1]
     if not (nbr < 0).any():
         types = np.zeros(0, dtype=np.int32)
+        data = np.zeros((0, BC_DATA_STRIDE))
     return AffineMesh(dim, np.ascontiguousarray(corners, dtype=np.float64), nbr.astype(np.int32), types,
-                      tuple(bc_attr), np.asarray(attrs), ov)
+                      tuple(bc_attr), np.asarray(attrs), ov, data)
 
 
 # ---------------------------------------------------------------------------
@@ -457,7 +466,8 @@ def uniform_refine(mesh: AffineMesh, levels: int = 1) -> AffineMesh:
                         pn = mesh.face_neighbor[:, 2 * d + s]
                         nbr[c::nc, 2 * d + s] = np.where(pn >= 0, pn * nc + (c ^ (1 << d)), pn)
         mesh = AffineMesh(dim, corners, nbr, mesh.bc_type, mesh.bc_attributes,
-                          np.repeat(np.asarray(mesh.elem_attr), nc) if mesh.elem_attr is not None else None, None)
+                          np.repeat(np.asarray(mesh.elem_attr), nc) if mesh.elem_attr is not None else None, None,
+                          mesh.bc_data)
     return mesh
 
 
@@ -514,4 +524,4 @@ def reorder(mesh: AffineMesh, order: np.ndarray) -> AffineMesh:
     nbr = np.where(nbr >= 0, inv[np.maximum(nbr, 0)], nbr).astype(np.int32)
     attr = np.asarray(mesh.elem_attr)[order] if mesh.elem_attr is not None else None
     vids = mesh.vertex_ids[order] if mesh.vertex_ids is not None else None
-    return AffineMesh(mesh.dim, mesh.corners[order], nbr, mesh.bc_type, mesh.bc_attributes, attr, vids)
+    return AffineMesh(mesh.dim, mesh.corners[order], nbr, mesh.bc_type, mesh.bc_attributes, attr, vids, mesh.bc_data)

@@ -99,6 +99,12 @@ def build_descriptor(rt: RunTime, mesh: BoxMesh):
     adj = np.ascontiguousarray(mesh.elem_adj, dtype=np.float64)
     nbr = np.ascontiguousarray(mesh.face_neighbor, dtype=np.int32)
     bct = np.ascontiguousarray(mesh.bc_type, dtype=np.int32)
+    bcd = np.zeros((bct.size, A.BC_DATA_STRIDE))
+    src = getattr(mesh, "bc_data", None)
+    if src is not None and bct.size:
+        src = np.asarray(src, dtype=np.float64).reshape(bct.size, -1)
+        bcd[:, : src.shape[1]] = src
+    bcd = np.ascontiguousarray(bcd.reshape(-1))
     d = A.CaseDesc()
     d.dim = rt.dim
     d.order = rt.order
@@ -120,7 +126,8 @@ def build_descriptor(rt: RunTime, mesh: BoxMesh):
     d.face_neighbor = nbr.ctypes.data_as(C.POINTER(C.c_int))
     d.num_bc = int(bct.size)
     d.bc_type = bct.ctypes.data_as(C.POINTER(C.c_int)) if bct.size else C.POINTER(C.c_int)()
-    return d, (jac, adj, nbr, bct)
+    d.bc_data = bcd.ctypes.data_as(C.POINTER(C.c_double)) if bct.size else C.POINTER(C.c_double)()
+    return d, (jac, adj, nbr, bct, bcd)
 
 
 class RHSOperator:
