@@ -351,12 +351,12 @@ static int enqueue_rhs(theseus_rhs *h, RHSArgs a, hipStream_t s) {
     h->pref_set = true;
   }
   if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
-    timed(h, 1, s, [&] { h->sk->ns_grad[h->pm](a, nb, s); });
-    timed(h, 0, s, [&] { h->fk->ns[h->pm](a, nb, s); });
+    timed(h, 1, s, [&] { h->sk->ns_grad[h->pm][h->has_boundary](a, nb, s); });
+    timed(h, 0, s, [&] { h->fk->ns[h->pm][h->has_boundary](a, nb, s); });
   } else if (h->lk) {
     timed(h, 0, s, [&] { h->lk->rhs(a, nb, s); });
   } else {
-    timed(h, 0, s, [&] { h->fk->euler[h->pm](a, nb, s); });
+    timed(h, 0, s, [&] { h->fk->euler[h->pm][h->has_boundary](a, nb, s); });
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(THESEUS_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -850,6 +850,7 @@ namespace {
 struct PoolCase {
   int start = -1, ne = 0;
   bool active = false;
+  bool boundary = false;  // the case's mesh has boundary faces
 };
 
 // Process-wide memory of the autotuner: (dim, order, vol, surf, precision) -> (tile, xcd_remap)
@@ -877,6 +878,7 @@ struct theseus_pool {
   GasC<float> gas32;
   int pm = 0, tile = 0, xcd = 1;
   int active = 0;  // slots a step launches over: the occupied slots' high-water mark, rounded up
+  int bnd = 0;     // 1 while an admitted case has boundary faces: the kernels with boundary conditions
   double adv_scale = 0, diff_scale = 0, surf_scale = 0;
   double *d_J = nullptr, *d_adj = nullptr, *d_alpha = nullptr, *d_Fv = nullptr, *d_U = nullptr;
   double *d_stage[2] = {nullptr, nullptr};
@@ -932,12 +934,12 @@ struct theseus_pool {
 
   void launch(const RHSArgs &a, hipStream_t s) const {
     if (cd.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
-      sk->ns_grad[pm](a, nblocks(), s);
-      fk->ns[pm](a, nblocks(), s);
+      sk->ns_grad[pm][bnd](a, nblocks(), s);
+      fk->ns[pm][bnd](a, nblocks(), s);
     } else if (tk) {
-      tk->euler[tile][pm](a, nblocks(), s);
+      tk->euler[tile][pm][bnd](a, nblocks(), s);
     } else {
-      fk->euler[pm](a, nblocks(), s);
+      fk->euler[pm][bnd](a, nblocks(), s);
     }
   }
 
@@ -991,16 +993,21 @@ static int pool_reset_slots(theseus_pool *p, int start, int n, hipStream_t s) {
 }
 
 // The step's launch range follows the occupied slots: their high-water mark
-// rounded up to 1024 slots (first-fit keeps it low).  A new range drops the
-// captured graph, which the next step captures again; `s` is synchronised
-// first so that no launch of the old graph is still in flight.
+// rounded up to 1024 slots (first-fit keeps it low); the kernels with the
+// boundary conditions run while an admitted case has boundary faces.  A new
+// range or kernel choice drops the captured graph, which the next step
+// captures again; `s` is synchronised first so that no launch of the old
+// graph is still in flight.
 static int pool_update_active(theseus_pool *p, hipStream_t s) {
-  int hwm = 0;
+  int hwm = 0, bnd = 0;
   for (const PoolCase &c : p->cases)
-    if (c.active) hwm = std::max(hwm, c.start + c.ne);
+    if (c.active) {
+      hwm = std::max(hwm, c.start + c.ne);
+      bnd |= c.boundary ? 1 : 0;
+    }
   const int gran = 1024;
   const int next = std::min(p->cap, (hwm + gran - 1) / gran * gran);
-  if (next == p->active) return THESEUS_OK;
+  if (next == p->active && bnd == p->bnd) return THESEUS_OK;
   HIP_TRY(hipStreamSynchronize(s));
   if (p->exec) {
     HIP_TRY(hipGraphExecDestroy(p->exec));
@@ -1011,6 +1018,7 @@ static int pool_update_active(theseus_pool *p, hipStream_t s) {
     p->graph = nullptr;
   }
   p->active = next;
+  p->bnd = bnd;
   return THESEUS_OK;
 }
 
@@ -1362,6 +1370,7 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
   p->cases[cid].start = start;
   p->cases[cid].ne = ne;
   p->cases[cid].active = true;
+  p->cases[cid].boundary = has_boundary;
   {
     const int rc = pool_update_active(p, s);
     if (rc) return rc;

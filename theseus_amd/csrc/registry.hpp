@@ -15,23 +15,25 @@ using RHSLaunch = void (*)(const RHSArgs &, int nblocks, hipStream_t);
 using StabLaunch = void (*)(const StabArgs &, int nblocks, hipStream_t);
 using MeasLaunch = void (*)(const MeasArgs &, int nblocks, hipStream_t);
 
-// [0]: FP64 arithmetic; [1]: FP32 flux arithmetic (mixed precision)
+// [precision][bnd]: precision 0 = FP64 arithmetic, 1 = FP32 flux
+// arithmetic (mixed precision); bnd 1 = the variant with the boundary
+// conditions (meshes with boundary faces), 0 = periodic meshes
 struct FluxKernels {
-  RHSLaunch euler[2] = {nullptr, nullptr};  // fused volume + face
-  RHSLaunch ns[2] = {nullptr, nullptr};     // Navier-Stokes pass 2
+  RHSLaunch euler[2][2] = {};  // fused volume + face
+  RHSLaunch ns[2][2] = {};     // Navier-Stokes pass 2
 };
 
 struct ShapeKernels {
-  RHSLaunch ns_grad[2] = {nullptr, nullptr};  // Navier-Stokes pass 1
+  RHSLaunch ns_grad[2][2] = {};  // Navier-Stokes pass 1
   StabLaunch stability = nullptr;
   MeasLaunch measures = nullptr;
   int tpb = 0, epb = 0;
 };
 
 // Tile shapes of the Euler kernel the serving pool's autotuner chooses from:
-// [tile][precision], tile 0 = the default elements per workgroup, tile 1 = twice that.
+// [tile][precision][bnd], tile 0 = the default elements per workgroup, tile 1 = twice that.
 struct TileKernels {
-  RHSLaunch euler[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  RHSLaunch euler[2][2][2] = {};
   int epb[2] = {0, 0};
   int tpb[2] = {0, 0};
 };
@@ -51,13 +53,14 @@ void register_shape(int dim, int p, ShapeKernels k);
 const FluxKernels *find_flux(int dim, int p, int vol, int surf);
 const ShapeKernels *find_shape(int dim, int p);
 
-template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX = 0>
+template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX = 0, bool BND = false>
 void launch_rhs(const RHSArgs &a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_rhs<R, DIM, P, VOL, SURF, VISC, EPBX>), dim3(nb), dim3(Shape<DIM, P, EPBX>::TPB), 0, s, a);
+  hipLaunchKernelGGL((k_rhs<R, DIM, P, VOL, SURF, VISC, EPBX, BND>), dim3(nb), dim3(Shape<DIM, P, EPBX>::TPB), 0, s,
+                     a);
 }
-template <typename R, int DIM, int P>
+template <typename R, int DIM, int P, bool BND = false>
 void launch_ns_grad(const RHSArgs &a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_ns_grad<R, DIM, P>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+  hipLaunchKernelGGL((k_ns_grad<R, DIM, P, 0, BND>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
 }
 template <int DIM, int P>
 void launch_stability(const StabArgs &a, int nb, hipStream_t s) {
@@ -73,10 +76,14 @@ template <int DIM, int VOL, int SURF, int P>
 struct FluxRegistrar {
   FluxRegistrar() {
     FluxKernels k;
-    k.euler[0] = &launch_rhs<double, DIM, P, VOL, SURF, false>;
-    k.ns[0] = &launch_rhs<double, DIM, P, VOL, SURF, true>;
-    k.euler[1] = &launch_rhs<float, DIM, P, VOL, SURF, false>;
-    k.ns[1] = &launch_rhs<float, DIM, P, VOL, SURF, true>;
+    k.euler[0][0] = &launch_rhs<double, DIM, P, VOL, SURF, false>;
+    k.ns[0][0] = &launch_rhs<double, DIM, P, VOL, SURF, true>;
+    k.euler[1][0] = &launch_rhs<float, DIM, P, VOL, SURF, false>;
+    k.ns[1][0] = &launch_rhs<float, DIM, P, VOL, SURF, true>;
+    k.euler[0][1] = &launch_rhs<double, DIM, P, VOL, SURF, false, 0, true>;
+    k.ns[0][1] = &launch_rhs<double, DIM, P, VOL, SURF, true, 0, true>;
+    k.euler[1][1] = &launch_rhs<float, DIM, P, VOL, SURF, false, 0, true>;
+    k.ns[1][1] = &launch_rhs<float, DIM, P, VOL, SURF, true, 0, true>;
     register_flux(DIM, P, VOL, SURF, k);
   }
 };
@@ -86,10 +93,14 @@ struct TileRegistrar {
   TileRegistrar() {
     constexpr int E0 = Shape<DIM, P>::EPB, E1 = 2 * Shape<DIM, P>::EPB;
     TileKernels k;
-    k.euler[0][0] = &launch_rhs<double, DIM, P, VOL, SURF, false>;
-    k.euler[0][1] = &launch_rhs<float, DIM, P, VOL, SURF, false>;
-    k.euler[1][0] = &launch_rhs<double, DIM, P, VOL, SURF, false, E1>;
-    k.euler[1][1] = &launch_rhs<float, DIM, P, VOL, SURF, false, E1>;
+    k.euler[0][0][0] = &launch_rhs<double, DIM, P, VOL, SURF, false>;
+    k.euler[0][1][0] = &launch_rhs<float, DIM, P, VOL, SURF, false>;
+    k.euler[1][0][0] = &launch_rhs<double, DIM, P, VOL, SURF, false, E1>;
+    k.euler[1][1][0] = &launch_rhs<float, DIM, P, VOL, SURF, false, E1>;
+    k.euler[0][0][1] = &launch_rhs<double, DIM, P, VOL, SURF, false, 0, true>;
+    k.euler[0][1][1] = &launch_rhs<float, DIM, P, VOL, SURF, false, 0, true>;
+    k.euler[1][0][1] = &launch_rhs<double, DIM, P, VOL, SURF, false, E1, true>;
+    k.euler[1][1][1] = &launch_rhs<float, DIM, P, VOL, SURF, false, E1, true>;
     k.epb[0] = E0;
     k.epb[1] = E1;
     k.tpb[0] = Shape<DIM, P>::TPB;
@@ -118,8 +129,10 @@ template <int DIM, int P>
 struct ShapeRegistrar {
   ShapeRegistrar() {
     ShapeKernels k;
-    k.ns_grad[0] = &launch_ns_grad<double, DIM, P>;
-    k.ns_grad[1] = &launch_ns_grad<float, DIM, P>;
+    k.ns_grad[0][0] = &launch_ns_grad<double, DIM, P>;
+    k.ns_grad[1][0] = &launch_ns_grad<float, DIM, P>;
+    k.ns_grad[0][1] = &launch_ns_grad<double, DIM, P, true>;
+    k.ns_grad[1][1] = &launch_ns_grad<float, DIM, P, true>;
     k.stability = &launch_stability<DIM, P>;
     k.measures = &launch_measures<DIM, P>;
     k.tpb = Shape<DIM, P>::TPB;

@@ -406,8 +406,10 @@ struct RhsScratch {
 // Fused volume + face kernel: the Euler right-hand side, or the second
 // Navier-Stokes pass when VISC.  R is the type of the flux arithmetic and of
 // the LDS staging (double, or float in the mixed mode); the state, the RK
-// combination and the blending indicator are FP64 in both.
-template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX = 0>
+// combination and the blending indicator are FP64 in both.  BND: the mesh
+// has boundary faces (a periodic mesh runs the variant without the
+// boundary-condition code).
+template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX = 0, bool BND = false>
 __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
   using S = Shape<DIM, P, EPBX>;
   using SC = RhsScratch<R, DIM, P>;
@@ -503,10 +505,11 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
     const int f = fl / NF, t = fl - f * NF;
     const int d = f >> 1, s = f & 1;
     const int nbs = max(fnb[r], 0);
+    size_t nbase = (size_t)nbs * ND + face_node<DIM, Np>(d, 1 - s, t);
     // a boundary point loads the element's own trace (FP64), the input of
     // its exterior state and wall pressure
-    const size_t nbase = fnb[r] >= 0 ? (size_t)nbs * ND + face_node<DIM, Np>(d, 1 - s, t)
-                                     : (size_t)min(ebase_blk + pe, a.ne - 1) * ND + face_node<DIM, Np>(d, s, t);
+    if constexpr (BND)
+      if (fnb[r] < 0) nbase = (size_t)min(ebase_blk + pe, a.ne - 1) * ND + face_node<DIM, Np>(d, s, t);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
     if constexpr (VISC) {
@@ -716,45 +719,64 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
         // F*(q, q_ext, n_out) (ApplyBoundaryConditionInviscid,
         // bc_kernels.hpp:219-287) with the interior trace first, or the
         // wall flux (0, p* n_out, 0).
-        const bool interior = fnb[r] >= 0;
-        double qx[NQ];  // FP64 exterior state: the neighbour's trace, or the boundary's
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) qx[q] = qn[r][q];
-        int bct = BC_SUPERSONIC_OUTFLOW;
-        const double *bcd = nullptr;
-        if (!interior) {
-          const int b = -(fnb[r] + 1);
-          bct = a.bc_type[b];
-          bcd = a.bc_data + (size_t)b * BC_STRIDE;
-          bc_exterior<DIM>(bct, bcd, sm.nrm[pe][f], qx);
-        }
-        Pt<R, DIM> oth;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) oth.q[q] = R(qx[q]);
-        derive(g, oth);
-        if constexpr (!FP64) {
-          // pressure from the FP64 state, as for the staged nodes
-          Pt<double, DIM> o64;
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) o64.q[q] = qx[q];
-          derive(a.gas, o64);
-          oth.p = R(o64.p);
-          oth.pd = R(o64.p - pref);
-        }
-        Pt<R, DIM> pl, pr;
-        const bool own_left = s != 0 || !interior;
-        pt_select(own_left, own, oth, pl);
-        pt_select(own_left, oth, own, pr);
+        const bool interior = !BND || fnb[r] >= 0;
         R F[NQ];
-        surface_flux<SURF, R, DIM>(g, pl, pr, n, F);
-        if (!interior && (bc_wall_pressure(bct, VISC) || bc_zero_flux(bct, VISC))) {
-          // SlipWallInviscidFluxKernel (bc_kernels.hpp:138-152) and the
-          // walls' pressure term (:181-183, :212-214), from the FP64 trace
-          const double pw = bc_zero_flux(bct, VISC) ? 0.0 : slipwall_pstar<DIM>(a.gas, qn[r], sm.nrm[pe][f]) - pref;
-          F[0] = R(0);
+        if constexpr (!BND) {
+          Pt<R, DIM> oth;  // the neighbour's trace (loaded at the top)
 #pragma unroll
-          for (int dd = 0; dd < DIM; ++dd) F[1 + dd] = bc_zero_flux(bct, VISC) ? R(0) : R(pw * sm.nrm[pe][f][dd]);
-          F[DIM + 1] = R(0);
+          for (int q = 0; q < NQ; ++q) oth.q[q] = R(qn[r][q]);
+          derive(g, oth);
+          if constexpr (!FP64) {
+            // pressure from the FP64 trace, as for the staged nodes
+            Pt<double, DIM> o64;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) o64.q[q] = qn[r][q];
+            derive(a.gas, o64);
+            oth.p = R(o64.p);
+            oth.pd = R(o64.p - pref);
+          }
+          Pt<R, DIM> pl, pr;
+          pt_select(s != 0, own, oth, pl);
+          pt_select(s != 0, oth, own, pr);
+          surface_flux<SURF, R, DIM>(g, pl, pr, n, F);
+        } else {
+          double qx[NQ];  // FP64 exterior state: the neighbour's trace, or the boundary's
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) qx[q] = qn[r][q];
+          int bct = BC_SUPERSONIC_OUTFLOW;
+          if (!interior) {
+            const int b = -(fnb[r] + 1);
+            bct = a.bc_type[b];
+            bc_exterior<DIM>(bct, a.bc_data + (size_t)b * BC_STRIDE, sm.nrm[pe][f], qx);
+          }
+          Pt<R, DIM> oth;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) oth.q[q] = R(qx[q]);
+          derive(g, oth);
+          if constexpr (!FP64) {
+            // pressure from the FP64 state, as for the staged nodes
+            Pt<double, DIM> o64;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) o64.q[q] = qx[q];
+            derive(a.gas, o64);
+            oth.p = R(o64.p);
+            oth.pd = R(o64.p - pref);
+          }
+          Pt<R, DIM> pl, pr;
+          const bool own_left = s != 0 || !interior;
+          pt_select(own_left, own, oth, pl);
+          pt_select(own_left, oth, own, pr);
+          surface_flux<SURF, R, DIM>(g, pl, pr, n, F);
+          if (!interior && (bc_wall_pressure(bct, VISC) || bc_zero_flux(bct, VISC))) {
+            // SlipWallInviscidFluxKernel (bc_kernels.hpp:138-152) and the
+            // walls' pressure term (:181-183, :212-214), from the FP64 trace
+            const bool zero = bc_zero_flux(bct, VISC);
+            const double pw = zero ? 0.0 : slipwall_pstar<DIM>(a.gas, qn[r], sm.nrm[pe][f]) - pref;
+            F[0] = R(0);
+#pragma unroll
+            for (int dd = 0; dd < DIM; ++dd) F[1 + dd] = zero ? R(0) : R(pw * sm.nrm[pe][f][dd]);
+            F[DIM + 1] = R(0);
+          }
         }
         if constexpr (VISC) {
           // F* - 0.5 n.(Fv_L + Fv_R)   (DGSEMIntegrator.hpp:261-316); the own
@@ -812,7 +834,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
 // -0.5 n.Fv lifted with the face weight, and n.Fv at every point of the
 // element's faces for the neighbours' halves (n the canonical area vector).
 // ---------------------------------------------------------------------------
-template <typename R, int DIM, int P, int EPBX = 0>
+template <typename R, int DIM, int P, int EPBX = 0, bool BND = false>
 __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs a) {
   using S = Shape<DIM, P, EPBX>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
@@ -828,7 +850,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
   __shared__ R sD[Np * Np];
   __shared__ R sDh[Np * Np];
   __shared__ ElemMeta<DIM, EPB> sm;
-  __shared__ int sfnb[EPB][2 * DIM];  // face neighbours (boundary codes < 0)
+  __shared__ int sfnb[BND ? EPB : 1][BND ? 2 * DIM : 1];  // face neighbours (boundary codes < 0)
   R (*sS)[NQ][ND] = reinterpret_cast<R (*)[NQ][ND]>(sbuf);
   R (*sjump)[NFACE][NQ] = reinterpret_cast<R (*)[NFACE][NQ]>(sbuf + EPB * NQ * ND);
   R (*sG)[DIM][NVD][ND] = reinterpret_cast<R (*)[DIM][NVD][ND]>(sbuf);
@@ -860,8 +882,10 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
     const int pe = fi / NFACE;
     const int ee = min(ebase_blk + pe, a.ne - 1);
     fnb[r] = a.nbr[(size_t)ee * 2 * DIM + (fi - pe * NFACE) / NF];
-    const int fu = tid + r * TPB;  // unclamped: one writer per face
-    if (fu < NFT && (fu - pe * NFACE) % NF == 0) sfnb[pe][(fu - pe * NFACE) / NF] = fnb[r];
+    if constexpr (BND) {
+      const int fu = tid + r * TPB;  // unclamped: one writer per face
+      if (fu < NFT && (fu - pe * NFACE) % NF == 0) sfnb[pe][(fu - pe * NFACE) / NF] = fnb[r];
+    }
   }
   MetaRegs<DIM, EPB, TPB> mr;
   meta_load_index<DIM, EPB, TPB>(a, ebase_blk, mr);
@@ -912,11 +936,11 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
     const int fi = tid + r * TPB;
     const int pe = fi / NFACE;
     const int fl = fi - pe * NFACE;
-    if (fi < NFT && ebase_blk + pe < a.ne) {
+    if (fi < NFT && ebase_blk + pe < a.ne && (BND || fnb[r] >= 0)) {
       const int f = fl / NF, t = fl - f * NF;
       const int d = f >> 1, s = f & 1;
       const int on = face_node<DIM, Np>(d, s, t);
-      if (fnb[r] >= 0) {
+      if (!BND || fnb[r] >= 0) {
         R qr[NQ], Sn[NQ];
 #pragma unroll
         for (int c = 0; c < NQ; ++c) qr[c] = R(qn[r][c]);
@@ -1028,8 +1052,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
           Gf_out[c * Gtot + gi] = v;
           vn[c] = v;
         }
-        const int fc = sfnb[el][f];
-        if (fc >= 0) {
+        const int fc = BND ? sfnb[el][f] : 0;
+        if (!BND || fc >= 0) {
           // face term scale: -w on the element's high face, +w on its low face
 #pragma unroll
           for (int c = 0; c < NVD; ++c) hv[c] += (hi ? wown : -wown) * (R(0.5) * vn[c]);
