@@ -86,39 +86,16 @@ struct LogMeanEps<float> {
   static constexpr double value = 1e-2;
 };
 
-// Logarithmic mean LM(x,y) = (y-x)/ln(y/x) and its reciprocal, following
-// the reference's two branches (theseus_kernels.hpp:106-123): with
-// u = ((y-x)/(y+x))^2 (the reference's (xi^2-2xi+1)/(xi^2+2xi+1), xi=y/x),
-//   u <  eps : LM = (x+y) 52.5/(105 + 35u + 21u^2 + 15u^3)
-//              = ((x+y)/2) / P(u),  P(u) = 1 + u/3 + u^2/5 + u^3/7,
-//   u >= eps : LM = (y-x)/ln(y/x).
-// In the series branch one division (1/(x+y)) yields both LM and 1/LM:
-// 1/LM = (2/(x+y)) P(u) exactly as the reference's rational form, and
-// LM = ((x+y)/2) Q(u) with Q = 1 - u/3 - 4u^2/45 - 44u^3/945, the series of
-// 1/P, whose truncation (O(u^4) < 1e-16 for u < 1e-4) is below FP64
-// rounding.  The log branch is the reference's formula.
+// Logarithmic mean and its reciprocal for the volume fluxes: log_mean_pair
+// (fastmath.hpp), the atanh series for moderate ratios and the logarithm
+// beyond, equal to the reference's formula (theseus_kernels.hpp:106-123) to
+// rounding.
 template <typename R>
-struct LogMean {
-  R lm, inv;
-};
+using LogMean = LogMeanPair<R>;
 
 template <typename R, bool NEED_LM, bool NEED_INV>
 __device__ __forceinline__ LogMean<R> log_mean2(R x, R y) {
-  LogMean<R> r;
-  const R s = x + y;
-  const R d = y - x;
-  if (d * d < R(LogMeanEps<R>::value) * (s * s)) {
-    const R rs = drcp(s);
-    const R t = d * rs;
-    const R u = t * t;
-    if constexpr (NEED_LM) r.lm = (R(0.5) * s) * (R(1) - u * (R(1.0 / 3.0) + u * (R(4.0 / 45.0) + u * R(44.0 / 945.0))));
-    if constexpr (NEED_INV) r.inv = (R(2) * rs) * (R(1) + u * (R(1.0 / 3.0) + u * (R(0.2) + u * R(1.0 / 7.0))));
-  } else {
-    const R l = dlog(y / x);
-    if constexpr (NEED_LM) r.lm = d / l;
-    if constexpr (NEED_INV) r.inv = l / d;
-  }
-  return r;
+  return log_mean_pair<R, NEED_LM, NEED_INV>(x, y);
 }
 
 // Logarithmic mean in the reference's exact formulation (kept for the

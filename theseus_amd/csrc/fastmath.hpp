@@ -69,4 +69,95 @@ __host__ __device__ __forceinline__ double ln_pos(double x) {
   return fma(de, ln2_hi, fma(de, ln2_lo, lnm));
 }
 
+// Reciprocal refined to about an ulp: FP64 as recip_nr; FP32 from the
+// hardware (device) or IEEE (host) reciprocal with one Newton step.
+template <typename R>
+__host__ __device__ __forceinline__ R recip_refined(R x);
+template <>
+__host__ __device__ __forceinline__ double recip_refined<double>(double x) {
+  return recip_nr(x);
+}
+template <>
+__host__ __device__ __forceinline__ float recip_refined<float>(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  float r = __builtin_amdgcn_rcpf(x);
+#else
+  float r = 1.0f / x;
+#endif
+  return std::fma(r, std::fma(-x, r, 1.0f), r);
+}
+
+// Logarithmic mean LM(x, y) = (y - x) / ln(y/x) of two positive numbers and
+// its reciprocal, without a logarithm when the ratio is moderate.  With
+// s = x + y, t = (y - x)/s and u = t^2, ln(y/x) = 2 atanh(t) = 2 t P(u),
+// P(u) = sum_k u^k/(2k+1), so 1/LM = 2 P(u)/s and LM = (s/2)/P(u):
+//   u <  eps    : the reference's series (theseus_kernels.hpp:106-123), P to
+//                 u^3, with LM = (s/2) Q(u), Q = 1 - u/3 - 4u^2/45 - 44u^3/945
+//                 the series of 1/P (one reciprocal gives both);
+//   u <  0.0294 : (|t| < 0.1716, y/x within [0.707, 1.414]) P summed to
+//                 u^11/23 in FP64 (remainder below 2e-20 relative), to u^5/11
+//                 in FP32 (below 1e-10), LM from its refined reciprocal;
+//   otherwise   : LM = (y - x)/ln(y/x), as the reference.
+// eps is the reference's 1e-4 in FP64 and 1e-2 in FP32 (the truncation
+// u^4/9 then stays below FP32 rounding).  The middle branch is the
+// reference's log branch evaluated without the logarithm: it agrees with it
+// to a few ulp, and keeps a wave in series arithmetic when a smooth flow
+// puts some of its lanes just above eps.
+template <typename R>
+struct LogMeanPair {
+  R lm, inv;
+};
+
+template <typename R, bool NEED_LM, bool NEED_INV>
+__host__ __device__ __forceinline__ LogMeanPair<R> log_mean_pair(R x, R y) {
+  constexpr bool F64 = sizeof(R) == sizeof(double);
+  LogMeanPair<R> r{R(0), R(0)};
+  const R s = x + y;
+  const R d = y - x;
+  if (d * d < R(F64 ? 1e-4 : 1e-2) * (s * s)) {
+    const R rs = recip_refined<R>(s);
+    const R t = d * rs;
+    const R u = t * t;
+    if constexpr (NEED_LM) r.lm = (R(0.5) * s) * (R(1) - u * (R(1.0 / 3.0) + u * (R(4.0 / 45.0) + u * R(44.0 / 945.0))));
+    if constexpr (NEED_INV) r.inv = (R(2) * rs) * (R(1) + u * (R(1.0 / 3.0) + u * (R(0.2) + u * R(1.0 / 7.0))));
+  } else if (d * d < R(0.0294) * (s * s)) {
+    const R rs = recip_refined<R>(s);
+    const R t = d * rs;
+    const R u = t * t;
+    R p;
+    if constexpr (F64) {
+      p = R(1.0 / 23.0);
+      p = std::fma(p, u, R(1.0 / 21.0));
+      p = std::fma(p, u, R(1.0 / 19.0));
+      p = std::fma(p, u, R(1.0 / 17.0));
+      p = std::fma(p, u, R(1.0 / 15.0));
+      p = std::fma(p, u, R(1.0 / 13.0));
+      p = std::fma(p, u, R(1.0 / 11.0));
+    } else {
+      p = R(1.0 / 11.0);
+    }
+    p = std::fma(p, u, R(1.0 / 9.0));
+    p = std::fma(p, u, R(1.0 / 7.0));
+    p = std::fma(p, u, R(1.0 / 5.0));
+    p = std::fma(p, u, R(1.0 / 3.0));
+    p = std::fma(p, u, R(1));
+    if constexpr (NEED_INV) r.inv = (R(2) * rs) * p;
+    if constexpr (NEED_LM) r.lm = (R(0.5) * s) * recip_refined<R>(p);
+  } else {
+    R l;
+    if constexpr (F64) {
+      l = ln_pos(y / x);
+    } else {
+#if defined(__HIP_DEVICE_COMPILE__)
+      l = logf(y / x);
+#else
+      l = std::log(y / x);
+#endif
+    }
+    if constexpr (NEED_LM) r.lm = d / l;
+    if constexpr (NEED_INV) r.inv = l / d;
+  }
+  return r;
+}
+
 }  // namespace thx
