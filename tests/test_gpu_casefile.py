@@ -68,3 +68,27 @@ def test_loaded_tgv2d_navier_stokes_rhs_parity(oracle):
     # RHS is fixed by FP64 only to ~7e-12 (the oracle's own builds with and
     # without fused multiply-add differ by 7.0e-12 here)
     assert err <= 2e-11, (err, per_field_rel(got, ref, 4))
+
+
+def test_loaded_lid_driven_cavity_reproduces_the_reference_golden_output():
+    """The reference's Navier-Stokes lid-driven cavity case file (no-slip
+    adiabatic walls, moving lid, named boundary data), loaded and run on the
+    engine for 4000 SSP-RK3 steps, against the reference's own output."""
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+
+    from .golden_cavity import DT, NSTEPS, cavity_case, check_fields, golden_permutation
+
+    _say("case golden lid-driven cavity 16^2 p=3 NS, 4000 steps")
+    rt, mesh, U0 = cavity_case()
+    d, perm = golden_permutation(mesh)
+    op = MakeRHSOperator(rt, mesh)
+    u = torch.from_numpy(U0).cuda()
+    t = 0.0
+    for _ in range(NSTEPS):
+        t = op.Step(u, t, DT)
+    torch.cuda.synchronize()
+    op.close()
+    errs = check_fields(u.cpu().numpy(), d, perm)
+    _say(f"  rel L2 per field {errs}")

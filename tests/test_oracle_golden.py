@@ -1,5 +1,7 @@
 """The CPU oracle against the reference's own golden output: isentropic
-vortex, 500 SSP-RK3 steps (see tests/golden_vortex.py).  The reference's
+vortex, 500 SSP-RK3 steps (see tests/golden_vortex.py), and the
+Navier-Stokes lid-driven cavity with no-slip walls, 4000 steps (see
+tests/golden_cavity.py).  The reference's
 regression compares at atol=rtol=1e-13 between runs of one binary
 (tests/recipes/regression/golden-vortex.sh); a restatement with its own GLL
 nodes and summation orders is held to 1e-12 relative."""
@@ -45,3 +47,51 @@ def test_golden_detects_a_planted_mistake(oracle):
     f = primitive_fields(U, gamma=1.4)
     g = d["p"][perm]
     assert np.linalg.norm(f["p"] - g) / np.linalg.norm(g) > 1e-9
+
+
+def test_oracle_reproduces_reference_golden_lid_driven_cavity(oracle):
+    """Pins the wall boundary conditions end to end: no-slip adiabatic walls
+    and a moving lid (wall pressure, BR1 boundary jumps, viscous wall flux)."""
+    from theseus_amd.operator import build_descriptor
+
+    from .golden_cavity import DT, NSTEPS, cavity_case, check_fields, golden_permutation
+
+    rt, mesh, U0 = cavity_case()
+    d, perm = golden_permutation(mesh)
+    desc, keep = build_descriptor(rt, mesh)
+    case = oracle.OracleCase(desc)
+    oracle._lib.oracle_set_num_threads(8)
+    U = U0.copy()
+    for _ in range(NSTEPS):
+        U = case.step(U, DT)
+    errs = check_fields(U, d, perm)
+    print("cavity rel L2 per field", errs)
+    alpha, _ = case.blending(U)
+    assert (alpha == 0).all() and (d["alpha"] == 0).all()
+
+
+def test_cavity_golden_detects_a_planted_lid_speed(oracle):
+    """A lid moving at 1.0001 instead of 1 leaves the 4000-step result far
+    outside the band: the fixture pins the wall data, not just the flow."""
+    from theseus_amd.operator import build_descriptor
+
+    from .golden_cavity import DT, cavity_case, golden_permutation
+
+    rt, mesh, U0 = cavity_case()
+    d, perm = golden_permutation(mesh)
+    lid = [i for i in range(mesh.bc_data.shape[0]) if mesh.bc_data[i, 0] == 1.0]
+    assert len(lid) == 1
+    mesh.bc_data[lid[0], 0] = 1.0001
+    desc, keep = build_descriptor(rt, mesh)
+    case = oracle.OracleCase(desc)
+    oracle._lib.oracle_set_num_threads(8)
+    U = U0.copy()
+    for _ in range(400):
+        U = case.step(U, DT)
+    ref_desc, ref_keep = build_descriptor(rt, cavity_case()[1])
+    ref = oracle.OracleCase(ref_desc)
+    V = U0.copy()
+    for _ in range(400):
+        V = ref.step(V, DT)
+    u_planted, u_ref = U.reshape(4, -1)[1], V.reshape(4, -1)[1]
+    assert np.linalg.norm(u_planted - u_ref) / np.linalg.norm(u_ref) > 1e-6

@@ -9,7 +9,9 @@ ProjectCoefficient does for a DG GLL space, src/Simulation.cpp:1076-1109).
 
 Initial conditions restated: IsentropicVortexIC, TaylorGreenVortexIC,
 TaylorGreenVortex2DIC, CompressibleTaylorGreenVortexIC,
-KelvinHelmholtzInstabilityIC.  The Helmholtz LTE gas needs a table; the
+KelvinHelmholtzInstabilityIC, LidDrivenCavityIC.  Boundaries:
+``conditions.boundary_conditions`` by Gmsh physical name, with the
+reference's boundary types and its named constant boundary data.  The Helmholtz LTE gas needs a table; the
 reference builds it with PLATO (SimFactory_impl.hpp:140-216), which is not
 available here, so a caller passes one (``lte_table``) or the synthetic
 dissociating gas is tabulated on the config's grid (N_rho, N_T, rho_min,
@@ -25,10 +27,11 @@ import numpy as np
 
 from . import cases
 from .config import RunTime
-from .meshio import AffineMesh, hilbert_order, read_mesh, reorder, uniform_refine
+from .meshio import AffineMesh, gmsh_physical_names, hilbert_order, read_mesh, reorder, uniform_refine
 
 # name -> (number of parameters, function(mesh, order, *params))
 INITIAL_CONDITIONS = {
+    "LidDrivenCavityIC": (2, cases.lid_driven_cavity),
     "IsentropicVortexIC": (5, cases.reference_isentropic_vortex),
     "TaylorGreenVortexIC": (2, cases.taylor_green),
     "TaylorGreenVortex2DIC": (2, cases.taylor_green_2d),
@@ -58,6 +61,99 @@ def resolve_mesh_path(config_path: Path, mesh_file: str) -> Path:
     raise FileNotFoundError(f"mesh_file {mesh_file!r} not found next to {config_path}")
 
 
+# The reference's named constant boundary data (StateInit.hpp, registered
+# with Prandtl::ConditionFactory): conservative states for inflow, wall
+# velocities and wall heat fluxes / temperatures.
+NAMED_VECTORS = {
+    "LidDrivenCavityWallVelBCVector": (0.0, 0.0),                                     # :296
+    "LidDrivenCavityLidVelBCVector": (1.0, 0.0),                                      # :309
+    "BackwardFacingStepLeftBCVector": (5.9970, 5.9970 * 98.5914, 0.0,
+                                       11666.5 / 0.4 + 0.5 * 5.9970 * 98.5914 * 98.5914),  # :477-478
+    "DoubleMachReflectionLeftBottom1BCVector": (8.0, 8.0 * 7.144709581221619, -8.0 * 4.125,
+                                                116.5 / 0.4 + 0.5 * 8.0 * (7.144709581221619 ** 2 + 4.125 ** 2)),  # :544-545
+    "RampInletBCVector": (1.0, 2.0, 0.0, (1.0 / 1.4) / 0.4 + 0.5 * 1.0 * 2.0 * 2.0),  # :733
+    "ViscousSphereFreestreamBCVector": (1.0, 0.3, 0.0, (1.0 / 1.4) / 0.4 + 0.5 * 0.3 * 0.3),  # :788-789
+    "ForwardFacingStepLeftBCVector": (1.4, 1.4 * 3.0, 0.0, 1.0 / 0.4 + 0.5 * 1.4 * 3.0 * 3.0),  # :819
+}
+NAMED_SCALARS = {
+    "LidDrivenCavityAdiaBCScalar": 0.0,    # :280
+    "LidDrivenCavityIsoBCScalar": 0.007,   # :283
+}
+
+
+def _named(table, key, what):
+    if key not in table:
+        raise NotImplementedError(f"named {what} {key!r} is not restated (have: {sorted(table)})")
+    return table[key]
+
+
+def boundary_spec(spec: dict, dim: int, gamma: float, R_gas: float):
+    """One entry of conditions.boundary_conditions as the reference's
+    configuration reads it (src/Simulation.cpp:555-900): returns the engine's
+    boundary type, or (type, payload).  Forms the reference itself rejects
+    (function-valued or time-dependent data, specified-state) raise
+    NotImplementedError, as do the axisymmetric axis and radial profiles."""
+    from . import _abi as A
+
+    t = spec["type"]
+    if t == "slip":
+        return A.BC_SLIP_WALL
+    if t == "symmetry":
+        return A.BC_SYMMETRY
+    if t == "supersonic-outflow":
+        return A.BC_SUPERSONIC_OUTFLOW
+    if t == "supersonic-inflow":
+        if "vector" not in spec:
+            raise NotImplementedError("supersonic-inflow with a function or time-dependent state")
+        return A.BC_SUPERSONIC_INFLOW, tuple(_named(NAMED_VECTORS, spec["vector"], "state vector"))
+    if t == "no-slip-adiabatic":
+        if "vector" not in spec.get("velocity", {}):
+            raise NotImplementedError("no-slip-adiabatic with function-valued wall data")
+        vel = _named(NAMED_VECTORS, spec["velocity"]["vector"], "wall velocity")
+        heat = _named(NAMED_SCALARS, spec["heat"]["scalar"], "wall heat flux")
+        return A.BC_NO_SLIP_ADIABATIC, tuple(vel[:dim]) + (float(heat),)
+    if t == "no-slip-isothermal":
+        v, T = spec.get("velocity", {}), spec.get("temperature", {})
+        vel = v["value"] if "value" in v else _named(NAMED_VECTORS, v["vector"], "wall velocity")
+        temp = T["value"] if "value" in T else _named(NAMED_SCALARS, T["scalar"], "wall temperature")
+        if len(vel) != dim:
+            raise ValueError("Wall velocity dimension mismatch")
+        if not (np.isfinite(temp) and temp > 0):
+            raise ValueError("Wall temperature must be positive")
+        return A.BC_NO_SLIP_ISOTHERMAL, tuple(float(x) for x in vel) + (float(temp),)
+    if t in ("exterior-state", "cpg-exterior-state"):
+        if t == "exterior-state":
+            st = spec["state"]
+            if set(st) - {"pressure", "temperature", "velocity"} or not {"pressure", "temperature"} <= set(st):
+                raise NotImplementedError("exterior-state other than pressure/temperature/velocity")
+            p, T = float(st["pressure"]), float(st["temperature"])
+            vel = [float(x) for x in st.get("velocity", [0.0] * dim)]
+        else:
+            p, T = float(spec["pressure"]), float(spec["temperature"])
+            vel = [float(spec.get("ux", 0.0)), float(spec.get("ur", 0.0))][:dim]
+        rho = p / (R_gas * T)
+        E = p / (gamma - 1.0) + 0.5 * rho * sum(x * x for x in vel)
+        return A.BC_PRESCRIBED_STATE, (rho, *[rho * x for x in vel], E)
+    raise NotImplementedError(f"boundary type {t!r} is not supported")
+
+
+def boundary_types(cfg: dict, mesh_path: Path, dim: int, gamma: float, R_gas: float):
+    """attribute -> boundary spec for read_mesh, from the config's named
+    boundaries and the mesh's Gmsh physical names."""
+    bcs = cfg["runTime"].get("conditions", {}).get("boundary_conditions")
+    if not bcs:
+        return 2
+    names = gmsh_physical_names(mesh_path) if Path(mesh_path).read_text()[:11] == "$MeshFormat" else {}
+    if not names:
+        raise NotImplementedError("named boundaries need a Gmsh mesh with $PhysicalNames")
+    out = {}
+    for name, spec in bcs.items():
+        if name not in names:
+            continue  # the reference skips names without faces (Simulation.cpp:529-533)
+        out[names[name]] = boundary_spec(spec, dim, gamma, R_gas)
+    return out
+
+
 def initial_condition(name: str, params: dict, mesh, order: int) -> np.ndarray:
     if name not in INITIAL_CONDITIONS:
         raise NotImplementedError(f"initial condition {name!r} is not restated (have: {sorted(INITIAL_CONDITIONS)})")
@@ -66,12 +162,16 @@ def initial_condition(name: str, params: dict, mesh, order: int) -> np.ndarray:
     return fn(mesh, order, *args)
 
 
-def load_case(config_path, mesh_path=None, lte_table=None, bc_types=2) -> CaseFile:
+def load_case(config_path, mesh_path=None, lte_table=None, bc_types=None) -> CaseFile:
+    """bc_types None: from the config's boundary_conditions (supersonic outflow
+    where it has none); else as read_mesh takes it."""
     config_path = Path(config_path)
     cfg = json.loads(config_path.read_text())
     rtj = cfg["runTime"]
     rt = RunTime.from_json(rtj, cfg.get("compileTime"))
     mpath = Path(mesh_path) if mesh_path else resolve_mesh_path(config_path, rtj["mesh_file"])
+    if bc_types is None:
+        bc_types = boundary_types(cfg, mpath, rt.dim, rt.gamma, rt.R_gas)
     mesh = read_mesh(mpath, bc_types)
     if mesh.dim != rt.dim:
         raise ValueError(f"mesh dimension {mesh.dim} != runTime dim {rt.dim}")

@@ -46,6 +46,16 @@ def taylor_green_2d(mesh: BoxMesh, order: int, gamma: float = 1.4, Ma: float = 0
     return _pack(den, [den * u, den * v], E)
 
 
+def lid_driven_cavity(mesh, order: int, Ma: float = 0.1, gamma: float = 1.4) -> np.ndarray:
+    """LidDrivenCavityIC (StateInit.hpp:258-268): fluid at rest, rho = 1,
+    p = 1/(Ma^2 gamma)."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    one = np.ones(X.shape[1])
+    p = 1.0 / (Ma * Ma * gamma)
+    return _pack(one, [0.0 * one for _ in range(mesh.dim)], one * (p / (gamma - 1.0)))
+
+
 def compressible_taylor_green(mesh: BoxMesh, order: int, Ma: float = 0.1) -> np.ndarray:
     """CompressibleTaylorGreenVortexIC (StateInit.hpp:176-205): p0 = 1e5,
     T0 = 273.15, R = 287.05, gamma = 1.4, U0 = Ma sqrt(gamma R T0), rho = p/(R T0)."""

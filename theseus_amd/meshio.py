@@ -428,6 +428,21 @@ def read_gmsh22(path, bc_types=2) -> AffineMesh:
     return _finish(dim, vids, coords, bnd, bc_types, attrs)
 
 
+def gmsh_physical_names(path) -> dict:
+    """{name: physical tag} from a Gmsh 2.2 file's $PhysicalNames section (the
+    names a case file's boundary_conditions refer to)."""
+    lines = Path(path).read_text().split("\n")
+    out = {}
+    for i, line in enumerate(lines):
+        if line.strip() == "$PhysicalNames":
+            n = int(lines[i + 1])
+            for row in lines[i + 2:i + 2 + n]:
+                dim_, tag, name = row.split(maxsplit=2)
+                out[name.strip().strip('"')] = int(tag)
+            break
+    return out
+
+
 def read_mesh(path, bc_types=2) -> AffineMesh:
     p = Path(path)
     head = p.read_text()[:64]
