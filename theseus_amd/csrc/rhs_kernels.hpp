@@ -99,7 +99,7 @@ __device__ __forceinline__ const GasC<float> &gas_of<float>(const RHSArgs &a) { 
 
 // Node index of in-face point t on local face (d, side)
 template <int DIM, int Np>
-__device__ __forceinline__ int face_node(int d, int side, int t) {
+__host__ __device__ constexpr int face_node(int d, int side, int t) {
   const int id = side ? Np - 1 : 0;
   if constexpr (DIM == 2) {
     return d == 0 ? id + Np * t : t + Np * id;
@@ -241,7 +241,7 @@ __device__ __forceinline__ int pair_index(int a, int b) {
   return a * Np - (a * (a + 1)) / 2 + (b - a - 1);
 }
 template <int Np>
-__device__ __forceinline__ void pair_nodes(int pr, int &a, int &b) {
+__host__ __device__ constexpr void pair_nodes(int pr, int &a, int &b) {
   int aa = 0, rem = pr;
 #pragma unroll
   for (int t = 0; t < Np - 1; ++t) {
@@ -267,8 +267,40 @@ constexpr bool kLdsPad = true;
 template <int Np, int ND>
 struct StagedIndex {
   static constexpr int size = kLdsPad ? ND + ND / Np : ND;
-  __device__ static __forceinline__ int at(int n) { return kLdsPad ? n + n / Np : n; }
+  __host__ __device__ static constexpr int at(int n) { return kLdsPad ? n + n / Np : n; }
 };
+
+// The volume pair list of one workgroup, decoded at compile time: entry w
+// of the direction-major list (pair r of direction dir in element pe) holds
+// the staged LDS indices of its two nodes (bits 0-15, 16-31), its scratch
+// row (dir & 1) * NPAIR + r (bits 32-43), the element (44-47) and the
+// direction (48-49).  The pair loop reads one 64-bit word per pair instead
+// of decoding w with integer divisions and branches on the direction.
+template <int DIM, int P, int EPBX>
+struct PairTable {
+  using S = Shape<DIM, P, EPBX>;
+  static constexpr int Np = S::Np, ND = S::NDOF, EPB = S::EPB;
+  static constexpr int NPL = Np * (Np - 1) / 2, NPAIR = (ND / Np) * NPL, NPT = EPB * NPAIR, N = DIM * NPT;
+  static_assert(2 * NPAIR <= 4096 && EPB <= 16, "pair table fields");
+  uint64_t v[N];
+  constexpr PairTable() : v{} {
+    for (int w = 0; w < N; ++w) {
+      const int dir = w / NPT, rem = w - dir * NPT;
+      const int pe = rem / NPAIR, r = rem - pe * NPAIR;
+      const int line = r / NPL;
+      int pa = 0, pb = 0;
+      pair_nodes<Np>(r - line * NPL, pa, pb);
+      const int sdir = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
+      const int base = face_node<DIM, Np>(dir, 0, line);
+      const uint64_t sa = (uint64_t)StagedIndex<Np, ND>::at(base + pa * sdir);
+      const uint64_t sb = (uint64_t)StagedIndex<Np, ND>::at(base + pb * sdir);
+      const uint64_t row = (uint64_t)((dir & 1) * NPAIR + r);
+      v[w] = sa | (sb << 16) | (row << 32) | ((uint64_t)pe << 44) | ((uint64_t)dir << 48);
+    }
+  }
+};
+template <int DIM, int P, int EPBX>
+__device__ constexpr PairTable<DIM, P, EPBX> kPairTable{};
 
 // A node's point sample from the staged state: velocity is m * (1/rho),
 // the same product derive() forms, so staged and fresh samples agree bitwise.
@@ -465,6 +497,10 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
   //      neighbour indices first, then the element's own data, then what
   //      depends on the indices (the neighbours' face traces and adjugate rows)
   constexpr int NFT = EPB * NFACE;
+  constexpr int NPW = DIM * EPB * NPAIR;  // volume pairs of the workgroup
+  // this lane's first volume pair, from the compile-time pair table (later
+  // rounds' entries are fetched one round ahead in the pair loop)
+  uint64_t pcode = kPairTable<DIM, P, EPBX>.v[min(tid, NPW - 1)];
   int fnb[NFR];
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
@@ -582,27 +618,23 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
 #pragma unroll 1
   for (int rr = 0; rr < NRND; ++rr) {
     const int w = rr * TPB + tid;
+    const uint64_t code = pcode;
+    if (rr + 1 < NRND) pcode = kPairTable<DIM, P, EPBX>.v[min(w + TPB, NPW - 1)];
     if (w < DIM * NPT) {
-      const int dir = w / NPT;
-      const int rem = w - dir * NPT;
-      const int pe = rem / NPAIR;
-      const int r = rem - pe * NPAIR;
+      const int pe = (int)((code >> 44) & 15u);
       if (ebase_blk + pe < a.ne) {
-        const int sdir = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
-        const int line = r / NPL;
-        int pa, pb;
-        pair_nodes<Np>(r - line * NPL, pa, pb);
-        const int base = face_node<DIM, Np>(dir, 0, line);
+        const int sa = (int)(code & 0xFFFFu), sb = (int)((code >> 16) & 0xFFFFu);
+        const int row = (int)((code >> 32) & 0xFFFu), dir = (int)((code >> 48) & 3u);
         Pt<R, DIM> lo, hi;
-        load_pt<R, DIM, NDS, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), SX::at(base + pa * sdir), lo);
-        load_pt<R, DIM, NDS, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), SX::at(base + pb * sdir), hi);
+        load_pt<R, DIM, NDS, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), sa, lo);
+        load_pt<R, DIM, NDS, NQ>(sq[pe], sp[pe], sir[pe], spd(pe), sb, hi);
         R met[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) met[d] = R(sm.adj[pe][dir * DIM + d]);
         R F[NQ];
         volume_flux<VOL, R, DIM>(g, lo, hi, met, F);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) sscr(pe, (dir & 1) * NPAIR + r, q) = F[q];
+        for (int q = 0; q < NQ; ++q) sscr(pe, row, q) = F[q];
       }
     }
 #pragma unroll
