@@ -302,6 +302,32 @@ struct PairTable {
 template <int DIM, int P, int EPBX>
 __device__ constexpr PairTable<DIM, P, EPBX> kPairTable{};
 
+// The face-point list of one workgroup, decoded at compile time for the
+// gradient pass: entry fi (point t of local face f of element pe) holds the
+// element's own node there, face_node(d, s, t) (bits 0-7), the neighbour's
+// node across the face, face_node(d, 1-s, t) (8-15), the point's row
+// f*NF + t (16-23), the element (24-27) and the face (28-30).
+template <int DIM, int P, int EPBX>
+struct FaceTable {
+  using S = Shape<DIM, P, EPBX>;
+  static constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, EPB = S::EPB;
+  static constexpr int NFACE = 2 * DIM * NF, N = EPB * NFACE;
+  static_assert(ND <= 256 && NFACE <= 256 && EPB <= 16, "face table fields");
+  uint32_t v[N];
+  constexpr FaceTable() : v{} {
+    for (int fi = 0; fi < N; ++fi) {
+      const int pe = fi / NFACE, fl = fi - pe * NFACE;
+      const int f = fl / NF, t = fl - f * NF;
+      const int d = f >> 1, s = f & 1;
+      const uint32_t own = (uint32_t)face_node<DIM, Np>(d, s, t);
+      const uint32_t nb = (uint32_t)face_node<DIM, Np>(d, 1 - s, t);
+      v[fi] = own | (nb << 8) | ((uint32_t)fl << 16) | ((uint32_t)pe << 24) | ((uint32_t)f << 28);
+    }
+  }
+};
+template <int DIM, int P, int EPBX>
+__device__ constexpr FaceTable<DIM, P, EPBX> kFaceTable{};
+
 // A node's point sample from the staged state: velocity is m * (1/rho),
 // the same product derive() forms, so staged and fresh samples agree bitwise.
 // `n` is a staged index (StagedIndex::at of the node).
@@ -910,13 +936,13 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
   int fnb[NFR];
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
-    const int fi = min(tid + r * TPB, NFT - 1);
-    const int pe = fi / NFACE;
+    const uint32_t fc = kFaceTable<DIM, P, EPBX>.v[min(tid + r * TPB, NFT - 1)];
+    const int pe = (int)((fc >> 24) & 15u), f = (int)(fc >> 28);
     const int ee = min(ebase_blk + pe, a.ne - 1);
-    fnb[r] = a.nbr[(size_t)ee * 2 * DIM + (fi - pe * NFACE) / NF];
+    fnb[r] = a.nbr[(size_t)ee * 2 * DIM + f];
     if constexpr (BND) {
       const int fu = tid + r * TPB;  // unclamped: one writer per face
-      if (fu < NFT && (fu - pe * NFACE) % NF == 0) sfnb[pe][(fu - pe * NFACE) / NF] = fnb[r];
+      if (fu < NFT && ((fc >> 16) & 255u) % NF == 0) sfnb[pe][f] = fnb[r];
     }
   }
   MetaRegs<DIM, EPB, TPB> mr;
@@ -933,12 +959,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
   double qn[NFR][NQ];
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
-    const int fi = min(tid + r * TPB, NFT - 1);
-    const int pe = fi / NFACE;
-    const int fl = fi - pe * NFACE;
-    const int f = fl / NF, t = fl - f * NF;
-    const int d = f >> 1, s = f & 1;
-    const size_t nbase = (size_t)max(fnb[r], 0) * ND + face_node<DIM, Np>(d, 1 - s, t);
+    const uint32_t fc = kFaceTable<DIM, P, EPBX>.v[min(tid + r * TPB, NFT - 1)];
+    const size_t nbase = (size_t)max(fnb[r], 0) * ND + ((fc >> 8) & 255u);
 #pragma unroll
     for (int c = 0; c < NQ; ++c) qn[r][c] = a.U[c * Ntot + nbase];
   }
@@ -966,12 +988,12 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs 
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
     const int fi = tid + r * TPB;
-    const int pe = fi / NFACE;
-    const int fl = fi - pe * NFACE;
+    const uint32_t fc = kFaceTable<DIM, P, EPBX>.v[min(fi, NFT - 1)];
+    const int pe = (int)((fc >> 24) & 15u);
+    const int fl = (int)((fc >> 16) & 255u);
     if (fi < NFT && ebase_blk + pe < a.ne && (BND || fnb[r] >= 0)) {
-      const int f = fl / NF, t = fl - f * NF;
-      const int d = f >> 1, s = f & 1;
-      const int on = face_node<DIM, Np>(d, s, t);
+      const int s = (int)(fc >> 28) & 1;
+      const int on = (int)(fc & 255u);
       if (!BND || fnb[r] >= 0) {
         R qr[NQ], Sn[NQ];
 #pragma unroll
