@@ -1,0 +1,85 @@
+"""The case file's boundary section on the CPU (src/Simulation.cpp:518-900):
+each boundary type the reference's configuration accepts maps to the
+engine's type and payload, named constant boundary data resolve to the
+reference's values (StateInit.hpp), forms the reference rejects raise, and
+the lid-driven cavity's names reach the right faces of its Gmsh mesh."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from theseus_amd import _abi as A
+from theseus_amd.casefile import NAMED_VECTORS, boundary_spec, load_case
+from theseus_amd.meshio import gmsh_physical_names
+
+CAVITY = Path(__file__).resolve().parent / "golden" / "cases" / "LidDrivenCavity"
+
+
+def test_simple_types():
+    assert boundary_spec({"type": "slip"}, 2, 1.4, 287.05) == A.BC_SLIP_WALL
+    assert boundary_spec({"type": "symmetry"}, 3, 1.4, 287.05) == A.BC_SYMMETRY
+    assert boundary_spec({"type": "supersonic-outflow"}, 2, 1.4, 287.05) == A.BC_SUPERSONIC_OUTFLOW
+
+
+def test_named_inflow_and_walls():
+    t, v = boundary_spec({"type": "supersonic-inflow", "vector": "ForwardFacingStepLeftBCVector"}, 2, 1.4, 287.05)
+    assert t == A.BC_SUPERSONIC_INFLOW
+    np.testing.assert_allclose(v, [1.4, 4.2, 0.0, 1.0 / 0.4 + 0.5 * 1.4 * 9.0], rtol=1e-15)
+    t, v = boundary_spec({"type": "no-slip-adiabatic", "velocity": {"vector": "LidDrivenCavityLidVelBCVector"},
+                          "heat": {"scalar": "LidDrivenCavityAdiaBCScalar"}}, 2, 1.4, 287.05)
+    assert t == A.BC_NO_SLIP_ADIABATIC and v == (1.0, 0.0, 0.0)
+    t, v = boundary_spec({"type": "no-slip-isothermal", "velocity": {"value": [0.0, 0.1, 0.0]},
+                          "temperature": {"value": 293.15}}, 3, 1.4, 287.05)
+    assert t == A.BC_NO_SLIP_ISOTHERMAL and v == (0.0, 0.1, 0.0, 293.15)
+    assert all(len(vec) in (2, 4) for vec in NAMED_VECTORS.values())
+
+
+def test_exterior_states_are_conservative():
+    t, q = boundary_spec({"type": "exterior-state",
+                          "state": {"pressure": 1.0e4, "temperature": 300.0, "velocity": [10.0, -5.0]}},
+                         2, 1.4, 287.05)
+    rho = 1.0e4 / (287.05 * 300.0)
+    assert t == A.BC_PRESCRIBED_STATE
+    np.testing.assert_allclose(q, [rho, rho * 10.0, -rho * 5.0, 1.0e4 / 0.4 + 0.5 * rho * 125.0], rtol=1e-15)
+    t, q2 = boundary_spec({"type": "cpg-exterior-state", "pressure": 1.0e4, "temperature": 300.0, "ux": 10.0,
+                           "ur": -5.0}, 2, 1.4, 287.05)
+    np.testing.assert_allclose(q2, q, rtol=1e-15)
+
+
+@pytest.mark.parametrize("spec", [
+    {"type": "supersonic-inflow", "function": "ShuOsherShockLeftBC", "signature": 1},
+    {"type": "no-slip-adiabatic", "velocity": {"function": "f"}, "heat": {"function": "g"}},
+    {"type": "specified-state", "vector": "NagashimaInletBCVector"},
+    {"type": "axis"},
+    {"type": "radial-profile", "file": "x", "pressure": 1.0},
+    {"type": "supersonic-inflow", "vector": "NagashimaInletBCVector"},
+])
+def test_forms_without_support_raise(spec):
+    with pytest.raises(NotImplementedError):
+        boundary_spec(spec, 2, 1.4, 287.05)
+
+
+def test_isothermal_wall_checks():
+    with pytest.raises(ValueError, match="positive"):
+        boundary_spec({"type": "no-slip-isothermal", "velocity": {"value": [0.0, 0.0]},
+                       "temperature": {"value": -1.0}}, 2, 1.4, 287.05)
+    with pytest.raises(ValueError, match="dimension"):
+        boundary_spec({"type": "no-slip-isothermal", "velocity": {"value": [0.0, 0.0, 0.0]},
+                       "temperature": {"value": 300.0}}, 2, 1.4, 287.05)
+
+
+def test_cavity_names_reach_the_right_faces():
+    names = gmsh_physical_names(CAVITY / "LidDrivenCavity.msh")
+    assert names == {"bottom": 1, "right": 2, "lid": 3, "left": 4, "fluid": 5}
+    cf = load_case(CAVITY / "config.json", CAVITY / "LidDrivenCavity.msh")
+    mesh = cf.mesh
+    assert mesh.num_elements == 256 and set(mesh.bc_type) == {A.BC_NO_SLIP_ADIABATIC}
+    lid = mesh.bc_attributes.index(3)
+    assert tuple(mesh.bc_data[lid, :3]) == (1.0, 0.0, 0.0)
+    # every face on y = 2 carries the lid's boundary index, every other boundary face a wall's
+    top = mesh.corners[:, :, 1].max(axis=1)
+    nbr = mesh.face_neighbor
+    codes = -(nbr[nbr < 0] + 1)
+    assert (codes == lid).sum() == 16 * 1 and codes.size == 64
+    for e in np.nonzero(np.isclose(top, 2.0))[0]:
+        assert (-(nbr[e][nbr[e] < 0] + 1) == lid).any()
