@@ -199,3 +199,53 @@ def test_pooled_cases_keep_their_own_boundary_tables():
         op.close()
         err = rel_l2(g.cpu().numpy(), u.cpu().numpy())
         assert err <= 1e-14, err
+
+
+@pytest.mark.parametrize("dim,order,n,flow,kind", [
+    (2, 1, (5, 4), A.FLOW_NAVIER_STOKES, "walls"),
+    (2, 5, (3, 2), A.FLOW_EULER, "euler"),
+    (3, 1, (3, 2, 3), A.FLOW_EULER, "euler"),
+    (3, 5, (2, 1, 2), A.FLOW_NAVIER_STOKES, "walls"),
+    (3, 4, (2, 2, 1), A.FLOW_NAVIER_STOKES, "walls"),
+])
+def test_boundaries_at_extreme_orders(oracle, dim, order, n, flow, kind):
+    """Orders 1 and 5 (and 4): boundary faces on partially filled workgroups
+    (p = 1 packs many elements per workgroup) and on the largest elements."""
+    _say(f"case bc extreme order dim={dim} p={order} n={n} flow={flow}")
+    rt, mesh = _case(dim, order, n, "ranocha_llf", flow, kind, mu=0.05)
+    _parity(oracle, rt, mesh, smooth_state(mesh, order, seed=81 + order, amp=0.2))
+
+
+def test_pooled_navier_stokes_cases_with_walls():
+    """Navier-Stokes cases with walls in one pool (gradient pass and second
+    pass of the boundary variant), each equal to its single-case operator."""
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+    from theseus_amd.serving import Pool
+
+    _say("case bc pool navier-stokes walls")
+    rt, m1 = _case(3, 3, (2, 2, 2), "ranocha_llf", A.FLOW_NAVIER_STOKES, "walls", mu=0.05)
+    _, m2 = _case(3, 3, (2, 3, 2), "ranocha_llf", A.FLOW_NAVIER_STOKES, "walls", mu=0.05)
+    us = [torch.from_numpy(smooth_state(m, 3, seed=s)).cuda() for m, s in ((m1, 91), (m2, 92))]
+    pool = Pool(rt, 64 << 20, 4, tile=0, xcd_remap=1)
+    ids = [pool.admit(m, u, 0.3) for m, u in ((m1, us[0]), (m2, us[1]))]
+    for _ in range(4):
+        pool.step()
+    got = []
+    for (cid, _), u in zip(ids, us):
+        out = torch.empty_like(u)
+        pool.read(cid, out)
+        got.append(out)
+    torch.cuda.synchronize()
+    pool.close()
+    for m, u0, (_, dt), g in zip((m1, m2), us, ids, got):
+        op = MakeRHSOperator(rt, m)
+        u = u0.clone()
+        t = 0.0
+        for _ in range(4):
+            t = op.Step(u, t, dt)
+        torch.cuda.synchronize()
+        op.close()
+        err = rel_l2(g.cpu().numpy(), u.cpu().numpy())
+        assert err <= 1e-14, err
