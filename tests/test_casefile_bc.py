@@ -66,6 +66,9 @@ def test_isothermal_wall_checks():
     with pytest.raises(ValueError, match="dimension"):
         boundary_spec({"type": "no-slip-isothermal", "velocity": {"value": [0.0, 0.0, 0.0]},
                        "temperature": {"value": 300.0}}, 2, 1.4, 287.05)
+    with pytest.raises(ValueError, match="dimension"):  # a 2-D named velocity on a 3-D case
+        boundary_spec({"type": "no-slip-adiabatic", "velocity": {"vector": "LidDrivenCavityWallVelBCVector"},
+                       "heat": {"scalar": "LidDrivenCavityAdiaBCScalar"}}, 3, 1.4, 287.05)
 
 
 def test_cavity_names_reach_the_right_faces():

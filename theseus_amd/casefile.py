@@ -111,7 +111,10 @@ def boundary_spec(spec: dict, dim: int, gamma: float, R_gas: float):
             raise NotImplementedError("no-slip-adiabatic with function-valued wall data")
         vel = _named(NAMED_VECTORS, spec["velocity"]["vector"], "wall velocity")
         heat = _named(NAMED_SCALARS, spec["heat"]["scalar"], "wall heat flux")
-        return A.BC_NO_SLIP_ADIABATIC, tuple(vel[:dim]) + (float(heat),)
+        if len(vel) != dim:
+            # the payload is the velocity followed by the heat flux at index dim
+            raise ValueError("Wall velocity dimension mismatch")
+        return A.BC_NO_SLIP_ADIABATIC, tuple(float(x) for x in vel) + (float(heat),)
     if t == "no-slip-isothermal":
         v, T = spec.get("velocity", {}), spec.get("temperature", {})
         vel = v["value"] if "value" in v else _named(NAMED_VECTORS, v["vector"], "wall velocity")
