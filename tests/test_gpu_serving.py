@@ -60,6 +60,29 @@ def test_case_alone_equals_case_in_mix(precision, order):
     assert np.array_equal(alone, mixed), np.abs(alone - mixed).max()
 
 
+@pytest.mark.parametrize("precision", [A.PRECISION_FP64, A.PRECISION_MIXED])
+@pytest.mark.parametrize("order", [2, 3])
+def test_case_sharing_workgroups_with_another_case(precision, order):
+    """A case placed at an odd slot after a 27-element case shares its first
+    workgroup (two elements each at p = 2 and 3) with that case: its result
+    must still equal the case alone (each element's face points use its own
+    case's background pressure in the mixed mode)."""
+    from theseus_amd.serving import CaseFactory, Pool, euler_runtime
+
+    _say(f"case pool straddling workgroup p={order} precision={precision}")
+    f = CaseFactory()
+    rt = euler_runtime(order, precision)
+    target = (4, 2.0, (0.3, 1.1, 2.0))
+    pool = Pool(rt, 256 << 20, 16, tile=0, xcd_remap=1)
+    alone, dt_alone = _run_pool_case(pool, f, [target], 0, 6)
+    pool.close()
+    pool = Pool(rt, 256 << 20, 16, tile=0, xcd_remap=1)
+    mixed, dt_mix = _run_pool_case(pool, f, [(3, 4.0, (0.0, 0.5, 1.0)), target, (3, 0.7, (2.0, 0.1, 0.4))], 1, 6)
+    pool.close()
+    assert dt_alone == dt_mix
+    assert np.array_equal(alone, mixed), np.abs(alone - mixed).max()
+
+
 def test_pool_matches_single_case_operator_and_oracle(oracle):
     import torch
 

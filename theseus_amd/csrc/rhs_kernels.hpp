@@ -490,6 +490,10 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
   __shared__ double sVi[Np * Np];
   __shared__ ElemMeta<DIM, EPB> sm;
   __shared__ int sblend;
+  // each element's background pressure (mixed mode): lanes evaluate face
+  // points of the other elements of the workgroup, which in the serving
+  // pool may belong to another case
+  __shared__ double sprefe[FP64 ? 1 : EPB];
   auto sscr = [&](int pe, int row, int q) -> R & { return reinterpret_cast<R *>(sraw[pe])[row * NQ + q]; };
   // pressure for the momentum fluxes: p itself in FP64, p - p_ref in the mixed mode
   auto spd = [&](int pe) -> const R * {
@@ -589,6 +593,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
   // ---- stage the element's state (coalesced: nodes are contiguous per equation)
   meta_store<DIM, EPB, TPB>(mr, sm);
   if (tid == 0) sblend = 0;
+  if constexpr (!FP64)
+    if (lane_ok && node == 0) sprefe[el] = pref;
   if (valid) {
     Pt<R, DIM> me;
 #pragma unroll
@@ -791,7 +797,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
             for (int q = 0; q < NQ; ++q) o64.q[q] = qn[r][q];
             derive(a.gas, o64);
             oth.p = R(o64.p);
-            oth.pd = R(o64.p - pref);
+            oth.pd = R(o64.p - sprefe[FP64 ? 0 : pe]);
           }
           Pt<R, DIM> pl, pr;
           pt_select(s != 0, own, oth, pl);
@@ -818,7 +824,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
             for (int q = 0; q < NQ; ++q) o64.q[q] = qx[q];
             derive(a.gas, o64);
             oth.p = R(o64.p);
-            oth.pd = R(o64.p - pref);
+            oth.pd = R(o64.p - sprefe[FP64 ? 0 : pe]);
           }
           Pt<R, DIM> pl, pr;
           const bool own_left = s != 0 || !interior;
@@ -829,7 +835,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
             // SlipWallInviscidFluxKernel (bc_kernels.hpp:138-152) and the
             // walls' pressure term (:181-183, :212-214), from the FP64 trace
             const bool zero = bc_zero_flux(bct, VISC);
-            const double pw = zero ? 0.0 : slipwall_pstar<DIM>(a.gas, qn[r], sm.nrm[pe][f]) - pref;
+            const double pw = zero ? 0.0 : slipwall_pstar<DIM>(a.gas, qn[r], sm.nrm[pe][f]) - (FP64 ? 0.0 : sprefe[FP64 ? 0 : pe]);
             F[0] = R(0);
 #pragma unroll
             for (int dd = 0; dd < DIM; ++dd) F[1 + dd] = zero ? R(0) : R(pw * sm.nrm[pe][f][dd]);
