@@ -4,6 +4,7 @@ path launches kernels only (no allocation, memset, copy or synchronisation
 on the stream).  Each replay restarts from the initial state and is checked
 against the eager result, with eager steps of another state and a
 synchronisation between replays."""
+import dataclasses
 import sys
 
 import numpy as np
@@ -23,17 +24,19 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("dim,order,n,flow,blending", [
-    (2, 3, (6, 5), A.FLOW_EULER, True),
-    (3, 3, (3, 2, 3), A.FLOW_NAVIER_STOKES, False),
+@pytest.mark.parametrize("dim,order,n,flow,blending,precision", [
+    (2, 3, (6, 5), A.FLOW_EULER, True, A.PRECISION_FP64),
+    (3, 3, (3, 2, 3), A.FLOW_NAVIER_STOKES, False, A.PRECISION_FP64),
+    (3, 3, (2, 3, 2), A.FLOW_NAVIER_STOKES, False, A.PRECISION_MIXED),
 ])
-def test_captured_steps_match_eager(dim, order, n, flow, blending):
+def test_captured_steps_match_eager(dim, order, n, flow, blending, precision):
     import torch
 
     from theseus_amd.operator import MakeRHSOperator
 
-    _say(f"case graph capture dim={dim} p={order} n={n} flow={flow} blending={blending}")
+    _say(f"case graph capture dim={dim} p={order} n={n} flow={flow} blending={blending} precision={precision}")
     rt, mesh = make_case(dim, order, n, "chandrashekar", flow=flow, blending=blending)
+    rt = dataclasses.replace(rt, precision=precision)
     U0 = smooth_state(mesh, order, seed=31, amp=0.2)
     op = MakeRHSOperator(rt, mesh)
     dt = 2e-4
