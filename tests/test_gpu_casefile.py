@@ -92,3 +92,55 @@ def test_loaded_lid_driven_cavity_reproduces_the_reference_golden_output():
     op.close()
     errs = check_fields(u.cpu().numpy(), d, perm)
     _say(f"  rel L2 per field {errs}")
+
+
+def test_forward_facing_step_reproduces_the_reference_golden_output():
+    """The reference's forward-facing-step case file (13,600 quads p=3,
+    Chandrashekar, subcell FV blending, slip walls, supersonic inflow and
+    outflow), 100 fixed steps on the engine, against the reference's own
+    output: fields to 2e-12 of each field's scale, and the blending
+    coefficients of the last stage (nonzero in 50 elements) to 1e-12."""
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+
+    from .golden_ffs import DT, NSTEPS, check, ffs_case
+
+    _say("case golden forward-facing step 13600 quads p=3, blending, 100 steps")
+    rt, mesh, U0 = ffs_case()
+    op = MakeRHSOperator(rt, mesh)
+    u = torch.from_numpy(U0).cuda()
+    t = 0.0
+    for _ in range(NSTEPS):
+        t = op.Step(u, t, DT)
+    alpha = op.Alpha().cpu().numpy()
+    torch.cuda.synchronize()
+    op.close()
+    errs = check(u.cpu().numpy(), alpha, mesh)
+    _say(f"  largest node error / field scale {errs}")
+
+
+@pytest.mark.parametrize("name", ["cpg-roe-cavity", "cpg-hll-cavity"])
+def test_numerical_flux_cavity_reproduces_the_reference_golden_output(name):
+    """The reference's Roe and HLL numerical-flux goldens (p=2 Navier-Stokes
+    cavity, no-slip walls, moving lid) at cycles 10 and 20, at the reference
+    checker's rtol 1e-10 and atol 1e-12."""
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+
+    from .golden_numflux import CYCLES, DT, compare, golden_rows, numflux_case
+
+    _say(f"case golden {name} p=2 NS, 20 steps")
+    rt, mesh, U0 = numflux_case(name)
+    op = MakeRHSOperator(rt, mesh)
+    u = torch.from_numpy(U0).cuda()
+    t = 0.0
+    for s in range(1, max(CYCLES) + 1):
+        t = op.Step(u, t, DT)
+        if s in CYCLES:
+            d, rows = golden_rows(mesh, name, s)
+            res = compare(u.cpu().numpy(), d, rows)
+            _say(f"  cycle {s}: {res}")
+            assert all(res[k] <= 0 for k in ("Density", "Pressure", "Velocity")), (s, res)
+    op.close()

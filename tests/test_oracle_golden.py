@@ -95,3 +95,77 @@ def test_cavity_golden_detects_a_planted_lid_speed(oracle):
         V = ref.step(V, DT)
     u_planted, u_ref = U.reshape(4, -1)[1], V.reshape(4, -1)[1]
     assert np.linalg.norm(u_planted - u_ref) / np.linalg.norm(u_ref) > 1e-6
+
+
+def _ffs_run(oracle, rt, mesh, U0):
+    """100 fixed steps; returns the final state and the blending
+    coefficients of the last RHS evaluation (third stage of the last step)."""
+    from theseus_amd.operator import build_descriptor
+
+    from .golden_ffs import DT, NSTEPS
+
+    desc, keep = build_descriptor(rt, mesh)
+    case = oracle.OracleCase(desc)
+    oracle._lib.oracle_set_num_threads(8)
+    U = U0.copy()
+    for _ in range(NSTEPS - 1):
+        U = case.step(U, DT)
+    u1 = U + DT * case.rhs(U)
+    u2 = 0.75 * U + 0.25 * (u1 + DT * case.rhs(u1))
+    alpha, _ = case.blending(u2)
+    return case.step(U, DT), alpha
+
+
+def test_oracle_reproduces_reference_golden_forward_facing_step(oracle):
+    """Pins the shock-capturing path end to end against the reference's own
+    output: the Persson-Peraire indicator and its eta -> alpha map (nonzero
+    in 50 elements, up to alpha_max), the subcell FV residual and the blend,
+    slip walls at the step and supersonic inflow/outflow."""
+    from .golden_ffs import check, ffs_case
+
+    rt, mesh, U0 = ffs_case()
+    U, alpha = _ffs_run(oracle, rt, mesh, U0)
+    errs = check(U, alpha, mesh)
+    print("forward-facing step: largest node error / field scale", errs)
+
+
+def test_ffs_golden_detects_a_planted_blending_limit(oracle):
+    """alpha_max entered as 0.49 instead of 0.5 changes only the elements
+    whose blending saturates; the 100-step result leaves the band, so the
+    fixture pins the FV blend and not just the smooth part of the flow."""
+    from .golden_ffs import ABS_TOL, golden_match, ffs_case, primitive
+
+    rt, mesh, U0 = ffs_case()
+    rt.alpha_max = 0.49
+    U, _ = _ffs_run(oracle, rt, mesh, U0)
+    d, ours, perm = golden_match(mesh)
+    f = primitive(U)["rho"].reshape(mesh.num_elements, -1)
+    assert np.abs(f[ours] - d["rho"][perm]).max() / np.abs(d["rho"]).max() > 1e3 * ABS_TOL
+
+
+def test_oracle_reproduces_reference_numerical_flux_cavities(oracle):
+    """The reference's Roe and HLL goldens (p=2 Navier-Stokes cavity with
+    no-slip walls, cycles 10 and 20) at the reference checker's rtol 1e-10,
+    atol 1e-12; the Roe run also fails the HLL golden (the two pin different
+    face fluxes)."""
+    from theseus_amd.operator import build_descriptor
+
+    from .golden_numflux import CASES, CYCLES, DT, compare, golden_rows, numflux_case
+
+    oracle._lib.oracle_set_num_threads(8)
+    finals = {}
+    for name in CASES:
+        rt, mesh, U0 = numflux_case(name)
+        desc, keep = build_descriptor(rt, mesh)
+        case = oracle.OracleCase(desc)
+        U = U0.copy()
+        for s in range(1, max(CYCLES) + 1):
+            U = case.step(U, DT)
+            if s in CYCLES:
+                d, rows = golden_rows(mesh, name, s)
+                res = compare(U, d, rows)
+                assert all(res[k] <= 0 for k in ("Density", "Pressure", "Velocity")), (name, s, res)
+        finals[name] = (U, mesh)
+    U, mesh = finals["cpg-roe-cavity"]
+    d, rows = golden_rows(mesh, "cpg-hll-cavity", max(CYCLES))
+    assert compare(U, d, rows)["Velocity"] > 1e-4

@@ -9,7 +9,7 @@ ProjectCoefficient does for a DG GLL space, src/Simulation.cpp:1076-1109).
 
 Initial conditions restated: IsentropicVortexIC, TaylorGreenVortexIC,
 TaylorGreenVortex2DIC, CompressibleTaylorGreenVortexIC,
-KelvinHelmholtzInstabilityIC, LidDrivenCavityIC.  Boundaries:
+KelvinHelmholtzInstabilityIC, LidDrivenCavityIC, ForwardFacingStepIC.  Boundaries:
 ``conditions.boundary_conditions`` by Gmsh physical name, with the
 reference's boundary types and its named constant boundary data.  The Helmholtz LTE gas needs a table; the
 reference builds it with PLATO (SimFactory_impl.hpp:140-216), which is not
@@ -37,6 +37,7 @@ INITIAL_CONDITIONS = {
     "TaylorGreenVortex2DIC": (2, cases.taylor_green_2d),
     "CompressibleTaylorGreenVortexIC": (1, cases.compressible_taylor_green),
     "KelvinHelmholtzInstabilityIC": (1, cases.kelvin_helmholtz),
+    "ForwardFacingStepIC": (1, cases.forward_facing_step),
 }
 
 
@@ -170,9 +171,16 @@ def load_case(config_path, mesh_path=None, lte_table=None, bc_types=None) -> Cas
     where it has none); else as read_mesh takes it."""
     config_path = Path(config_path)
     cfg = json.loads(config_path.read_text())
+    mpath = Path(mesh_path) if mesh_path else resolve_mesh_path(config_path, cfg["runTime"]["mesh_file"])
+    return load_case_config(cfg, mpath, lte_table, bc_types)
+
+
+def load_case_config(cfg: dict, mesh_path, lte_table=None, bc_types=None) -> CaseFile:
+    """load_case for a configuration already parsed ({"compileTime": ...,
+    "runTime": ...}) and an explicit mesh file."""
     rtj = cfg["runTime"]
     rt = RunTime.from_json(rtj, cfg.get("compileTime"))
-    mpath = Path(mesh_path) if mesh_path else resolve_mesh_path(config_path, rtj["mesh_file"])
+    mpath = Path(mesh_path)
     if bc_types is None:
         bc_types = boundary_types(cfg, mpath, rt.dim, rt.gamma, rt.R_gas)
     mesh = read_mesh(mpath, bc_types)

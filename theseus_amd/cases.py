@@ -88,6 +88,16 @@ def kelvin_helmholtz(mesh: BoxMesh, order: int, gammaM1Inverse: float = 2.5) -> 
     return _pack(rho, [rho * u, rho * v], E)
 
 
+def forward_facing_step(mesh, order: int, gammaM1Inverse: float = 2.5) -> np.ndarray:
+    """ForwardFacingStepIC (StateInit.hpp:805-816): uniform Mach-3 flow,
+    rho = 1.4, u = (3, 0), p = 1; the energy uses the parameter 1/(gamma-1)."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    one = np.ones(X.shape[1])
+    E = 1.0 * gammaM1Inverse + 0.5 * 1.4 * 3.0 * 3.0
+    return _pack(1.4 * one, [1.4 * 3.0 * one, 0.0 * one], E * one)
+
+
 def shu_vortex(mesh: BoxMesh, order: int, gamma: float = 1.4, strength: float = 5.0,
                center=(0.0, 0.0), u_inf=(1.0, 1.0)) -> np.ndarray:
     """Textbook isentropic vortex (Shu): rho_inf = p_inf = 1, u_inf = (1,1)."""
