@@ -43,9 +43,11 @@ FREESTREAM = {"rho": 1.4, "u": 3.0, "v": 0.0, "p": 1.0}
 # few 1e-12 of each field's largest magnitude: the bound held is
 # |ours - golden| <= ABS_TOL * max|golden field| at every node.
 ABS_TOL = 2e-12
-# Blending coefficients: sigmoid of the modal indicator, dalpha/deta up to
-# ~1e3 at p=3; the states agree to ~1e-12, so alpha to ~1e-12.
-ALPHA_TOL = 1e-12
+# Blending coefficients: a sigmoid of the modal indicator eta with slope up
+# to 9.21/(4 theta) ~ 1.1e3 at p=3, and eta = 1 - |M1|^2/|m|^2 cancels, so
+# state differences at the 1e-14 level move alpha by ~1e-12 (the oracle is
+# 4e-14 from the reference's values here).
+ALPHA_TOL = 1e-10
 
 
 def ffs_case():

@@ -46,6 +46,41 @@ def test_exterior_states_are_conservative():
     np.testing.assert_allclose(q2, q, rtol=1e-15)
 
 
+def test_exterior_state_other_thermodynamic_pairs():
+    rho, p, T = 0.5, 4.0e4, 4.0e4 / (0.5 * 287.05)
+    ref = boundary_spec({"type": "exterior-state", "state": {"pressure": p, "temperature": T, "velocity": [1.0, 2.0]}},
+                        2, 1.4, 287.05)[1]
+    for st in ({"density": rho, "temperature": T}, {"density": rho, "pressure": p}):
+        q = boundary_spec({"type": "exterior-state", "state": {**st, "velocity": [1.0, 2.0]}}, 2, 1.4, 287.05)[1]
+        np.testing.assert_allclose(q, ref, rtol=1e-14)
+
+
+@pytest.mark.parametrize("state, dim, msg", [
+    ({"pressure": 1e5, "temperature": 300.0}, 2, "velocity must be an array"),          # no velocity
+    ({"pressure": 1e5, "temperature": 300.0, "velocity": [1.0, 2.0, 3.0]}, 2, "dimension"),
+    ({"pressure": 1e5, "temperature": 300.0, "velocity": [1.0]}, 3, "dimension"),
+    ({"pressure": 1e5, "velocity": [0.0, 0.0]}, 2, "exactly two"),
+    ({"pressure": 1e5, "temperature": 300.0, "density": 1.0, "velocity": [0.0, 0.0]}, 2, "exactly two"),
+    ({"pressure": -1e5, "temperature": 300.0, "velocity": [0.0, 0.0]}, 2, "Pressure must be finite and positive"),
+    ({"pressure": 1e5, "temperature": 0.0, "velocity": [0.0, 0.0]}, 2, "Temperature must be finite and positive"),
+    ({"pressure": 1e5, "temperature": 300.0, "velocity": [0.0, float("nan")]}, 2, "Velocity must be finite"),
+    ({"pressure": 1e5, "temperature": 300.0, "velocity": [0.0, 0.0], "species": 1}, 2, "unsupported field"),
+])
+def test_exterior_state_rejects_what_the_reference_rejects(state, dim, msg):
+    """PhysicalStateConfig.hpp:13-50 and PhysicalState.hpp:33-57."""
+    with pytest.raises(ValueError, match=msg):
+        boundary_spec({"type": "exterior-state", "state": state}, dim, 1.4, 287.05)
+
+
+def test_cpg_exterior_state_checks():
+    """Simulation.cpp:777-785: 2-D only; CPGState rejects p or T <= 0."""
+    with pytest.raises(ValueError, match="2D"):
+        boundary_spec({"type": "cpg-exterior-state", "pressure": 1e5, "temperature": 300.0}, 3, 1.4, 287.05)
+    for p, T in ((0.0, 300.0), (1e5, -1.0)):
+        with pytest.raises(ValueError, match="Invalid CPG"):
+            boundary_spec({"type": "cpg-exterior-state", "pressure": p, "temperature": T}, 2, 1.4, 287.05)
+
+
 @pytest.mark.parametrize("spec", [
     {"type": "supersonic-inflow", "function": "ShuOsherShockLeftBC", "signature": 1},
     {"type": "no-slip-adiabatic", "velocity": {"function": "f"}, "heat": {"function": "g"}},

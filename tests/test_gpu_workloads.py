@@ -9,8 +9,11 @@ the oracle from the same initial state:
   at the fixed dt of CFL 0.5 on the initial state, with the state and the
   blending coefficients compared every 10 steps.
 
-Bar: state after 100 steps within 1e-10 relative L2 of the oracle; blending
-coefficients within 1e-12 with the same set of blended elements."""
+Bar: state after 100 steps within 1e-10 relative L2 of the oracle; the same
+set of blended elements, and blending coefficients within 1e-10: alpha is a
+sigmoid of the indicator eta with slope up to 9.21/(4 theta) ~ 1.6e3 at p=4,
+and eta = 1 - |M1|^2/|m|^2 cancels, so state differences of ~1e-14 move alpha
+by ~1e-12."""
 import sys
 
 import numpy as np
@@ -19,6 +22,8 @@ import pytest
 from .gpu_helpers import oracle_case, per_field_rel, rel_l2
 
 pytestmark = pytest.mark.gpu
+
+ALPHA_TOL = 1e-10
 
 
 def _say(msg):
@@ -87,6 +92,6 @@ def test_c3_riemann_full_size_100_steps_match_the_oracle(oracle):
                  f"{blended} blended elements")
             assert err <= 1e-10, (s, err, per_field_rel(got, U, 4))
             assert ((a > 0) == (aref > 0)).all()
-            assert np.abs(a - aref).max() <= 1e-12
+            assert np.abs(a - aref).max() <= ALPHA_TOL
     op.close()
     assert blended > 0

@@ -125,20 +125,75 @@ def boundary_spec(spec: dict, dim: int, gamma: float, R_gas: float):
         if not (np.isfinite(temp) and temp > 0):
             raise ValueError("Wall temperature must be positive")
         return A.BC_NO_SLIP_ISOTHERMAL, tuple(float(x) for x in vel) + (float(temp),)
-    if t in ("exterior-state", "cpg-exterior-state"):
-        if t == "exterior-state":
-            st = spec["state"]
-            if set(st) - {"pressure", "temperature", "velocity"} or not {"pressure", "temperature"} <= set(st):
-                raise NotImplementedError("exterior-state other than pressure/temperature/velocity")
-            p, T = float(st["pressure"]), float(st["temperature"])
-            vel = [float(x) for x in st.get("velocity", [0.0] * dim)]
-        else:
-            p, T = float(spec["pressure"]), float(spec["temperature"])
-            vel = [float(spec.get("ux", 0.0)), float(spec.get("ur", 0.0))][:dim]
+    if t == "exterior-state":
+        return A.BC_PRESCRIBED_STATE, exterior_state(spec, dim, gamma, R_gas)
+    if t == "cpg-exterior-state":
+        # Simulation.cpp:777-801: 2-D calorically perfect gas only; CPGState
+        # (RadialProfile.hpp:108-116) checks p, T > 0
+        if dim != 2:
+            raise ValueError("CPG exterior BC requires 2D CPG; LTE is not supported")
+        p, T = float(spec["pressure"]), float(spec["temperature"])
+        ux, ur = float(spec.get("ux", 0.0)), float(spec.get("ur", 0.0))
+        if not (p > 0 and T > 0) or not np.isfinite(p + T + ux + ur):
+            raise ValueError("Invalid CPG primitive state")
         rho = p / (R_gas * T)
-        E = p / (gamma - 1.0) + 0.5 * rho * sum(x * x for x in vel)
-        return A.BC_PRESCRIBED_STATE, (rho, *[rho * x for x in vel], E)
+        return A.BC_PRESCRIBED_STATE, (rho, rho * ux, rho * ur, p / (gamma - 1.0) + 0.5 * rho * (ux * ux + ur * ur))
     raise NotImplementedError(f"boundary type {t!r} is not supported")
+
+
+def exterior_state(spec: dict, dim: int, gamma: float, R_gas: float):
+    """The conservative state of an ``exterior-state`` boundary as the
+    reference parses and checks it (BoundaryStateConfig.hpp:18-45,
+    PhysicalStateConfig.hpp:13-50, PhysicalState.hpp:33-57): only the keys
+    type and state; state has exactly two of density, pressure and
+    temperature plus a velocity array of the case's dimension, all finite,
+    the thermodynamic values positive; ideal-gas conversion p = rho R T."""
+    extra = set(spec) - {"type", "state"}
+    if extra:
+        raise ValueError(f"Unsupported boundary field '{sorted(extra)[0]}'")
+    st = spec.get("state")
+
+    def bad(msg):
+        return ValueError(f"state: {msg}")
+
+    if not isinstance(st, dict):
+        raise bad("must be an object")
+    for key in st:
+        if key not in ("density", "pressure", "temperature", "velocity"):
+            raise bad(f"unsupported field '{key}'")
+    have = [k for k in ("density", "pressure", "temperature") if k in st]
+    if len(have) != 2:
+        raise bad("requires exactly two of density, pressure, temperature")
+    vals = {}
+    for k in have:
+        if not isinstance(st[k], (int, float)) or isinstance(st[k], bool):
+            raise bad(f"{k} must be a number")
+        vals[k] = float(st[k])
+    if "velocity" not in st:
+        raise bad("velocity must be an array")
+    vel = st["velocity"]
+    if not isinstance(vel, (list, tuple)):
+        raise bad("velocity must be an array")
+    if any(not isinstance(u, (int, float)) or isinstance(u, bool) for u in vel):
+        raise bad("velocity entries must be numbers")
+    vel = [float(u) for u in vel]
+    if len(vel) != dim:
+        raise bad("Physical state velocity dimension does not match gas layout")
+    if not all(np.isfinite(vel)):
+        raise bad("Velocity must be finite")
+    for k in have:
+        if not (np.isfinite(vals[k]) and vals[k] > 0):
+            raise bad(f"{k.capitalize()} must be finite and positive")
+    if "density" not in vals:
+        rho = vals["pressure"] / (R_gas * vals["temperature"])
+        p = vals["pressure"]
+    elif "pressure" not in vals:
+        rho = vals["density"]
+        p = rho * R_gas * vals["temperature"]
+    else:
+        rho, p = vals["density"], vals["pressure"]
+    E = p / (gamma - 1.0) + 0.5 * rho * sum(x * x for x in vel)
+    return (rho, *[rho * x for x in vel], E)
 
 
 def boundary_types(cfg: dict, mesh_path: Path, dim: int, gamma: float, R_gas: float):
