@@ -229,6 +229,22 @@ typedef struct theseus_integral_measures {
  * baseline (RHSOperator_impl.hpp:711-713).  Synchronises `stream`. */
 THESEUS_API int theseus_rhs_integral_measures(theseus_rhs *h, const double *u,
                                   theseus_integral_measures *out, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the integral measures of theseus_rhs_integral_measures,
+ * computed by the same kernels in the same fixed order (bit-identical
+ * results), written as nine doubles in theseus_integral_measures order to
+ * the DEVICE pointer `out`, asynchronously on `stream`, with no host
+ * synchronisation and no allocation.  The workgroup partials live in a fixed
+ * device array of the library (65,536 workgroups * 9 doubles = 4.5 MiB per
+ * GPU, held until the process ends): a case needing more than 65,536
+ * workgroups (more than 65,536 elements at 3-D p >= 4, 131,072 at 3-D
+ * p = 3 with its two elements per workgroup, more at lower orders) fails
+ * with THESEUS_ERR_CAPACITY.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads, other handles) race and
+ * must be ordered by the caller.  The handle's device must be current.  The
+ * baseline of theseus_rhs_integral_measures_baseline is not updated. */
+THESEUS_API int theseus_rhs_integral_measures_noalloc(theseus_rhs *h, const double *u, double *out, void *stream);
 /* RHSOperator::GetIntegralMeasuresBaseline (RHSOperator.hpp:92); all zero
  * before the first theseus_rhs_integral_measures call. */
 THESEUS_API int theseus_rhs_integral_measures_baseline(const theseus_rhs *h, theseus_integral_measures *out);

@@ -60,6 +60,7 @@ EXPORTED_SYMBOLS = (
     "theseus_rhs_get_alpha",
     "theseus_rhs_integral_measures",
     "theseus_rhs_integral_measures_baseline",
+    "theseus_rhs_integral_measures_noalloc",
     "theseus_rhs_step_ssprk3_cfl",
     "theseus_rhs_enable_kernel_timing",
     "theseus_rhs_kernel_timing",
@@ -205,6 +206,8 @@ def load(path: Path | str | None = None) -> C.CDLL:
     lib.theseus_rhs_integral_measures.restype = C.c_int
     lib.theseus_rhs_integral_measures_baseline.argtypes = [C.c_void_p, C.POINTER(IntegralMeasures)]
     lib.theseus_rhs_integral_measures_baseline.restype = C.c_int
+    lib.theseus_rhs_integral_measures_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.theseus_rhs_integral_measures_noalloc.restype = C.c_int
     lib.theseus_rhs_step_ssprk3_cfl.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.theseus_rhs_step_ssprk3_cfl.restype = C.c_int
     lib.theseus_last_error.argtypes = []

@@ -286,6 +286,7 @@ struct theseus_rhs {
   double *d_meas = nullptr;           // integral measures: [nblocks][9] partials, then the 9 results
   theseus_integral_measures diag0{};  // baseline: the first measures with nonzero mass
   bool has_boundary = false;
+  int device = 0;  // the device current at create time, which holds the handle's arrays
   // per-kernel event timing (instrumentation)
   int timing_capacity = 0;
   std::vector<hipEvent_t> ev[2][2];  // [kind][start/stop]
@@ -462,6 +463,37 @@ static void init_gas(const theseus_case_desc &d, GasC<double> &g, GasC<float> &g
   g32.R_gas = (float)g.R_gas;
 }
 
+// Workgroup partials of the integral measures for callers that cannot
+// allocate (theseus_rhs_integral_measures_noalloc): a file-scope device array
+// of NOALLOC_MAX_WORKGROUPS * NMEAS doubles (4.5 MiB), of which each GPU holds
+// its own copy until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in symbol): the device
+// address of each GPU's copy is looked up once with hipGetSymbolAddress, with
+// that device current, and kept per device.
+static constexpr int NOALLOC_MAX_WORKGROUPS = 65536;
+__device__ __align__(256) double scratch[NOALLOC_MAX_WORKGROUPS * NMEAS];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;  // device ordinal -> address of its copy of `scratch`
+
+// The current device's copy of `scratch`; resolved on the first call for
+// that device (the caller makes the device current).
+static int scratch_for_current_device(double **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(THESEUS_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    it = g_scratch_ptr.emplace(dev, static_cast<double *>(ptr)).first;
+  }
+  *out = it->second;
+  return THESEUS_OK;
+}
+
 extern "C" {
 
 const char *theseus_last_error(void) { return g_last_error.c_str(); }
@@ -518,8 +550,14 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
         return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas supports supersonic-outflow boundaries only");
   const int ne = d.num_elements, dim = d.dim;
 
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  double *scratch_dev = nullptr;  // set-up: resolve this device's scratch copy once
+  const int rs = scratch_for_current_device(&scratch_dev);
+  if (rs) return rs;
   theseus_rhs *h = new theseus_rhs();
   h->d = d;
+  h->device = device;
   h->d.elem_jac = nullptr;
   h->d.elem_adj = nullptr;
   h->d.face_neighbor = nullptr;
@@ -813,6 +851,42 @@ int theseus_rhs_integral_measures(theseus_rhs *h, const double *u, theseus_integ
 int theseus_rhs_integral_measures_baseline(const theseus_rhs *h, theseus_integral_measures *out) {
   if (!h || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle or output");
   *out = h->diag0;
+  return THESEUS_OK;
+}
+
+int theseus_rhs_integral_measures_noalloc(theseus_rhs *h, const double *u, double *out, void *stream) {
+  if (!h || !u || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle, vector or output");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != h->device)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "the handle's device (" + std::to_string(h->device) +
+                                                  ") must be current (device " + std::to_string(dev) + " is)");
+  const int nb = h->nblocks();
+  if (nb > NOALLOC_MAX_WORKGROUPS)
+    return fail(THESEUS_ERR_CAPACITY, "integral measures without allocation: " + std::to_string(nb) +
+                                          " workgroups exceed the fixed scratch of " +
+                                          std::to_string(NOALLOC_MAX_WORKGROUPS) + " (" +
+                                          std::to_string(NOALLOC_MAX_WORKGROUPS * h->sk->epb) +
+                                          " elements at this order); use theseus_rhs_integral_measures");
+  double *partial = nullptr;
+  const int rc = scratch_for_current_device(&partial);
+  if (rc) return rc;
+  // every workgroup writes its own partials, so nothing needs clearing
+  MeasArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.U = u;
+  a.partial = partial;
+  a.Ntot = h->Ntot;
+  a.e0 = 0;
+  a.ne = h->d.num_elements;
+  a.J = h->d_J;
+  a.gas = h->gas;
+  std::memcpy(a.w, h->basis.w, sizeof(a.w));
+  a.lte = h->ltev;
+  hipStream_t s = (hipStream_t)stream;
+  h->sk->measures(a, nb, s);
+  hipLaunchKernelGGL(k_measures_final, dim3(1), dim3(256), 0, s, partial, nb, out);
+  HIP_TRY(hipGetLastError());
   return THESEUS_OK;
 }
 

@@ -210,6 +210,17 @@ class RHSOperator:
                                                                    _stream_ptr(stream)))
         return IntegralMeasures.from_c(m)
 
+    def ComputeIntegralMeasuresNoAlloc(self, u, out, stream=None) -> None:
+        """The same measures written to the device tensor `out` (9 float64, in
+        IntegralMeasures field order) without allocating and without a host
+        synchronisation (theseus_rhs_integral_measures_noalloc); the baseline
+        is not updated."""
+        self._check(u, "u")
+        if out.dtype != u.dtype or not out.is_cuda or out.numel() < 9 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 device tensor of at least 9 values")
+        A.check(self._lib, self._lib.theseus_rhs_integral_measures_noalloc(
+            self._h, C.c_void_p(u.data_ptr()), C.c_void_p(out.data_ptr()), _stream_ptr(stream)))
+
     def GetIntegralMeasuresBaseline(self) -> IntegralMeasures:
         """RHSOperator::GetIntegralMeasuresBaseline (RHSOperator.hpp:92)."""
         m = A.IntegralMeasures()
