@@ -120,6 +120,9 @@ __device__ __forceinline__ int face_index(int d, int i, int j, int k) {
   }
 }
 
+// Whether lane t of a workgroup of EPB elements of ND nodes holds a node.
+__device__ __forceinline__ bool lane_ok_pre(int t, int ND, int EPB) { return t / ND < EPB; }
+
 // Workgroup barrier for LDS traffic only: global loads still in flight stay
 // in flight across it and are waited for where their registers are used.
 __device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -140,18 +143,16 @@ __device__ __forceinline__ int logical_block(int remap) {
 // alpha_max.  The reference applies the inverse of the full Legendre
 // Vandermonde (ModalBasis.cpp:115-140); that matrix is the Kronecker
 // product of the 1-D inverse, applied here as one 1-D sweep per direction.
-// `buf` holds at least 2*ND + 3*NPOW2 doubles of the element's scratch;
-// `srp_el` is the element's pressure*density per node, in FP64 whatever the
-// precision of the flux arithmetic.
+// `buf` holds at least 2*ND + 3*NPOW2 doubles of the element's scratch, the
+// first ND of them the element's pressure*density per node (FP64 whatever
+// the precision of the flux arithmetic; 0 on lanes without an element),
+// written and synchronised by the caller.
 template <int DIM, int P>
-__device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, const double *sVi,
-                                                 const double *srp_el, int node, int i, int j, int k, bool lane_ok,
-                                                 bool valid) {
+__device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, const double *sVi, int node, int i,
+                                                 int j, int k, bool lane_ok, bool valid) {
   using S = Shape<DIM, P>;
   constexpr int Np = S::Np, ND = S::NDOF, NP2 = S::NPOW2;
   double *A = buf, *B = buf + ND;
-  if (lane_ok) A[node] = valid ? srp_el[node] : 0.0;
-  lds_sync();
   // sweep along x: B(a,j,k) = sum_l Vinv(a,l) A(l,j,k)
   if (lane_ok) {
     double acc = 0.0;
@@ -270,10 +271,50 @@ struct StagedIndex {
   __host__ __device__ static constexpr int at(int n) { return kLdsPad ? n + n / Np : n; }
 };
 
+// Flux scratch of the volume phase of one workgroup.  The pair list is
+// direction-major (direction d: pairs r = pe * NPAIR + line * NPL + pair of
+// its NPT = EPB * NPAIR) and processed in rounds of TPB lanes; direction d is
+// gathered after the round holding its last pair.  When d's first round
+// also holds the previous direction's last pairs, those first nshared(d)
+// pairs of d go to an overflow region (one per direction parity, so that a
+// direction gathered in that round is not overwritten); every other pair of
+// every direction goes to row r of a common region of NPT rows, which the
+// direction before has released by then.  That is NPT + the overflow rows
+// instead of the 2 * NPT of one region per direction in flight (3-D p = 3,
+// two elements: 256 rows instead of 384).
+template <int DIM, int P, int EPBX>
+struct VolPlan {
+  using S = Shape<DIM, P, EPBX>;
+  static constexpr int Np = S::Np, ND = S::NDOF, EPB = S::EPB, TPB = S::TPB;
+  static constexpr int NPL = Np * (Np - 1) / 2, NPAIR = (ND / Np) * NPL, NPT = EPB * NPAIR;
+  static constexpr int NRND = (DIM * NPT + TPB - 1) / TPB;
+  __host__ __device__ static constexpr int first(int d) { return d * NPT / TPB; }
+  __host__ __device__ static constexpr int last(int d) { return ((d + 1) * NPT - 1) / TPB; }
+  __host__ __device__ static constexpr int nshared(int d) {
+    if (d == 0 || first(d) != last(d - 1)) return 0;
+    const int n = (first(d) + 1) * TPB - d * NPT;
+    return n < NPT ? n : NPT;
+  }
+  __host__ __device__ static constexpr int bsize(int parity) {
+    int m = 0;
+    for (int d = parity; d < DIM; d += 2) m = nshared(d) > m ? nshared(d) : m;
+    return m;
+  }
+  __host__ __device__ static constexpr int boff(int d) { return NPT + ((d & 1) ? bsize(0) : 0); }
+  static constexpr int ROWS = NPT + bsize(0) + bsize(1);
+  __host__ __device__ static constexpr int phys(int d, int r) { return r < nshared(d) ? boff(d) + r : r; }
+  // an overflow region is reused only once the direction before has been gathered
+  __host__ __device__ static constexpr bool valid() {
+    for (int d = 2; d < DIM; ++d)
+      if (nshared(d) > 0 && nshared(d - 2) > 0 && !(last(d - 2) < first(d))) return false;
+    return true;
+  }
+};
+
 // The volume pair list of one workgroup, decoded at compile time: entry w
 // of the direction-major list (pair r of direction dir in element pe) holds
 // the staged LDS indices of its two nodes (bits 0-15, 16-31), its scratch
-// row (dir & 1) * NPAIR + r (bits 32-43), the element (44-47) and the
+// row VolPlan::phys(dir, r) (bits 32-43), the element (44-47) and the
 // direction (48-49).  The pair loop reads one 64-bit word per pair instead
 // of decoding w with integer divisions and branches on the direction.
 template <int DIM, int P, int EPBX>
@@ -281,7 +322,8 @@ struct PairTable {
   using S = Shape<DIM, P, EPBX>;
   static constexpr int Np = S::Np, ND = S::NDOF, EPB = S::EPB;
   static constexpr int NPL = Np * (Np - 1) / 2, NPAIR = (ND / Np) * NPL, NPT = EPB * NPAIR, N = DIM * NPT;
-  static_assert(2 * NPAIR <= 4096 && EPB <= 16, "pair table fields");
+  static_assert(VolPlan<DIM, P, EPBX>::ROWS <= 4096 && EPB <= 16, "pair table fields");
+  static_assert(VolPlan<DIM, P, EPBX>::valid(), "volume flux scratch plan");
   uint64_t v[N];
   constexpr PairTable() : v{} {
     for (int w = 0; w < N; ++w) {
@@ -294,7 +336,7 @@ struct PairTable {
       const int base = face_node<DIM, Np>(dir, 0, line);
       const uint64_t sa = (uint64_t)StagedIndex<Np, ND>::at(base + pa * sdir);
       const uint64_t sb = (uint64_t)StagedIndex<Np, ND>::at(base + pb * sdir);
-      const uint64_t row = (uint64_t)((dir & 1) * NPAIR + r);
+      const uint64_t row = (uint64_t)VolPlan<DIM, P, EPBX>::phys(dir, rem);
       v[w] = sa | (sb << 16) | (row << 32) | ((uint64_t)pe << 44) | ((uint64_t)dir << 48);
     }
   }
@@ -461,6 +503,38 @@ struct RhsScratch {
   static constexpr int DOUBLES = FLUX_D > IND_D ? FLUX_D : IND_D;
 };
 
+// Workgroup scratch of k_rhs, in doubles: the volume flux rows of VolPlan,
+// the face rows (EPB * NFACE), the subcell FV rows (EPB * NI), all in the
+// compute type R, or the indicator's FP64 sweeps (EPB * IND_D), reused in
+// turn.  The indicator's input (pressure * density per node) is written
+// into its first sweep buffer while the state is staged.
+template <typename R, int DIM, int P, int EPBX>
+struct RhsWgScratch {
+  using S = Shape<DIM, P, EPBX>;
+  using VP = VolPlan<DIM, P, EPBX>;
+  static constexpr int ND = S::NDOF, NQ = S::NEQ, EPB = S::EPB, Np = S::Np;
+  static constexpr int NFACE = 2 * DIM * S::NFP;
+  static constexpr int NI = (Np - 1) * (ND / Np);
+  static constexpr int IND_D = 2 * ND + 3 * S::NPOW2;
+  static constexpr int rdoubles(int rows) { return (rows * NQ * (int)sizeof(R) + 7) / 8; }
+  static constexpr int mx(int a, int b) { return a > b ? a : b; }
+  static constexpr int DOUBLES = mx(mx(rdoubles(VP::ROWS), rdoubles(EPB * NFACE)), mx(rdoubles(EPB * NI), EPB * IND_D));
+};
+
+// Occupancy the register allocator is asked to reach for k_rhs (waves per
+// SIMD).  At FP64 3-D p = 3 (two elements per workgroup) the LDS (19.9 KB per
+// workgroup) allows 8 workgroups, i.e. 4 waves per SIMD, if the registers fit
+// 128 VGPRs; they do without spilling for the central and Ranocha volume
+// fluxes with the HLL or Lax-Friedrichs face flux (not for Navier-Stokes
+// with boundary conditions), which get the request; elsewhere it is not made
+// (1), since forcing it spills to scratch memory.
+template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX, bool BND>
+struct RhsWaves {
+  static constexpr bool fits = sizeof(R) == sizeof(double) && DIM == 3 && P == 3 && EPBX == 0 &&
+                               (VOL == 0 || VOL == 2) && (SURF == 4 || SURF == 5) && !(VISC && BND);
+  static constexpr int value = fits ? 4 : 1;
+};
+
 // Fused volume + face kernel: the Euler right-hand side, or the second
 // Navier-Stokes pass when VISC.  R is the type of the flux arithmetic and of
 // the LDS staging (double, or float in the mixed mode); the state, the RK
@@ -468,14 +542,16 @@ struct RhsScratch {
 // has boundary faces (a periodic mesh runs the variant without the
 // boundary-condition code).
 template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX = 0, bool BND = false>
-__global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
+__global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB))
+__attribute__((amdgpu_waves_per_eu(RhsWaves<R, DIM, P, VOL, SURF, VISC, EPBX, BND>::value))) k_rhs(RHSArgs a) {
   using S = Shape<DIM, P, EPBX>;
-  using SC = RhsScratch<R, DIM, P>;
+  using SC = RhsWgScratch<R, DIM, P, EPBX>;
+  using VP = VolPlan<DIM, P, EPBX>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NVD = DIM + 1;                // viscous terms: momentum and energy
   constexpr int NFACE = 2 * DIM * NF;
   constexpr int NFR = (EPB * NFACE + TPB - 1) / TPB;  // face-point rounds per lane
-  constexpr int NPL = SC::NPL, NPAIR = SC::NPAIR, NI = SC::NI;
+  constexpr int NPL = VP::NPL, NPAIR = VP::NPAIR, NI = SC::NI;
   constexpr bool FP64 = sizeof(R) == sizeof(double);
 
   using SX = StagedIndex<Np, ND>;
@@ -484,8 +560,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
   __shared__ R sp[EPB][NDS];
   __shared__ R sir[EPB][NDS];
   __shared__ R spd_[FP64 ? 1 : EPB][FP64 ? 1 : NDS];  // p - p_ref (mixed precision)
-  __shared__ double sraw[EPB][SC::DOUBLES];  // flux scratch (as R) or indicator sweeps (FP64)
-  __shared__ double srp[EPB][ND];            // pressure * density, FP64, for the indicator
+  __shared__ double sraw[SC::DOUBLES];  // flux rows (as R) or indicator sweeps (FP64), see RhsWgScratch
   __shared__ R sD2[Np * Np];
   __shared__ double sVi[Np * Np];
   __shared__ ElemMeta<DIM, EPB> sm;
@@ -494,7 +569,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
   // points of the other elements of the workgroup, which in the serving
   // pool may belong to another case
   __shared__ double sprefe[FP64 ? 1 : EPB];
-  auto sscr = [&](int pe, int row, int q) -> R & { return reinterpret_cast<R *>(sraw[pe])[row * NQ + q]; };
+  auto sscr = [&](int row, int q) -> R & { return reinterpret_cast<R *>(sraw)[row * NQ + q]; };
+  double *const sind = sraw + (lane_ok_pre(threadIdx.x, ND, EPB) ? (threadIdx.x / ND) * SC::IND_D : 0);
   // pressure for the momentum fluxes: p itself in FP64, p - p_ref in the mixed mode
   auto spd = [&](int pe) -> const R * {
     if constexpr (FP64) return sp[pe];
@@ -551,40 +627,11 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
     qown[q] = a.U[q * Ntot + ebase];
     fsum[q] = R(0);
   }
-  if constexpr (VISC) {
-    // viscous volume divergence and the own half of the face term, from the gradient kernel
-#pragma unroll
-    for (int c = 0; c < NVD; ++c) fsum[1 + c] = Vd[c * Ntot + ebase];
-  }
   static_assert(Np * Np <= TPB, "one basis entry per lane");
   const int kb = min(tid, Np * Np - 1);
   const double bD2 = a.basis.Dhat2[kb], bVi = a.basis.Vinv[kb];
 
   meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
-  double qn[NFR][NQ];
-  R gn[NFR][VISC ? NVD : 1];
-#pragma unroll
-  for (int r = 0; r < NFR; ++r) {
-    const int fi = min(tid + r * TPB, NFT - 1);
-    const int pe = fi / NFACE;
-    const int fl = fi - pe * NFACE;
-    const int f = fl / NF, t = fl - f * NF;
-    const int d = f >> 1, s = f & 1;
-    const int nbs = max(fnb[r], 0);
-    size_t nbase = (size_t)nbs * ND + face_node<DIM, Np>(d, 1 - s, t);
-    // a boundary point loads the element's own trace (FP64), the input of
-    // its exterior state and wall pressure
-    if constexpr (BND)
-      if (fnb[r] < 0) nbase = (size_t)min(ebase_blk + pe, a.ne - 1) * ND + face_node<DIM, Np>(d, s, t);
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
-    if constexpr (VISC) {
-      // the neighbour's n.Fv at this point (the own side's half is already in Vd)
-      const size_t gb_n = (size_t)nbs * NFACE + (f ^ 1) * NF + t;
-#pragma unroll
-      for (int c = 0; c < NVD; ++c) gn[r][c] = Gf[c * Gtot + gb_n];
-    }
-  }
   if (tid < Np * Np) {
     sD2[tid] = R(bD2);
     sVi[tid] = bVi;
@@ -606,7 +653,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
     sir[el][SX::at(node)] = me.ir;
     if constexpr (FP64) {
       sp[el][SX::at(node)] = me.p;
-      if (a.blending) srp[el][node] = me.p * me.q[0];
+      if (a.blending) sind[node] = me.p * me.q[0];
     } else {
       // pressure from the FP64 state: p rounded once, p - p_ref formed before rounding
       Pt<double, DIM> m64;
@@ -615,8 +662,10 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
       derive(a.gas, m64);
       sp[el][SX::at(node)] = R(m64.p);
       spd_[el][SX::at(node)] = R(m64.p - pref);
-      if (a.blending) srp[el][node] = m64.p * m64.q[0];
+      if (a.blending) sind[node] = m64.p * m64.q[0];
     }
+  } else if (lane_ok && a.blending) {
+    sind[node] = 0.0;
   }
   lds_sync();
 
@@ -630,7 +679,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
   // ---- blending coefficient (needs only the staged state; FP64)
   double alpha = 0.0;
   if (a.blending) {
-    alpha = blending_alpha<DIM, P>(a, sraw[elc], sVi, srp[elc], node, i, j, k, lane_ok, valid);
+    alpha = blending_alpha<DIM, P>(a, sind, sVi, node, i, j, k, lane_ok, valid);
     if (valid && node == 0) a.alpha[e] = alpha;
     if (valid && alpha > 1e-16) sblend = 1;
     lds_sync();
@@ -641,9 +690,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
   //      The pairs of all directions form one list processed in full
   //      rounds; direction dir's fluxes go to scratch slot dir & 1 and are
   //      gathered as soon as the round holding its last pair is done.
-  constexpr int NPT = EPB * NPAIR;                     // pairs of one direction
-  constexpr int NRND = (DIM * NPT + TPB - 1) / TPB;
-  static_assert(DIM < 3 || (2 * NPT) / TPB > (NPT - 1) / TPB, "slot reuse needs a round boundary");
+  constexpr int NPT = VP::NPT;    // pairs of one direction
+  constexpr int NRND = VP::NRND;
   R rate[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) rate[q] = R(0);
@@ -666,12 +714,12 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
         R F[NQ];
         volume_flux<VOL, R, DIM>(g, lo, hi, met, F);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) sscr(pe, row, q) = F[q];
+        for (int q = 0; q < NQ; ++q) sscr(row, q) = F[q];
       }
     }
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
-      if (((dir + 1) * NPT - 1) / TPB != rr) continue;
+      if (VP::last(dir) != rr) continue;
       lds_sync();
       if (valid) {
         const int mi = dir == 0 ? i : (dir == 1 ? j : k);
@@ -681,8 +729,9 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
           if (m == mi) continue;
           const int pr = m < mi ? pair_index<Np>(m, mi) : pair_index<Np>(mi, m);
           const R coef = sD2[mi * Np + m];
+          const int prow = VP::phys(dir, el * NPAIR + line * NPL + pr);
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr(el, (dir & 1) * NPAIR + line * NPL + pr, q);
+          for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr(prow, q);
         }
       }
       lds_sync();
@@ -717,7 +766,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
           R F[NQ];
           surface_flux<SURF, R, DIM>(g, sl, sr, nor, F);
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) sscr(pe, ii, q) = F[q];
+          for (int q = 0; q < NQ; ++q) sscr(pe * NI + ii, q) = F[q];
         }
       }
       lds_sync();
@@ -727,8 +776,8 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
         const R den = R(Je * a.basis.w[pos]);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          const R fm = pos > 0 ? sscr(el, L * (Np - 1) + pos - 1, q) : R(0);
-          const R fp = pos < Np - 1 ? sscr(el, L * (Np - 1) + pos, q) : R(0);
+          const R fm = pos > 0 ? sscr(el * NI + L * (Np - 1) + pos - 1, q) : R(0);
+          const R fp = pos < Np - 1 ? sscr(el * NI + L * (Np - 1) + pos, q) : R(0);
           const R val = (fm - fp) / den;
           dfv[q] = dir == 0 ? val : dfv[q] + val;
         }
@@ -742,21 +791,32 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
     }
   }
 
-  // RK base state (mode 2), needed by the epilogue; its loads overlap the
-  // face work.  Loaded unconditionally, from the stage state itself in the
-  // other modes (lines already in cache), so that waits stay exact.  The
-  // mixed mode also re-reads the FP64 stage state here (its LDS copy is FP32).
-  double xb[NQ], q64[FP64 ? 1 : NQ];
-  {
-    const double *Xp = a.mode == 2 ? a.X : a.U;
+  // the neighbours' face traces (state and, for Navier-Stokes, n.Fv),
+  // issued once the volume and subcell terms no longer hold registers
+  double qn[NFR][NQ];
+  R gn[NFR][VISC ? NVD : 1];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) xb[q] = Xp[q * Ntot + ebase];
-    if constexpr (!FP64) {
+  for (int r = 0; r < NFR; ++r) {
+    const int fi = min(tid + r * TPB, NFT - 1);
+    const int pe = fi / NFACE;
+    const int fl = fi - pe * NFACE;
+    const int f = fl / NF, t = fl - f * NF;
+    const int d = f >> 1, s = f & 1;
+    const int nbs = max(fnb[r], 0);
+    size_t nbase = (size_t)nbs * ND + face_node<DIM, Np>(d, 1 - s, t);
+    // a boundary point loads the element's own trace (FP64), the input of
+    // its exterior state and wall pressure
+    if constexpr (BND)
+      if (fnb[r] < 0) nbase = (size_t)min(ebase_blk + pe, a.ne - 1) * ND + face_node<DIM, Np>(d, s, t);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) q64[q] = a.U[q * Ntot + ebase];
+    for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
+    if constexpr (VISC) {
+      // the neighbour's n.Fv at this point (the own side's half is already in Vd)
+      const size_t gb_n = (size_t)nbs * NFACE + (f ^ 1) * NF + t;
+#pragma unroll
+      for (int c = 0; c < NVD; ++c) gn[r][c] = Gf[c * Gtot + gb_n];
     }
   }
-
   // ---- faces: Riemann flux at every point of the element's own faces
   //      (EulerOperator_impl.hpp:211-392; NSOperator_impl.hpp:579-713),
   //      neighbour traces from the registers loaded at the top, which
@@ -850,9 +910,30 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
         }
         const R cs = (interior && !s) ? wown : -wown;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) sscr(pe, fl, q) = cs * F[q];
+        for (int q = 0; q < NQ; ++q) sscr(pe * NFACE + fl, q) = cs * F[q];
       }
     }
+  }
+  // RK base state (mode 2) and, for Navier-Stokes, the viscous volume
+  // divergence with the own half of the face term (from the gradient
+  // kernel): needed only by the face gather and the epilogue, so issued
+  // after the face fluxes to keep them out of that phase's registers.
+  // The base state is loaded unconditionally, from the stage state itself
+  // in the other modes (lines already in cache), so that waits stay exact;
+  // the mixed mode also re-reads the FP64 stage state (its LDS copy is FP32).
+  double xb[NQ], q64[FP64 ? 1 : NQ];
+  {
+    const double *Xp = a.mode == 2 ? a.X : a.U;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) xb[q] = Xp[q * Ntot + ebase];
+    if constexpr (!FP64) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) q64[q] = a.U[q * Ntot + ebase];
+    }
+  }
+  if constexpr (VISC) {
+#pragma unroll
+    for (int c = 0; c < NVD; ++c) fsum[1 + c] = Vd[c * Ntot + ebase];
   }
   lds_sync();
   if (valid) {
@@ -864,7 +945,7 @@ __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_rhs(RHSArgs a) {
       const int row = (2 * d + (hi ? 1 : 0)) * NF + face_index<DIM, Np>(d, i, j, k);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        const R v = sscr(el, row, q);
+        const R v = sscr(el * NFACE + row, q);
         fsum[q] += on ? v : R(0);
       }
     }
