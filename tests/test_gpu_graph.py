@@ -68,3 +68,46 @@ def test_captured_steps_match_eager(dim, order, n, flow, blending, precision):
         op.Step(other, 0.0, dt)
         torch.cuda.synchronize()
     assert np.isfinite(other.cpu().numpy()).all()
+
+
+def test_mixed_first_evaluation_under_capture_is_rejected():
+    """The mixed mode fixes its background pressure from the first state a
+    handle evaluates; recorded into a graph, that computation would rerun on
+    every replay, so a first evaluation under capture fails loudly and leaves
+    the handle usable: eager steps afterwards, then a capture, replay as the
+    eager steps do."""
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+
+    _say("case graph capture: mixed first evaluation under capture")
+    rt, mesh = make_case(3, 3, (2, 2, 2), "chandrashekar", flow=A.FLOW_NAVIER_STOKES, blending=False)
+    rt = dataclasses.replace(rt, precision=A.PRECISION_MIXED)
+    U0 = smooth_state(mesh, 3, seed=41, amp=0.2)
+    op = MakeRHSOperator(rt, mesh)
+    dt = 2e-4
+    u0 = torch.from_numpy(U0).cuda()
+    state = u0.clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with pytest.raises(A.TheseusError, match="captured"):
+        with torch.cuda.graph(graph, stream=side):
+            op.Step(state, 0.0, dt)
+    torch.cuda.synchronize()
+    eager = u0.clone()
+    for _ in range(2):
+        op.Step(eager, 0.0, dt)
+    torch.cuda.synchronize()
+    ref = eager.cpu().numpy()
+    state.copy_(u0)
+    graph2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph2, stream=side):
+        for _ in range(2):
+            op.Step(state, 0.0, dt)
+    torch.cuda.synchronize()
+    state.copy_(u0)
+    graph2.replay()
+    torch.cuda.synchronize()
+    assert np.array_equal(state.cpu().numpy(), ref)
+    op.close()

@@ -346,7 +346,16 @@ static void timed(theseus_rhs *h, int kind, hipStream_t s, F &&launch) {
 static int enqueue_rhs(theseus_rhs *h, RHSArgs a, hipStream_t s) {
   const int nb = h->nblocks();
   if (h->pm && !h->pref_set) {
-    // the mixed mode's background pressure, fixed from the first state evaluated
+    // the mixed mode's background pressure, fixed from the first state
+    // evaluated; that evaluation must run eagerly: recorded into a graph, the
+    // kernel computing it would rerun from each replay's state
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const hipError_t ce = hipStreamIsCapturing(s, &cs);
+    if (ce != hipSuccess) return fail(THESEUS_ERR_DEVICE, std::string("hipStreamIsCapturing: ") + hipGetErrorString(ce));
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(THESEUS_ERR_INVALID_ARGUMENT,
+                  "mixed precision: the first evaluation of a handle fixes its background pressure and must not be "
+                  "captured into a graph; evaluate or step once before capturing");
     if (h->d.dim == 2) hipLaunchKernelGGL(k_pref<2>, dim3(1), dim3(64), 0, s, a.U, h->Ntot, h->gas, h->d_pref);
     else hipLaunchKernelGGL(k_pref<3>, dim3(1), dim3(64), 0, s, a.U, h->Ntot, h->gas, h->d_pref);
     h->pref_set = true;
@@ -622,6 +631,7 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     h->ltev.inv = h->d_lte + nx + 2 * ny + 6 * nx * ny;
   }
   hipError_t e = hipMemset(h->d_alpha, 0, ne * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(h->d_pref, 0, sizeof(double));  // defined before the first evaluation sets it
   if (e != hipSuccess) {
     theseus_rhs_destroy(h);
     return fail(THESEUS_ERR_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
