@@ -147,6 +147,27 @@ typedef struct theseus_case_desc {
   int num_bc;
   const int *bc_type;          /* [num_bc] THESEUS_BC_* */
   const double *bc_data;       /* [num_bc][THESEUS_BC_DATA_STRIDE]; may be NULL when no boundary needs data */
+
+  /* Element orientation (may be NULL: every element is stored in the
+   * aligned frame).  The kernels need neighbouring elements to share their
+   * faces with the same in-face node order (the "aligned" frame, which
+   * face_neighbor, elem_jac and elem_adj describe).  A mesh whose elements
+   * keep their own local frames (an MFEM mesh whose faces carry
+   * orientations, dgsem_cache_utilities.hpp:533-541) is described in the
+   * aligned frame plus, per element, the symmetry of the reference
+   * square/cube that maps its storage frame onto the aligned one:
+   * elem_orient[e] = code (theseus_orient_elements computes them), and the
+   * state vectors keep the storage order: the node with aligned
+   * lexicographic index n of element e is stored at
+   * e * ndof + theseus_orient_node(dim, order, elem_orient[e], n).
+   * Code c = perm * 2^dim + flip (perm: the index of an axis permutation
+   * in lexicographic order; flip: bit dim-1-k set when aligned axis k runs
+   * opposite to its storage axis): aligned index a_k = s_{perm[k]}, or
+   * p - s_{perm[k]} when flipped, from the storage multi-index s.  A
+   * handle with any nonzero code permutes its inputs into the aligned frame
+   * and its outputs back in two extra passes over the state (16 bytes per
+   * value each way); a handle with none runs as without the array. */
+  const unsigned char *elem_orient; /* [ne] or NULL */
 } theseus_case_desc;
 
 typedef struct theseus_rhs theseus_rhs; /* opaque */
@@ -285,6 +306,33 @@ THESEUS_API int theseus_rhs_enable_kernel_timing(theseus_rhs *h, int capacity);
 /* Synchronises the recorded events; returns the mean duration in ms and the
  * number of timed launches of one kernel kind. */
 THESEUS_API int theseus_rhs_kernel_timing(theseus_rhs *h, int kind, double *mean_ms, int *count);
+
+/* ---- element orientation (the counterpart of MFEM's face orientations,
+ * dgsem_cache_utilities.hpp:533-541, GetPermutedIndex) -------------------
+ * Aligns the elements of a conforming quad/hex mesh: from each element's
+ * corner vertex ids in its own lexicographic corner order (corner c has
+ * local coordinates ((c>>0)&1, (c>>1)&1, (c>>2)&1); MFEM's counterclockwise
+ * quad corners 0,1,2,3 are lexicographic 0,1,3,2, its hex corners 0..7 are
+ * 0,1,3,2,4,5,7,6), with periodic vertices already identified, a
+ * breadth-first walk over face neighbours chooses for every element the
+ * symmetry (elem_orient code, as in theseus_case_desc) under which each
+ * shared face is seen with the same in-face corner order from both sides.
+ * Outputs, each may be NULL: elem_orient[ne]; face_neighbor[ne][2*dim] in
+ * the aligned frame, -1 on faces without a neighbour (the caller puts its
+ * boundary index there); and, when corner_coords ([ne][2^dim][dim], the
+ * corners' physical coordinates in the same storage order) is given,
+ * elem_jac[ne] and elem_adj[ne][dim][dim] of the aligned frame, after
+ * turning every connected component whose aligned frame is mirrored
+ * (negative Jacobians) around.  Fails with THESEUS_ERR_UNSUPPORTED when the
+ * mesh cannot be aligned (unstructured vertices), a face has more than two
+ * elements, an element is not affine or the Jacobian signs disagree. */
+THESEUS_API int theseus_orient_elements(int dim, int num_elements, const long long *elem_vertices,
+                                        const double *corner_coords, unsigned char *elem_orient, int *face_neighbor,
+                                        double *elem_jac, double *elem_adj);
+/* Storage index within its element of the aligned node n (lexicographic) of
+ * an element with orientation `code` (see theseus_case_desc); negative on
+ * invalid arguments. */
+THESEUS_API int theseus_orient_node(int dim, int order, int code, int node);
 
 /* Thread-local message of the last failing call. */
 THESEUS_API const char *theseus_last_error(void);

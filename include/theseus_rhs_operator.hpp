@@ -42,10 +42,32 @@ inline void check_status(int rc) {
   throw std::runtime_error(msg);
 }
 
+/* theseus_orient_elements (see theseus_rhs.h): aligns a quad/hex mesh given
+ * each element's corner vertex ids and coordinates in its own lexicographic
+ * corner order; throws std::invalid_argument when it cannot. */
+inline void OrientElements(int dim, int num_elements, const long long *elem_vertices, const double *corner_coords,
+                           unsigned char *elem_orient, int *face_neighbor, double *elem_jac, double *elem_adj) {
+  check_status(theseus_orient_elements(dim, num_elements, elem_vertices, corner_coords, elem_orient, face_neighbor,
+                                       elem_jac, elem_adj));
+}
+
+/* Storage corner (lexicographic) of aligned corner c of an element with
+ * orientation `code`: the corners are the nodes of order 1. */
+inline int StorageCorner(int dim, int code, int c) {
+  const int r = theseus_orient_node(dim, 1, code, c);
+  check_status(r < 0 ? r : THESEUS_OK);
+  return r;
+}
+
 class RHSOperator {
  public:
+  /* An empty operator, created later by move-assignment (the reference
+   * builds its operator before the boundary data exists and completes it in
+   * Finalize, src/Simulation.cpp:493,996-997). */
+  RHSOperator() = default;
   /* MakeRHSOperator + Finalize (SimFactory.hpp:22-31, RHSOperator_impl.hpp:273-314) */
   explicit RHSOperator(const theseus_case_desc &desc) { check_status(theseus_rhs_create(&desc, &h_)); }
+  explicit operator bool() const { return h_ != nullptr; }
   ~RHSOperator() {
     if (h_) theseus_rhs_destroy(h_);
   }

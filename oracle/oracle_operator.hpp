@@ -213,6 +213,11 @@ struct Case {
   }
 
   explicit Case(const theseus_case_desc &d) : basis(d.order) {
+    // the oracle works in the aligned frame only (the engine's orientation
+    // pass is checked against it by permuting the states)
+    if (d.elem_orient)
+      for (int e = 0; e < d.num_elements; ++e)
+        if (d.elem_orient[e] != 0) throw std::invalid_argument("the oracle takes aligned meshes (elem_orient all zero)");
     dim = d.dim;
     p = d.order;
     Np = p + 1;

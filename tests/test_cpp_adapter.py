@@ -61,8 +61,27 @@ int main() {
   theseus::StabilityEstimate s;
   s.advective_rate = 2.0; s.surface_rate = 3.0; s.diffusive_rate = 0.5;
   if (s.TotalRate() == 3.5) seen |= 8;
-  std::printf("seen=%d\n", seen);
-  return seen == 15 ? 0 : 1;
+  // deferred creation (the adapter builds its engine in Finalize)
+  theseus::RHSOperator later;
+  if (!later) seen |= 16;
+  // orientation helpers: two quads sharing an edge, the second stored rotated
+  // by a quarter turn; aligned, their shared face is (0, 1) of the first
+  const long long vids[8] = {0, 1, 3, 4, 2, 5, 1, 4};
+  const double xyz[16] = {0, 0, 1, 0, 0, 1, 1, 1, 2, 0, 2, 1, 1, 0, 1, 1};
+  unsigned char code[2];
+  int nbr[8];
+  double jac[2], adj[8];
+  theseus::OrientElements(2, 2, vids, xyz, code, nbr, jac, adj);
+  if (code[0] == 0 && code[1] != 0 && nbr[1] == 1 && nbr[4 + 0] == 0 && jac[0] == 1.0 && jac[1] == 1.0) seen |= 32;
+  if (theseus::StorageCorner(2, 0, 3) == 3) seen |= 64;
+  try {
+    theseus::OrientElements(4, 2, vids, xyz, code, nbr, jac, adj);
+  } catch (const std::invalid_argument &) {
+    seen |= 128;
+  }
+  std::printf("seen=%d code=%d,%d nbr=%d,%d,%d,%d|%d,%d,%d,%d\n", seen, code[0], code[1], nbr[0], nbr[1], nbr[2], nbr[3],
+              nbr[4], nbr[5], nbr[6], nbr[7]);
+  return seen == 255 ? 0 : 1;
 }
 """
 

@@ -61,6 +61,8 @@ EXPORTED_SYMBOLS = (
     "theseus_rhs_integral_measures",
     "theseus_rhs_integral_measures_baseline",
     "theseus_rhs_integral_measures_noalloc",
+    "theseus_orient_elements",
+    "theseus_orient_node",
     "theseus_rhs_step_ssprk3_cfl",
     "theseus_rhs_enable_kernel_timing",
     "theseus_rhs_kernel_timing",
@@ -99,6 +101,7 @@ class CaseDesc(C.Structure):
         ("num_bc", C.c_int),
         ("bc_type", C.POINTER(C.c_int)),
         ("bc_data", C.POINTER(C.c_double)),
+        ("elem_orient", C.POINTER(C.c_ubyte)),
     ]
 
 
@@ -208,6 +211,11 @@ def load(path: Path | str | None = None) -> C.CDLL:
     lib.theseus_rhs_integral_measures_baseline.restype = C.c_int
     lib.theseus_rhs_integral_measures_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.theseus_rhs_integral_measures_noalloc.restype = C.c_int
+    lib.theseus_orient_elements.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]
+    lib.theseus_orient_elements.restype = C.c_int
+    lib.theseus_orient_node.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.theseus_orient_node.restype = C.c_int
     lib.theseus_rhs_step_ssprk3_cfl.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.theseus_rhs_step_ssprk3_cfl.restype = C.c_int
     lib.theseus_last_error.argtypes = []

@@ -9,6 +9,8 @@
 // update into the RHS kernel's epilogue.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -242,6 +244,70 @@ __global__ void k_pref(const double *U, long long Ntot, GasC<double> g, double *
   out[0] = s.p;
 }
 
+// ---------------------------------------------------------------------------
+// Element orientation: the symmetries of [0,1]^dim (codes as in
+// theseus_case_desc) and the pass that moves a state between the storage
+// frames of the elements and the aligned frame the kernels work in.
+// ---------------------------------------------------------------------------
+struct Sym {
+  int perm[3];  // aligned axis k is storage axis perm[k] ...
+  int flip[3];  // ... reversed when flip[k]
+};
+
+// In code order: axis permutations in lexicographic order, then the flips
+// with aligned axis 0 the slowest bit.
+static std::vector<Sym> symmetries(int dim) {
+  std::vector<Sym> out;
+  int perm[3] = {0, 1, 2};
+  do {
+    for (int f = 0; f < (1 << dim); ++f) {
+      Sym y{};
+      for (int k = 0; k < dim; ++k) {
+        y.perm[k] = perm[k];
+        y.flip[k] = (f >> (dim - 1 - k)) & 1;
+      }
+      out.push_back(y);
+    }
+  } while (std::next_permutation(perm, perm + dim));
+  return out;
+}
+
+// Storage corner of aligned corner c (corner bits = local coordinates).
+static int sym_corner(const Sym &y, int dim, int c) {
+  int old = 0;
+  for (int k = 0; k < dim; ++k) old |= (((c >> k) & 1) ^ y.flip[k]) << y.perm[k];
+  return old;
+}
+
+// Storage node index of aligned node n (lexicographic, Np nodes per axis).
+static int sym_node(const Sym &y, int dim, int Np, int n) {
+  int sidx[3] = {0, 0, 0};
+  for (int k = 0, r = n; k < dim; ++k, r /= Np) {
+    const int a = r % Np;
+    sidx[y.perm[k]] = y.flip[k] ? Np - 1 - a : a;
+  }
+  int out = 0;
+  for (int k = dim - 1; k >= 0; --k) out = out * Np + sidx[k];
+  return out;
+}
+
+// dst <- src between storage and aligned frames: one lane per (element,
+// aligned node), all equations.  tab[code * ND + n] = storage index of
+// aligned node n.
+template <bool TO_ALIGNED>
+__global__ void __launch_bounds__(256) k_orient(const double *src, double *dst, const unsigned char *code,
+                                                const int *tab, int ND, long long Ntot, int neq) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Ntot) return;
+  const long long e = t / ND;
+  const int n = (int)(t - e * ND);
+  const long long st = e * ND + tab[(int)code[e] * ND + n];
+  for (int q = 0; q < neq; ++q) {
+    if (TO_ALIGNED) dst[q * Ntot + t] = src[q * Ntot + st];
+    else dst[q * Ntot + st] = src[q * Ntot + t];
+  }
+}
+
 }  // namespace thx
 
 using namespace thx;
@@ -287,6 +353,11 @@ struct theseus_rhs {
   theseus_integral_measures diag0{};  // baseline: the first measures with nonzero mass
   bool has_boundary = false;
   int device = 0;  // the device current at create time, which holds the handle's arrays
+  // element orientation: codes, node tables, and the aligned copies of the
+  // input and output state (only when some element is not stored aligned)
+  unsigned char *d_orient = nullptr;
+  int *d_otab = nullptr;
+  double *d_ual = nullptr, *d_dual = nullptr;
   // per-kernel event timing (instrumentation)
   int timing_capacity = 0;
   std::vector<hipEvent_t> ev[2][2];  // [kind][start/stop]
@@ -371,6 +442,28 @@ static int enqueue_rhs(theseus_rhs *h, RHSArgs a, hipStream_t s) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(THESEUS_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
   return THESEUS_OK;
+}
+
+// The orientation pass of a handle whose elements are not all aligned.
+static int orient_pass(const theseus_rhs *h, const double *src, double *dst, bool to_aligned, hipStream_t s) {
+  const int nb = (int)((h->Ntot + 255) / 256);
+  if (to_aligned)
+    hipLaunchKernelGGL(k_orient<true>, dim3(nb), dim3(256), 0, s, src, dst, h->d_orient, h->d_otab, h->ndof, h->Ntot,
+                       h->neq);
+  else
+    hipLaunchKernelGGL(k_orient<false>, dim3(nb), dim3(256), 0, s, src, dst, h->d_orient, h->d_otab, h->ndof,
+                       h->Ntot, h->neq);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(THESEUS_ERR_DEVICE, std::string("orientation pass: ") + hipGetErrorString(e));
+  return THESEUS_OK;
+}
+
+// u in the aligned frame: u itself, or its aligned copy in d_ual.
+static int aligned_input(theseus_rhs *h, const double *u, const double **out, hipStream_t s) {
+  *out = u;
+  if (!h->d_orient) return THESEUS_OK;
+  *out = h->d_ual;
+  return orient_pass(h, u, h->d_ual, true, s);
 }
 
 template <typename T>
@@ -630,6 +723,29 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     h->ltev.planes = h->d_lte + nx + 2 * ny;
     h->ltev.inv = h->d_lte + nx + 2 * ny + 6 * nx * ny;
   }
+  if (d.elem_orient) {
+    const std::vector<Sym> syms = symmetries(dim);
+    bool any = false;
+    for (int el = 0; el < ne; ++el) {
+      if (d.elem_orient[el] >= syms.size()) {
+        theseus_rhs_destroy(h);
+        return fail(THESEUS_ERR_INVALID_ARGUMENT, "elem_orient[" + std::to_string(el) + "] is not a symmetry code (< " +
+                                                      std::to_string(syms.size()) + ")");
+      }
+      any = any || d.elem_orient[el] != 0;
+    }
+    if (any) {
+      std::vector<int> tab(syms.size() * (size_t)h->ndof);
+      for (size_t c = 0; c < syms.size(); ++c)
+        for (int n = 0; n < h->ndof; ++n) tab[c * h->ndof + n] = sym_node(syms[c], dim, h->Np, n);
+      if ((rc = upload(&h->d_orient, d.elem_orient, ne)) || (rc = upload(&h->d_otab, tab.data(), tab.size())) ||
+          (rc = upload<double>(&h->d_ual, nullptr, M)) || (rc = upload<double>(&h->d_dual, nullptr, M))) {
+        theseus_rhs_destroy(h);
+        return rc;
+      }
+    }
+  }
+  h->d.elem_orient = nullptr;
   hipError_t e = hipMemset(h->d_alpha, 0, ne * sizeof(double));
   if (e == hipSuccess) e = hipMemset(h->d_pref, 0, sizeof(double));  // defined before the first evaluation sets it
   if (e != hipSuccess) {
@@ -700,6 +816,10 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_stage[1]);
   hipFree(h->d_rates);
   hipFree(h->d_meas);
+  hipFree(h->d_orient);
+  hipFree(h->d_otab);
+  hipFree(h->d_ual);
+  hipFree(h->d_dual);
   delete h;
   return THESEUS_OK;
 }
@@ -714,8 +834,14 @@ int theseus_rhs_sizes(const theseus_rhs *h, long long *ndof_total, int *num_equa
 int theseus_rhs_eval(theseus_rhs *h, const double *u, double *dudt, void *stream) {
   if (!h || !u || !dudt) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle or vector");
   if ((const double *)dudt == u) return fail(THESEUS_ERR_INVALID_ARGUMENT, "u and dudt must not alias");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->d_orient) {
+    int rc = orient_pass(h, u, h->d_ual, true, s);
+    if (!rc) rc = enqueue_rhs(h, h->args(h->d_ual, h->d_dual), s);
+    return rc ? rc : orient_pass(h, h->d_dual, dudt, false, s);
+  }
   RHSArgs a = h->args(u, dudt);
-  return enqueue_rhs(h, a, (hipStream_t)stream);
+  return enqueue_rhs(h, a, s);
 }
 
 // The three Shu-Osher stages; dt_dev (device, may be null) overrides dt.
@@ -805,16 +931,30 @@ int theseus_rhs_step_ssprk3(theseus_rhs *h, double *u, double t, double dt, void
   (void)t;  // the operator is autonomous (no time-dependent sources or boundary data)
   if (!h || !u) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle or vector");
   if (!std::isfinite(dt)) return fail(THESEUS_ERR_INVALID_ARGUMENT, "dt must be finite");
-  return enqueue_ssprk3(h, u, dt, nullptr, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  if (h->d_orient) {
+    int rc = orient_pass(h, u, h->d_ual, true, s);
+    if (!rc) rc = enqueue_ssprk3(h, h->d_ual, dt, nullptr, s);
+    return rc ? rc : orient_pass(h, h->d_ual, u, false, s);
+  }
+  return enqueue_ssprk3(h, u, dt, nullptr, s);
 }
 
 int theseus_rhs_step_ssprk3_cfl(theseus_rhs *h, double *u, theseus_clock *clock, void *stream) {
   if (!h || !u || !clock) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle, vector or clock");
   hipStream_t s = (hipStream_t)stream;
-  enqueue_stability(h, u, s);
+  double *ua = u;
+  if (h->d_orient) {
+    const int rc = orient_pass(h, u, h->d_ual, true, s);
+    if (rc) return rc;
+    ua = h->d_ual;
+  }
+  enqueue_stability(h, ua, s);
   hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, clock, h->d_rates);
   HIP_TRY(hipGetLastError());
-  return enqueue_ssprk3(h, u, 0.0, &clock->dt_real, s);
+  const int rc = enqueue_ssprk3(h, ua, 0.0, &clock->dt_real, s);
+  if (rc || !h->d_orient) return rc;
+  return orient_pass(h, h->d_ual, u, false, s);
 }
 
 // Integral measures of elements [e0, e0 + ne) of U; partial holds the
@@ -851,8 +991,11 @@ int theseus_rhs_integral_measures(theseus_rhs *h, const double *u, theseus_integ
     const int rc = upload<double>(&h->d_meas, nullptr, (size_t)(h->nblocks() + 1) * NMEAS);
     if (rc) return rc;
   }
-  const int rc = measures_impl(h->sk, u, h->Ntot, 0, h->d.num_elements, h->d_J, h->gas, h->basis, h->ltev, h->d_meas,
-                               out, (hipStream_t)stream);
+  const double *ua = u;
+  int rc = aligned_input(h, u, &ua, (hipStream_t)stream);
+  if (rc) return rc;
+  rc = measures_impl(h->sk, ua, h->Ntot, 0, h->d.num_elements, h->d_J, h->gas, h->basis, h->ltev, h->d_meas, out,
+                     (hipStream_t)stream);
   if (rc) return rc;
   if (h->diag0.mass == 0.0) h->diag0 = *out;
   return THESEUS_OK;
@@ -879,12 +1022,15 @@ int theseus_rhs_integral_measures_noalloc(theseus_rhs *h, const double *u, doubl
                                           std::to_string(NOALLOC_MAX_WORKGROUPS * h->sk->epb) +
                                           " elements at this order); use theseus_rhs_integral_measures");
   double *partial = nullptr;
-  const int rc = scratch_for_current_device(&partial);
+  int rc = scratch_for_current_device(&partial);
   if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const double *ua = u;
+  if ((rc = aligned_input(h, u, &ua, s))) return rc;
   // every workgroup writes its own partials, so nothing needs clearing
   MeasArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.U = u;
+  a.U = ua;
   a.partial = partial;
   a.Ntot = h->Ntot;
   a.e0 = 0;
@@ -893,7 +1039,6 @@ int theseus_rhs_integral_measures_noalloc(theseus_rhs *h, const double *u, doubl
   a.gas = h->gas;
   std::memcpy(a.w, h->basis.w, sizeof(a.w));
   a.lte = h->ltev;
-  hipStream_t s = (hipStream_t)stream;
   h->sk->measures(a, nb, s);
   hipLaunchKernelGGL(k_measures_final, dim3(1), dim3(256), 0, s, partial, nb, out);
   HIP_TRY(hipGetLastError());
@@ -903,7 +1048,10 @@ int theseus_rhs_integral_measures_noalloc(theseus_rhs *h, const double *u, doubl
 int theseus_rhs_estimate_stability(theseus_rhs *h, const double *u, theseus_stability *out, void *stream) {
   if (!h || !u || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle, vector or output");
   hipStream_t s = (hipStream_t)stream;
-  enqueue_stability(h, u, s);
+  const double *ua = u;
+  const int rc = aligned_input(h, u, &ua, s);
+  if (rc) return rc;
+  enqueue_stability(h, ua, s);
   HIP_TRY(hipGetLastError());
   unsigned long long r[3];
   HIP_TRY(hipMemcpyAsync(r, h->d_rates, sizeof r, hipMemcpyDeviceToHost, s));
@@ -913,6 +1061,192 @@ int theseus_rhs_estimate_stability(theseus_rhs *h, const double *u, theseus_stab
   out->advective_rate = v[0];
   out->surface_rate = v[1];
   out->diffusive_rate = v[2];
+  return THESEUS_OK;
+}
+
+int theseus_orient_node(int dim, int order, int code, int node) {
+  if ((dim != 2 && dim != 3) || order < 1) return fail(THESEUS_ERR_INVALID_ARGUMENT, "dim must be 2 or 3, order >= 1");
+  const std::vector<Sym> syms = symmetries(dim);
+  const int Np = order + 1, ND = dim == 2 ? Np * Np : Np * Np * Np;
+  if (code < 0 || code >= (int)syms.size() || node < 0 || node >= ND)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "code or node out of range");
+  return sym_node(syms[code], dim, Np, node);
+}
+
+// Breadth-first alignment of the elements (a restatement for the C-ABI of
+// the case reader's orientation walk): the first element of each connected
+// component keeps its frame; a neighbour across face (d, s) takes the
+// symmetry under which its face (d, 1-s) lists the shared corners in the
+// order the aligned element lists them.
+int theseus_orient_elements(int dim, int ne, const long long *vids, const double *coords, unsigned char *orient_out,
+                            int *nbr_out, double *jac_out, double *adj_out) {
+  if ((dim != 2 && dim != 3) || ne < 1 || !vids)
+    return fail(THESEUS_ERR_INVALID_ARGUMENT, "dim must be 2 or 3, num_elements >= 1, elem_vertices given");
+  const int nc = 1 << dim, nfc = nc / 2;
+  const std::vector<Sym> syms = symmetries(dim);
+  std::vector<std::array<int, 8>> cmap(syms.size());
+  for (size_t c = 0; c < syms.size(); ++c)
+    for (int k = 0; k < nc; ++k) cmap[c][k] = sym_corner(syms[c], dim, k);
+  int faces[3][2][4];
+  for (int d = 0; d < dim; ++d)
+    for (int sd = 0; sd < 2; ++sd)
+      for (int c = 0, m = 0; c < nc; ++c)
+        if (((c >> d) & 1) == sd) faces[d][sd][m++] = c;
+  auto face_key = [&](const long long *v) {
+    std::vector<long long> k(v, v + nfc);
+    std::sort(k.begin(), k.end());
+    return k;
+  };
+  std::map<std::vector<long long>, std::vector<int>> owners;
+  for (int e = 0; e < ne; ++e)
+    for (int d = 0; d < dim; ++d)
+      for (int sd = 0; sd < 2; ++sd) {
+        long long fv[4];
+        for (int m = 0; m < nfc; ++m) fv[m] = vids[(size_t)e * nc + faces[d][sd][m]];
+        std::vector<int> &o = owners[face_key(fv)];
+        o.push_back(e);
+        if (o.size() > 2) return fail(THESEUS_ERR_UNSUPPORTED, "a face is shared by more than two elements");
+      }
+  std::vector<int> code(ne, -1), nbr((size_t)ne * 2 * dim, -1), comp(ne, -1);
+  auto oriented = [&](int e, long long *ov) {
+    for (int c = 0; c < nc; ++c) ov[c] = vids[(size_t)e * nc + cmap[code[e]][c]];
+  };
+  for (int start = 0; start < ne; ++start) {
+    if (code[start] >= 0) continue;
+    code[start] = 0;
+    std::vector<int> stack{start};
+    while (!stack.empty()) {
+      const int e = stack.back();
+      stack.pop_back();
+      comp[e] = start;
+      long long ov[8];
+      oriented(e, ov);
+      for (int d = 0; d < dim; ++d)
+        for (int sd = 0; sd < 2; ++sd) {
+          long long fv[4];
+          for (int m = 0; m < nfc; ++m) fv[m] = ov[faces[d][sd][m]];
+          const std::vector<int> &o = owners[face_key(fv)];
+          int f = -1;
+          for (int x : o)
+            if (x != e) f = x;
+          if (f < 0) {
+            if (o.size() == 2) f = e;  // an element touching itself across a periodic face
+            else continue;
+          }
+          if (code[f] >= 0) {
+            long long of[8];
+            oriented(f, of);
+            for (int m = 0; m < nfc; ++m)
+              if (of[faces[d][1 - sd][m]] != fv[m])
+                return fail(THESEUS_ERR_UNSUPPORTED, "elements " + std::to_string(e) + " and " + std::to_string(f) +
+                                                         " cannot be aligned (the mesh has unstructured vertices)");
+          } else {
+            int found = -1;
+            for (size_t c = 0; c < syms.size() && found < 0; ++c) {
+              bool ok = true;
+              for (int m = 0; m < nfc && ok; ++m) ok = vids[(size_t)f * nc + cmap[c][faces[d][1 - sd][m]]] == fv[m];
+              if (ok) found = (int)c;
+            }
+            if (found < 0)
+              return fail(THESEUS_ERR_UNSUPPORTED, "no orientation of element " + std::to_string(f) +
+                                                       " matches its face with element " + std::to_string(e));
+            code[f] = found;
+            stack.push_back(f);
+          }
+          nbr[(size_t)e * 2 * dim + 2 * d + sd] = f;
+        }
+    }
+  }
+  if (coords) {
+    // corners in the aligned frame; a component aligned from a mirrored
+    // first element has negative Jacobians throughout: reversing aligned
+    // axis 0 in all of it keeps every face pairing and makes them positive
+    auto edges = [&](int e, double E[3][3], double X0[3]) {
+      const double *c0 = coords + ((size_t)e * nc + cmap[code[e]][0]) * dim;
+      for (int k = 0; k < dim; ++k) X0[k] = c0[k];
+      for (int r = 0; r < dim; ++r) {
+        const double *cr = coords + ((size_t)e * nc + cmap[code[e]][1 << r]) * dim;
+        for (int k = 0; k < dim; ++k) E[r][k] = cr[k] - c0[k];
+      }
+    };
+    auto det = [&](double E[3][3]) {
+      return dim == 2 ? E[0][0] * E[1][1] - E[0][1] * E[1][0]
+                      : E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) -
+                            E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
+                            E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
+    };
+    std::map<int, std::pair<int, int>> signs;  // component -> (negative, positive) counts
+    std::vector<double> dets(ne);
+    for (int e = 0; e < ne; ++e) {
+      double E[3][3], X0[3];
+      edges(e, E, X0);
+      dets[e] = det(E);
+      auto &sg = signs[comp[e]];
+      (dets[e] < 0 ? sg.first : sg.second)++;
+    }
+    // symmetry code of "code, then aligned axis 0 reversed"
+    auto flip0 = [&](int c) {
+      for (size_t k = 0; k < syms.size(); ++k) {
+        bool ok = true;
+        for (int m = 0; m < nc && ok; ++m) ok = cmap[k][m] == cmap[c][m ^ 1];
+        if (ok) return (int)k;
+      }
+      return -1;
+    };
+    for (int e = 0; e < ne; ++e) {
+      const auto &sg = signs[comp[e]];
+      if (sg.first && sg.second)
+        return fail(THESEUS_ERR_UNSUPPORTED, "element orientations disagree with the geometry (mixed Jacobian signs)");
+      if (sg.first) {
+        code[e] = flip0(code[e]);
+        std::swap(nbr[(size_t)e * 2 * dim], nbr[(size_t)e * 2 * dim + 1]);
+      }
+    }
+    for (int e = 0; e < ne; ++e) {
+      double E[3][3], X0[3];
+      edges(e, E, X0);
+      double h = 0;
+      for (int r = 0; r < dim; ++r) {
+        double l2 = 0;
+        for (int k = 0; k < dim; ++k) l2 += E[r][k] * E[r][k];
+        h = std::max(h, std::sqrt(l2));
+      }
+      for (int c = 0; c < nc; ++c) {  // affinity: each corner is the origin plus its edges
+        const double *xc = coords + ((size_t)e * nc + cmap[code[e]][c]) * dim;
+        double dev = 0;
+        for (int k = 0; k < dim; ++k) {
+          double pred = X0[k];
+          for (int r = 0; r < dim; ++r) pred += ((c >> r) & 1) * E[r][k];
+          dev += (xc[k] - pred) * (xc[k] - pred);
+        }
+        if (std::sqrt(dev) > 1e-9 * h)
+          return fail(THESEUS_ERR_UNSUPPORTED, "element " + std::to_string(e) + " is not affine");
+      }
+      const double J = det(E);
+      if (!(J > 0)) return fail(THESEUS_ERR_UNSUPPORTED, "element " + std::to_string(e) + " has a non-positive Jacobian");
+      if (jac_out) jac_out[e] = J;
+      if (adj_out) {
+        // adjugate of A (columns = edges): row r = J d(xi_r)/dx
+        double *o = adj_out + (size_t)e * dim * dim;
+        if (dim == 2) {
+          o[0] = E[1][1];
+          o[1] = -E[1][0];
+          o[2] = -E[0][1];
+          o[3] = E[0][0];
+        } else {
+          for (int r = 0; r < 3; ++r) {
+            const int a = (r + 1) % 3, b = (r + 2) % 3;
+            o[r * 3 + 0] = E[a][1] * E[b][2] - E[a][2] * E[b][1];
+            o[r * 3 + 1] = E[a][2] * E[b][0] - E[a][0] * E[b][2];
+            o[r * 3 + 2] = E[a][0] * E[b][1] - E[a][1] * E[b][0];
+          }
+        }
+      }
+    }
+  }
+  if (orient_out)
+    for (int e = 0; e < ne; ++e) orient_out[e] = (unsigned char)code[e];
+  if (nbr_out) std::copy(nbr.begin(), nbr.end(), nbr_out);
   return THESEUS_OK;
 }
 
@@ -1368,6 +1702,10 @@ int theseus_pool_info(const theseus_pool *p, int *capacity_elements, int *free_e
 
 int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double *u0, double cfl, int *case_id,
                        double *dt_out, void *stream) {
+  if (c && c->elem_orient)
+    for (int e = 0; e < c->num_elements; ++e)
+      if (c->elem_orient[e] != 0)
+        return fail(THESEUS_ERR_UNSUPPORTED, "the serving pool takes aligned meshes (elem_orient all zero or NULL)");
   if (!p || !c || !u0) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null pool, case or state");
   if (!(cfl > 0) || !std::isfinite(cfl)) return fail(THESEUS_ERR_INVALID_ARGUMENT, "cfl must be positive");
   const theseus_case_desc &d = *c;

@@ -55,6 +55,10 @@ class AffineMesh:
     elem_attr: np.ndarray = field(default=None, repr=False)
     vertex_ids: np.ndarray = field(default=None, repr=False)  # (ne, 2^dim) after periodic identification
     bc_data: np.ndarray = field(default=None, repr=False)     # (nbc, BC_DATA_STRIDE) boundary payload
+    # per element, the code (index into _symmetries, the engine's
+    # theseus_case_desc.elem_orient convention) of the symmetry that maps the
+    # file's corner order onto the aligned one; None after refinement
+    elem_orient: np.ndarray = field(default=None, repr=False)
 
     @property
     def num_elements(self) -> int:
@@ -205,6 +209,59 @@ def _orient(dim: int, vids: np.ndarray):
     return perm, nbr, comp
 
 
+def storage_frames(mesh: AffineMesh):
+    """The file's own element frames: (corner vertex ids (ne, 2^dim) and
+    corner coordinates (ne, 2^dim, dim), both in each element's file corner
+    order, lexicographic)."""
+    syms = _symmetries(mesh.dim)
+    ne, nc = mesh.num_elements, 1 << mesh.dim
+    vids = np.empty((ne, nc), dtype=np.int64)
+    xyz = np.empty_like(mesh.corners)
+    for e in range(ne):
+        cm = syms[int(mesh.elem_orient[e])]
+        vids[e, cm] = mesh.vertex_ids[e]
+        xyz[e, cm] = mesh.corners[e]
+    return vids, xyz
+
+
+def node_storage_index(dim: int, order: int, code: int) -> np.ndarray:
+    """Storage index within its element of every aligned node for a symmetry
+    code (the engine's theseus_orient_node)."""
+    perm, flip = list(itertools.permutations(range(dim)))[code >> dim], [(code >> (dim - 1 - k)) & 1 for k in range(dim)]
+    Np = order + 1
+    a = np.indices((Np,) * dim).reshape(dim, -1)[::-1]  # aligned multi-index, axis 0 fastest
+    s = np.empty_like(a)
+    for k in range(dim):
+        s[perm[k]] = np.where(flip[k], Np - 1 - a[k], a[k])
+    return sum(s[k] * Np ** k for k in range(dim))
+
+
+def to_storage(U: np.ndarray, mesh: AffineMesh, order: int) -> np.ndarray:
+    """An aligned L-vector rearranged into the elements' file frames."""
+    return _permute_state(U, mesh, order, True)
+
+
+def to_aligned(U: np.ndarray, mesh: AffineMesh, order: int) -> np.ndarray:
+    return _permute_state(U, mesh, order, False)
+
+
+def _permute_state(U, mesh, order, to_st):
+    dim, ne = mesh.dim, mesh.num_elements
+    nd = (order + 1) ** dim
+    V = np.asarray(U).reshape(-1, ne, nd)
+    out = np.empty_like(V)
+    for c in np.unique(mesh.elem_orient):
+        sel = mesh.elem_orient == c
+        idx = node_storage_index(dim, order, int(c))
+        if to_st:
+            tmp = np.empty_like(V[:, sel])
+            tmp[:, :, idx] = V[:, sel]
+            out[:, sel] = tmp
+        else:
+            out[:, sel] = V[:, sel][:, :, idx]
+    return out.reshape(np.shape(U))
+
+
 def _finish(dim, vids, coords_lex, boundary, bc_types, attrs, rtol=1e-9):
     """Orient, check affinity, attach boundary indices."""
     perm, nbr, comp = _orient(dim, vids)
@@ -260,8 +317,10 @@ def _finish(dim, vids, coords_lex, boundary, bc_types, attrs, rtol=1e-9):
     if not (nbr < 0).any():
         types = np.zeros(0, dtype=np.int32)
         data = np.zeros((0, BC_DATA_STRIDE))
+    index = {tuple(cm): k for k, cm in enumerate(_symmetries(dim))}
+    codes = np.array([index[tuple(perm[e])] for e in range(ne)], dtype=np.uint8)
     return AffineMesh(dim, np.ascontiguousarray(corners, dtype=np.float64), nbr.astype(np.int32), types,
-                      tuple(bc_attr), np.asarray(attrs), ov, data)
+                      tuple(bc_attr), np.asarray(attrs), ov, data, codes)
 
 
 # ---------------------------------------------------------------------------
@@ -539,4 +598,6 @@ def reorder(mesh: AffineMesh, order: np.ndarray) -> AffineMesh:
     nbr = np.where(nbr >= 0, inv[np.maximum(nbr, 0)], nbr).astype(np.int32)
     attr = np.asarray(mesh.elem_attr)[order] if mesh.elem_attr is not None else None
     vids = mesh.vertex_ids[order] if mesh.vertex_ids is not None else None
-    return AffineMesh(mesh.dim, mesh.corners[order], nbr, mesh.bc_type, mesh.bc_attributes, attr, vids, mesh.bc_data)
+    codes = mesh.elem_orient[order] if mesh.elem_orient is not None else None
+    return AffineMesh(mesh.dim, mesh.corners[order], nbr, mesh.bc_type, mesh.bc_attributes, attr, vids, mesh.bc_data,
+                      codes)

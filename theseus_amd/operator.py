@@ -91,8 +91,10 @@ def _stream_ptr(stream):
     return C.c_void_p(stream.cuda_stream)
 
 
-def build_descriptor(rt: RunTime, mesh: BoxMesh):
-    """Fill a theseus_case_desc; returns (desc, keepalive arrays)."""
+def build_descriptor(rt: RunTime, mesh: BoxMesh, elem_orient=None):
+    """Fill a theseus_case_desc; returns (desc, keepalive arrays).
+    elem_orient: per-element symmetry codes of the storage frames (None:
+    every element stored aligned)."""
     if rt.dim != mesh.dim:
         raise ValueError(f"runTime dim {rt.dim} does not match mesh dim {mesh.dim}")
     jac = np.ascontiguousarray(mesh.elem_jac, dtype=np.float64)
@@ -127,17 +129,23 @@ def build_descriptor(rt: RunTime, mesh: BoxMesh):
     d.num_bc = int(bct.size)
     d.bc_type = bct.ctypes.data_as(C.POINTER(C.c_int)) if bct.size else C.POINTER(C.c_int)()
     d.bc_data = bcd.ctypes.data_as(C.POINTER(C.c_double)) if bct.size else C.POINTER(C.c_double)()
-    return d, (jac, adj, nbr, bct, bcd)
+    ori = None
+    if elem_orient is not None:
+        ori = np.ascontiguousarray(elem_orient, dtype=np.uint8)
+        if ori.shape != (mesh.num_elements,):
+            raise ValueError("elem_orient needs one code per element")
+        d.elem_orient = ori.ctypes.data_as(C.POINTER(C.c_ubyte))
+    return d, (jac, adj, nbr, bct, bcd, ori)
 
 
 class RHSOperator:
     """RHSOperator<PhysicsT> (include/RHSOperator.hpp:119-188) over the GPU engine."""
 
-    def __init__(self, runtime: RunTime, mesh: BoxMesh, lte_table=None):
+    def __init__(self, runtime: RunTime, mesh: BoxMesh, lte_table=None, elem_orient=None):
         self._lib = A.load()
         self.runtime = runtime
         self.mesh = mesh
-        desc, keep = build_descriptor(runtime, mesh)
+        desc, keep = build_descriptor(runtime, mesh, elem_orient)
         h = C.c_void_p()
         if lte_table is None:
             A.check(self._lib, self._lib.theseus_rhs_create(C.byref(desc), C.byref(h)))
@@ -277,10 +285,11 @@ def lte_descriptor(table):
     return d, arrays
 
 
-def MakeRHSOperator(runtime: dict | RunTime, mesh: BoxMesh, lte_table=None) -> RHSOperator:
+def MakeRHSOperator(runtime: dict | RunTime, mesh: BoxMesh, lte_table=None, elem_orient=None) -> RHSOperator:
     """SimFactory_impl.hpp:63-488: pick the operator from the runTime strings;
     with `lte_table` (a theseus_amd.lte.HelmholtzTable) the gas is the
-    tabulated LTE gas with the LTEEC flux (LTEGasModel.hpp:339)."""
+    tabulated LTE gas with the LTEEC flux (LTEGasModel.hpp:339);
+    `elem_orient` gives the elements' storage frames (theseus_case_desc)."""
     rt = runtime if isinstance(runtime, RunTime) else RunTime.from_json(runtime)
     cls = NSOperator if rt.flow_model == A.FLOW_NAVIER_STOKES else EulerOperator
-    return cls(rt, mesh, lte_table) if lte_table is not None else cls(rt, mesh)
+    return cls(rt, mesh, lte_table, elem_orient)
