@@ -301,7 +301,8 @@ THESEUS_API int theseus_rhs_step_ssprk3_cfl(theseus_rhs *h, double *u, theseus_c
  * kernel the handle enqueues is bracketed by hipEvents on its own stream, up
  * to `capacity` launches per kernel kind.  Kinds: 0 = the fused
  * volume+face kernel (Euler) / Navier-Stokes pass 2, 1 = Navier-Stokes
- * pass 1 (BR1 gradient + viscous flux).  enable(h, 0) disables and clears. */
+ * pass 1 (BR1 gradient + viscous flux) or, for the tabulated LTE gas, the
+ * per-node temperature recovery.  enable(h, 0) disables and clears. */
 THESEUS_API int theseus_rhs_enable_kernel_timing(theseus_rhs *h, int capacity);
 /* Synchronises the recorded events; returns the mean duration in ms and the
  * number of timed launches of one kernel kind. */

@@ -334,6 +334,8 @@ struct theseus_rhs {
   const LteKernels *lk = nullptr;  // tabulated LTE gas
   lte::View ltev;                  // its device arrays
   double *d_lte = nullptr;
+  double *d_lteT = nullptr;   // per node: recovered temperature of the stage state
+  int *d_ltecell = nullptr;   // per node: its table cell
   BasisArgs basis;
   GasC<double> gas;
   GasC<float> gas32;
@@ -435,6 +437,7 @@ static int enqueue_rhs(theseus_rhs *h, RHSArgs a, hipStream_t s) {
     timed(h, 1, s, [&] { h->sk->ns_grad[h->pm][h->has_boundary](a, nb, s); });
     timed(h, 0, s, [&] { h->fk->ns[h->pm][h->has_boundary](a, nb, s); });
   } else if (h->lk) {
+    timed(h, 1, s, [&] { h->lk->nodes(a, nb, s); });
     timed(h, 0, s, [&] { h->lk->rhs(a, nb, s); });
   } else {
     timed(h, 0, s, [&] { h->fk->euler[h->pm][h->has_boundary](a, nb, s); });
@@ -722,6 +725,12 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     h->ltev.e = h->d_lte + nx + ny;
     h->ltev.planes = h->d_lte + nx + 2 * ny;
     h->ltev.inv = h->d_lte + nx + 2 * ny + 6 * nx * ny;
+    if ((rc = upload<double>(&h->d_lteT, nullptr, h->Ntot)) || (rc = upload<int>(&h->d_ltecell, nullptr, h->Ntot))) {
+      theseus_rhs_destroy(h);
+      return rc;
+    }
+    h->ltev.node_T = h->d_lteT;
+    h->ltev.node_cell = h->d_ltecell;
   }
   if (d.elem_orient) {
     const std::vector<Sym> syms = symmetries(dim);
@@ -812,6 +821,8 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_Fv);
   hipFree(h->d_pref);
   hipFree(h->d_lte);
+  hipFree(h->d_lteT);
+  hipFree(h->d_ltecell);
   hipFree(h->d_stage[0]);
   hipFree(h->d_stage[1]);
   hipFree(h->d_rates);

@@ -29,6 +29,10 @@ struct View {
   const double *rho = nullptr, *T = nullptr, *e = nullptr;  // grids
   const double *planes = nullptr;                           // 6 * nx * ny: H, H_rho, H_T, H_rhoT, mu, lambda
   const double *inv = nullptr;                              // nx * ny temperature guesses
+  // per node of the state being evaluated (written by k_lte_nodes): the
+  // recovered temperature and the table cell i | j << 16 (-1 outside)
+  double *node_T = nullptr;
+  int *node_cell = nullptr;
 };
 
 // bisection: largest i with grid[i] <= value (count-2 at the top end); -1 outside
@@ -126,6 +130,7 @@ __device__ __forceinline__ double divided(const Cell &c, double xi1, double eta1
 struct Eval {
   Cell cell;
   double xi = 0, eta = 0, rho = 0, T = 0;
+  int ci = 0, cj = 0;  // the cell's grid indices
   bool valid = false;
   template <int DR = 0, int DT = 0>
   __device__ double H() const { return evaluate<DR, DT>(cell, xi, eta); }
@@ -157,7 +162,21 @@ __device__ __forceinline__ Eval at_cell(const View &v, double rho, double T, int
   r.eta = (T - v.T[j]) / dt;
   r.rho = rho;
   r.T = T;
+  r.ci = i;
+  r.cj = j;
   r.valid = true;
+  return r;
+}
+
+// Evaluation at (rho, T) whose cell is known (i | j << 16, from
+// find_cell of the two grids), or a flagged NaN evaluation for -1, as at()
+// gives for a query outside the table.
+__device__ __forceinline__ Eval at_packed(const View &v, double rho, double T, int cell) {
+  Eval r = at_cell(v, rho, T, cell < 0 ? 0 : (cell & 0xFFFF), cell < 0 ? 0 : (cell >> 16));
+  if (cell < 0) {
+    r.valid = false;
+    r.xi = r.eta = __builtin_nan("");
+  }
   return r;
 }
 
@@ -198,6 +217,11 @@ __device__ __forceinline__ double temperature_from_energy(const View &v, double 
   return __builtin_nan("");
 }
 
+// A scheduling fence: the compiler keeps the table evaluations before and
+// after it apart instead of interleaving all of them, which held so many
+// partial sums live that the pair flux spilled registers to scratch.
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 enum class Q { pressure, gibbs };
 template <Q P>
 __device__ __forceinline__ double value(const Eval &s) {
@@ -208,12 +232,18 @@ template <Q P>
 __device__ __forceinline__ double density_secant(const Eval &a, const Eval &b, double eta) {
   const Cell &c = a.cell;
   const double r1 = a.rho, r2 = b.rho;
-  const double mean_hr = 0.5 * (evaluate<1, 0>(c, a.xi, eta) + evaluate<1, 0>(c, b.xi, eta));
+  const double h1 = evaluate<1, 0>(c, a.xi, eta);
+  sched_fence();
+  const double h2 = evaluate<1, 0>(c, b.xi, eta);
+  sched_fence();
+  const double mean_hr = 0.5 * (h1 + h2);
   const double div_hr = divided<0, 1, 0>(c, a.xi, eta, b.xi, eta);
+  sched_fence();
   if constexpr (P == Q::pressure) {
     return 0.5 * (r1 * r1 + r2 * r2) * div_hr + (r1 + r2) * mean_hr;
   } else {
     const double div_h = divided<0>(c, a.xi, eta, b.xi, eta);
+    sched_fence();
     return div_h + 0.5 * (r1 + r2) * div_hr + mean_hr;
   }
 }
@@ -221,14 +251,24 @@ template <Q P>
 __device__ __forceinline__ double beta_secant(const Eval &a, const Eval &b, double xi, double rho) {
   const Cell &c = a.cell;
   const double mean_t = 0.5 * (a.T + b.T);
-  const double mean_hr = 0.5 * (evaluate<1, 0>(c, xi, a.eta) + evaluate<1, 0>(c, xi, b.eta));
+  const double h1 = evaluate<1, 0>(c, xi, a.eta);
+  sched_fence();
+  const double h2 = evaluate<1, 0>(c, xi, b.eta);
+  sched_fence();
+  const double mean_hr = 0.5 * (h1 + h2);
   const double div_hr = divided<1, 1, 0>(c, xi, a.eta, xi, b.eta);
+  sched_fence();
   const double hr_beta = mean_hr - mean_t * div_hr;
   if constexpr (P == Q::pressure) {
     return rho * rho * hr_beta;
   } else {
-    const double mean_h = 0.5 * (evaluate<0, 0>(c, xi, a.eta) + evaluate<0, 0>(c, xi, b.eta));
+    const double g1 = evaluate<0, 0>(c, xi, a.eta);
+    sched_fence();
+    const double g2 = evaluate<0, 0>(c, xi, b.eta);
+    sched_fence();
+    const double mean_h = 0.5 * (g1 + g2);
     const double div_h = divided<1>(c, xi, a.eta, xi, b.eta);
+    sched_fence();
     return mean_h - mean_t * div_h + rho * hr_beta;
   }
 }
@@ -250,11 +290,26 @@ __device__ __forceinline__ Gradient discrete_gradient(const View &v, const Eval 
   Gradient g;
   const bool same_cell = a.cell.H == b.cell.H;
   if (same_cell) {
-    if (dd) g.density = 0.5 * (density_secant<P>(a, b, a.eta) * b1 + density_secant<P>(a, b, b.eta) * b2);
-    if (bd) g.beta = 0.5 * (beta_secant<P>(a, b, a.xi, a.rho) + beta_secant<P>(a, b, b.xi, b.rho));
+    if (dd) {
+      const double s1 = density_secant<P>(a, b, a.eta);
+      sched_fence();
+      const double s2 = density_secant<P>(a, b, b.eta);
+      sched_fence();
+      g.density = 0.5 * (s1 * b1 + s2 * b2);
+    }
+    if (bd) {
+      const double s1 = beta_secant<P>(a, b, a.xi, a.rho);
+      sched_fence();
+      const double s2 = beta_secant<P>(a, b, b.xi, b.rho);
+      sched_fence();
+      g.beta = 0.5 * (s1 + s2);
+    }
   } else if (dd || bd) {
-    const double f12 = value<P>(at(v, a.rho, b.T));
-    const double f21 = value<P>(at(v, b.rho, a.T));
+    // the cross corners (rho_a, T_b), (rho_b, T_a) lie in the cells (i_a, j_b)
+    // and (i_b, j_a): the searches of at() would find exactly these
+    const bool ok = a.valid && b.valid;
+    const double f12 = value<P>(ok ? at_cell(v, a.rho, b.T, a.ci, b.cj) : at(v, a.rho, b.T));
+    const double f21 = value<P>(ok ? at_cell(v, b.rho, a.T, b.ci, a.cj) : at(v, b.rho, a.T));
     if (dd) g.density = 0.5 * ((f21 * b1 - f11 * b1) + (f22 * b2 - f12 * b2)) / dr;
     if (bd) g.beta = 0.5 * ((f12 * b2 - f11 * b1) + (f22 * b2 - f21 * b1)) / db;
   }
@@ -280,7 +335,9 @@ __device__ __forceinline__ Gradient discrete_gradient(const View &v, const Eval 
 __device__ __forceinline__ void thermodynamic_means(const View &v, const Eval &l, const Eval &r, double &rho_mean,
                                                     double &e_mean) {
   const Gradient p = discrete_gradient<Q::pressure>(v, l, r);
+  sched_fence();
   const Gradient g = discrete_gradient<Q::gibbs>(v, l, r);
+  sched_fence();
   rho_mean = p.density / g.density;
   e_mean = g.beta - p.beta / rho_mean;
 }
@@ -321,6 +378,28 @@ __device__ __forceinline__ void lteec_flux(const View &v, const double *q1, cons
     F[1 + d] = mass * vel + pm * dir[d] - diss * (q2[1 + d] - q1[1 + d]);
   }
   F[DIM + 1] = mass * (e_m + ck) + pw - diss * (q2[DIM + 1] - q1[DIM + 1]);
+}
+
+// Temperature and table cell of every node of a state (one lane per node):
+// the recovery of state_eval, done once per node and stage, for the RHS
+// kernel's staged nodes and its neighbours' face points.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_lte_nodes(const double *U, long long Ntot, View v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Ntot) return;
+  double q[DIM + 2];
+#pragma unroll
+  for (int c = 0; c < DIM + 2; ++c) q[c] = U[c * Ntot + t];
+  double m2 = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) m2 += q[1 + d] * q[1 + d];
+  const double rho = q[0];
+  const double e = (q[DIM + 1] - 0.5 * m2 / rho) / rho;
+  const double T = temperature_from_energy(v, rho, e);
+  const int i = find_cell(v.rho, v.nx, rho);
+  const int j = find_cell(v.T, v.ny, T);
+  v.node_T[t] = T;
+  v.node_cell[t] = (i < 0 || j < 0) ? -1 : (i | (j << 16));
 }
 
 // A point's evaluated state from its conservative vector (LTEHelmholtzGasEOS.hpp:60-77)

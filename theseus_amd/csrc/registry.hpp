@@ -41,6 +41,7 @@ struct TileKernels {
 // Euler with the tabulated LTE gas and the LTEEC flux
 struct LteKernels {
   RHSLaunch rhs = nullptr;
+  RHSLaunch nodes = nullptr;  // temperature recovery per node (k_lte_nodes), before rhs
   int tpb = 0, epb = 0;
 };
 
@@ -114,11 +115,18 @@ void launch_rhs_lte(const RHSArgs &a, int nb, hipStream_t s) {
   hipLaunchKernelGGL((k_rhs_lte<DIM, P>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
 }
 
+template <int DIM>
+void launch_lte_nodes(const RHSArgs &a, int, hipStream_t s) {
+  hipLaunchKernelGGL((lte::k_lte_nodes<DIM>), dim3((unsigned)((a.Ntot + 255) / 256)), dim3(256), 0, s, a.U, a.Ntot,
+                     a.lte);
+}
+
 template <int DIM, int P>
 struct LteRegistrar {
   LteRegistrar() {
     LteKernels k;
     k.rhs = &launch_rhs_lte<DIM, P>;
+    k.nodes = &launch_lte_nodes<DIM>;
     k.tpb = Shape<DIM, P>::TPB;
     k.epb = Shape<DIM, P>::EPB;
     register_lte(DIM, P, k);

@@ -2,10 +2,10 @@
 // (LTEECFlux.hpp:93-107; EulerOperator_impl.hpp:28-421 with the
 // LTEHelmholtzGas of LTEGasModel.hpp:339), FP64.  Same element-per-
 // workgroup structure as k_rhs: the element's temperatures are recovered
-// once per node (inverse-table guess + Newton) and staged in LDS with the
-// table cell and its coordinates, so the pair fluxes only evaluate the
-// interpolant; each GLL-line pair is evaluated once; the neighbours' face
-// traces are recovered where the face flux needs them.
+// once per node per stage by k_lte_nodes (inverse-table guess + Newton)
+// and staged in LDS with the table cell, so the pair fluxes only evaluate
+// the interpolant; each GLL-line pair is evaluated once; the neighbours' face
+// traces come with their recovered temperatures and cells.
 #pragma once
 
 #include "rhs_kernels.hpp"
@@ -64,24 +64,14 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   static_assert(Np * Np <= TPB, "one basis entry per lane");
   const double bD2 = a.basis.Dhat2[min(tid, Np * Np - 1)];
   meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
-  double qn[NFR][NQ];
-#pragma unroll
-  for (int r = 0; r < NFR; ++r) {
-    const int fi = min(tid + r * TPB, NFT - 1);
-    const int pe = fi / NFACE;
-    const int fl = fi - pe * NFACE;
-    const int f = fl / NF, t = fl - f * NF;
-    const int d = f >> 1, s = f & 1;
-    const size_t nbase = (size_t)max(fnb[r], 0) * ND + face_node<DIM, Np>(d, 1 - s, t);
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
-  }
+  const double Town = tv.node_T[ebase];
+  const int cown = tv.node_cell[ebase];
   if (tid < Np * Np) sD2[tid] = bD2;
 
-  // ---- stage: recover each node's temperature once
+  // ---- stage: each node's temperature and cell, recovered once per node by k_lte_nodes
   meta_store<DIM, EPB, TPB>(mr, sm);
   if (valid) {
-    const lte::Eval ev = lte::state_eval<DIM>(tv, qown);
+    const lte::Eval ev = lte::at_packed(tv, qown[0], Town, cown);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) sq[el][q][node] = qown[q];
     sT[el][node] = ev.T;
@@ -164,6 +154,23 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
 #pragma unroll
   for (int q = 0; q < NQ; ++q) du[q] = (-1.0 / Je) * rate[q];
 
+  // the neighbours' face traces (state, temperature, cell) and the RK base
+  // state, loaded once the pair fluxes no longer hold registers
+  double qn[NFR][NQ], Tn[NFR];
+  int cn[NFR];
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int fi = min(tid + r * TPB, NFT - 1);
+    const int pe = fi / NFACE;
+    const int fl = fi - pe * NFACE;
+    const int f = fl / NF, t = fl - f * NF;
+    const int d = f >> 1, s = f & 1;
+    const size_t nbase = (size_t)max(fnb[r], 0) * ND + face_node<DIM, Np>(d, 1 - s, t);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
+    Tn[r] = tv.node_T[nbase];
+    cn[r] = tv.node_cell[nbase];
+  }
   double xb[NQ];
   {
     const double *Xp = a.mode == 2 ? a.X : a.U;
@@ -195,7 +202,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
         lte::Eval st = so;
         double pt = po, ct = co;
         if (interior) {
-          st = lte::state_eval<DIM>(tv, qt);
+          st = lte::at_packed(tv, qt[0], Tn[r], cn[r]);
           pt = st.pressure();
           ct = sqrt(st.c2());
         }
