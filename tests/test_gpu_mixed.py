@@ -105,3 +105,43 @@ def test_mixed_tgv_rhs_and_steps_per_field(oracle):
     # field is only discretisation error, so its relative error is not a
     # precision measure; momenta and energy are bounded here
     assert max(errs[1:]) <= 1e-4, errs
+
+
+def test_mixed_tgv_full_size_state_and_evolved_rhs():
+    """The bench workload itself (32^3 hexes, p=3, Ma 0.1, Re 1600): the
+    mixed state after its full 200 SSP-RK3 steps against the FP64 engine's
+    (itself within 1e-10 of the oracle over 100 steps), per field <= 1e-5;
+    and the RHS of both on the evolved FP64 state.  In this nearly
+    incompressible flow the density and energy RHS are small differences of
+    O(1) fluxes (the energy flux carries rho E + p ~ 250), which FP32 resolves
+    only to a few percent of their norms (measured 1.6e-2 and 3.3e-2); the
+    momenta to ~1e-4 (9.4e-5 and 2.2e-5).  The bounds hold those levels with
+    a factor 3-5 of margin so that a regression shows."""
+    import torch
+
+    from theseus_amd.cases import c2_tgv3d
+    from theseus_amd.operator import MakeRHSOperator
+
+    _say("case mixed tgv 32^3 p=3: 200 steps vs FP64, RHS on the evolved state")
+    rt, mesh, U0 = c2_tgv3d()
+    op = MakeRHSOperator(rt, mesh)
+    opm = MakeRHSOperator(_mixed(rt), mesh)
+    u = torch.from_numpy(U0).cuda()
+    dt = 0.5 / op.EstimateStability(u).TotalRate()
+    um = u.clone()
+    for _ in range(200):
+        op.Step(u, 0.0, dt)
+        opm.Step(um, 0.0, dt)
+    r = torch.empty_like(u)
+    rm = torch.empty_like(u)
+    op.Mult(u, r)
+    opm.Mult(u, rm)
+    torch.cuda.synchronize()
+    serr = per_field_rel(um.cpu().numpy(), u.cpu().numpy(), 5)
+    rerr = per_field_rel(rm.cpu().numpy(), r.cpu().numpy(), 5)
+    op.close()
+    opm.close()
+    _say(f"  state per-field rel L2 after 200 steps {serr}; evolved RHS {rerr}")
+    assert max(serr) <= 1e-5, serr
+    assert rerr[0] <= 5e-2 and rerr[4] <= 1e-1, rerr
+    assert max(rerr[1:4]) <= 5e-4, rerr
