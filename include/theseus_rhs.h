@@ -348,8 +348,14 @@ THESEUS_API const char *theseus_last_error(void);
  * capacity is set by an HBM budget; admission fails with
  * THESEUS_ERR_CAPACITY when no free range is large enough, and a released
  * case's slots are reused.  The SSP-RK3 step of all admitted cases is one
- * hipGraph, captured at the first step and replayed; each case advances with
- * its own dt, fixed at admission from its stability estimate.  The tile
+ * hipGraph over the occupied slots' high-water mark, rounded up to a ladder
+ * of ranges (1024 slots, then steps of 25 %); the graph of each range is
+ * captured at its first step and kept, so admissions and releases never
+ * re-capture a range seen before.  Each case advances with its own dt,
+ * computed on the device at admission from its stability estimate.
+ * Admission, step and release only enqueue work on `stream` (host tables go
+ * through pinned staging); the calls on one pool must be ordered, e.g. all
+ * made on one stream.  The tile
  * shape (elements per workgroup, block order) is picked at create time by
  * timing the candidates on a synthetic case (tile/xcd_remap = -1), or fixed. */
 typedef struct theseus_pool theseus_pool; /* opaque */
@@ -372,7 +378,12 @@ THESEUS_API int theseus_pool_info(const theseus_pool *p, int *capacity_elements,
  * geometry/connectivity arrays and boundary table (at most
  * THESEUS_POOL_MAX_BC boundaries) are host pointers as for
  * theseus_rhs_create), with initial state u0 (its L-vector, host or device
- * memory) and dt = cfl / total stability rate of u0.  Synchronises `stream`. */
+ * memory) and dt = cfl / total stability rate of u0.  Asynchronous on
+ * `stream` when `dt` is NULL and u0 is device or pinned memory (pageable u0
+ * is copied as hipMemcpyAsync copies it, which may wait for the stream); a
+ * case whose rate is not positive and finite then stays in place with dt = 0.
+ * With `dt` non-NULL the call waits for the estimate, returns it, and
+ * refuses such a case with THESEUS_ERR_NONFINITE. */
 #define THESEUS_POOL_MAX_BC 16
 THESEUS_API int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double *u0, double cfl,
                                    int *case_id, double *dt, void *stream);
@@ -384,7 +395,7 @@ THESEUS_API int theseus_pool_read(theseus_pool *p, int case_id, double *u, void 
  * as theseus_rhs_integral_measures computes them.  Synchronises `stream`. */
 THESEUS_API int theseus_pool_integral_measures(theseus_pool *p, int case_id, theseus_integral_measures *out,
                                                void *stream);
-/* Free a case's slots for later admissions. */
+/* Free a case's slots for later admissions (asynchronous on `stream`). */
 THESEUS_API int theseus_pool_release(theseus_pool *p, int case_id, void *stream);
 
 #ifdef __cplusplus

@@ -160,3 +160,49 @@ def test_server_drains_a_poisson_stream():
     assert res["cases_done"] == 24
     assert math.isfinite(res["tts_p50_s"]) and res["tts_p99_s"] >= res["tts_p50_s"] > 0
     assert all(r.result is not None and np.isfinite(r.result).all() for r in reqs)
+
+
+def test_async_admission_across_launch_ranges():
+    """Admissions that only enqueue (no dt requested) and releases, with no
+    host synchronisation between them and the steps, move the launch range
+    over 2048 -> 4096 slots and back through kept graphs; the case in the
+    middle ends bitwise as it does alone, where its dt came back to the host."""
+    import torch
+
+    from theseus_amd.serving import CaseFactory, Pool, euler_runtime
+
+    _say("case pool asynchronous admission across launch ranges p=2")
+    f = CaseFactory()
+    rt = euler_runtime(2, A.PRECISION_FP64)
+    per = 3 * 5 * 27 * 8 + 8 * 10 + 4 * 7 + 8
+    target = (4, 2.0, (0.3, 1.1, 2.0))
+    pool = Pool(rt, 4608 * per + per // 2, 8, tile=0, xcd_remap=1)
+    assert pool.info()["capacity_elements"] == 4608
+    alone, _ = _run_pool_case(pool, f, [target], 0, 6)
+    pool.close()
+
+    pool = Pool(rt, 4608 * per + per // 2, 8, tile=0, xcd_remap=1)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        big = f.state(rt, 12, 1.0, (0.0, 0.0, 0.0))   # 1728 elements: range 2048
+        tgt = f.state(rt, *target)                    # 64 elements after it
+        big2 = f.state(rt, 13, 0.8, (1.0, 0.0, 0.0))  # 2197 elements
+        a, dt_a = pool.admit(f.mesh(12), big, 0.5, s, want_dt=False)
+        b, _ = pool.admit(f.mesh(4), tgt, 0.5, s, want_dt=False)
+        assert dt_a is None
+        for _ in range(3):
+            pool.step(s)
+        pool.release(a, s)                            # range stays 2048 (the target ends at 1792)
+        c, _ = pool.admit(f.mesh(13), big2, 0.5, s, want_dt=False)  # after the target: range 4096
+        for _ in range(2):
+            pool.step(s)
+        pool.release(c, s)                            # back to 2048, its graph reused
+        pool.step(s)
+    s.synchronize()
+    out = torch.empty(5 * 4 ** 3 * 27, dtype=torch.float64, device="cuda")
+    pool.read(b, out, s)
+    mixed = out.cpu().numpy()
+    info = pool.info()
+    pool.close()
+    assert info["active_cases"] == 1 and info["free_elements"] == 4608 - 64, info
+    assert np.array_equal(alone, mixed), np.abs(alone - mixed).max()

@@ -938,6 +938,17 @@ __global__ void k_clock(theseus_clock *c, const unsigned long long *rates) {
   if (dt_real > 0.0) c->steps += 1;
 }
 
+// A pooled case's time step from its stability rates: cfl / total rate, or 0
+// (the case does not advance) when the rate is not positive and finite.
+__global__ void k_case_dt(const unsigned long long *rates, double cfl, double *dt) {
+  if (threadIdx.x != 0) return;
+  const double adv = __longlong_as_double((long long)rates[0]);
+  const double surf = __longlong_as_double((long long)rates[1]);
+  const double diff = __longlong_as_double((long long)rates[2]);
+  const double rate = fmax(adv, surf) + diff;
+  dt[0] = (rate > 0.0 && isfinite(rate)) ? cfl / rate : 0.0;
+}
+
 int theseus_rhs_step_ssprk3(theseus_rhs *h, double *u, double t, double dt, void *stream) {
   (void)t;  // the operator is autonomous (no time-dependent sources or boundary data)
   if (!h || !u) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle or vector");
@@ -1276,6 +1287,12 @@ int theseus_rhs_get_alpha(const theseus_rhs *h, double *alpha, void *stream) {
 // ---------------------------------------------------------------------------
 namespace {
 
+struct PoolStaging {
+  char *host = nullptr;  // pinned
+  size_t bytes = 0;
+  hipEvent_t done = nullptr;  // recorded after the copies out of it
+};
+
 struct PoolCase {
   int start = -1, ne = 0;
   bool active = false;
@@ -1320,9 +1337,14 @@ struct theseus_pool {
   double *d_meas = nullptr;  // integral measures: partials of the largest case, then the results
   std::vector<PoolCase> cases;
   std::vector<unsigned char> slot_used;  // per element slot
+  PoolStaging staging[2];
+  int next_staging = 0;
   hipStream_t capture_stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // captured SSP-RK3 steps, one per (launch range, kernel choice): a step
+  // replays the one for the current range, capturing it on first use; none
+  // is destroyed before the pool, so a change of range needs no
+  // synchronisation with launches still in flight
+  std::map<std::pair<int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 
   int epb() const { return (tk && cd.flow_model == THESEUS_FLOW_EULER) ? tk->epb[tile] : sk->epb; }
   int nblocks() const { return (active + epb() - 1) / epb(); }
@@ -1404,30 +1426,60 @@ static size_t pool_bytes_per_element(const theseus_pool_desc &d, int ndof, int n
 
 // Free slots point at themselves, so the kernels' unconditional neighbour
 // loads stay inside the arrays; their owner is -1 (skipped).
+__global__ void __launch_bounds__(256) k_pool_reset(int *nbr, int *owner, double *J, double *adj, int start, int n,
+                                                    int dim) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int e = start + i;
+  for (int f = 0; f < 2 * dim; ++f) nbr[(size_t)e * 2 * dim + f] = e;
+  owner[e] = -1;
+  J[e] = 1.0;
+  for (int r = 0; r < dim; ++r)
+    for (int c = 0; c < dim; ++c) adj[(size_t)e * dim * dim + r * dim + c] = r == c ? 1.0 : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_fill_i32(int *p, int n, int v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 static int pool_reset_slots(theseus_pool *p, int start, int n, hipStream_t s) {
-  const int dim = p->cd.dim;
-  std::vector<int> nbr((size_t)n * 2 * dim), owner(n, -1);
-  std::vector<double> J(n, 1.0), adj((size_t)n * dim * dim, 0.0);
-  for (int e = 0; e < n; ++e) {
-    for (int f = 0; f < 2 * dim; ++f) nbr[(size_t)e * 2 * dim + f] = start + e;
-    for (int dd = 0; dd < dim; ++dd) adj[(size_t)e * dim * dim + dd * dim + dd] = 1.0;
+  if (n <= 0) return THESEUS_OK;
+  hipLaunchKernelGGL(k_pool_reset, dim3((n + 255) / 256), dim3(256), 0, s, p->d_nbr, p->d_elem_case, p->d_J, p->d_adj,
+                     start, n, p->cd.dim);
+  HIP_TRY(hipGetLastError());
+  return THESEUS_OK;
+}
+
+// Pinned staging for the host tables of an admission: copies from it are
+// asynchronous, where copies from pageable memory make the host wait for the
+// stream.  Two buffers alternate; an admission waits only for the copies of
+// the admission before the last one (long done in a serving loop).  A buffer
+// grows when a larger case arrives; freeing the old one synchronises, once.
+static int pool_stage(theseus_pool *p, size_t bytes, char **host, hipEvent_t *done) {
+  PoolStaging &b = p->staging[p->next_staging];
+  p->next_staging ^= 1;
+  if (!b.done) HIP_TRY(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+  else HIP_TRY(hipEventSynchronize(b.done));
+  if (bytes > b.bytes) {
+    if (b.host) HIP_TRY(hipHostFree(b.host));
+    b.host = nullptr;
+    b.bytes = 0;
+    const size_t want = std::max(bytes, (size_t)1 << 20);
+    HIP_TRY(hipHostMalloc((void **)&b.host, want, hipHostMallocDefault));
+    b.bytes = want;
   }
-  HIP_TRY(hipMemcpyAsync(p->d_nbr + (size_t)start * 2 * dim, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(p->d_elem_case + start, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(p->d_J + start, J.data(), J.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(p->d_adj + (size_t)start * dim * dim, adj.data(), adj.size() * sizeof(double),
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  *host = b.host;
+  *done = b.done;
   return THESEUS_OK;
 }
 
 // The step's launch range follows the occupied slots: their high-water mark
-// rounded up to 1024 slots (first-fit keeps it low); the kernels with the
-// boundary conditions run while an admitted case has boundary faces.  A new
-// range or kernel choice drops the captured graph, which the next step
-// captures again; `s` is synchronised first so that no launch of the old
-// graph is still in flight.
-static int pool_update_active(theseus_pool *p, hipStream_t s) {
+// rounded up to the next of a geometric ladder of ranges (1024 slots, then
+// steps of 25 %), so that first-fit admissions and releases move between a
+// few ranges whose captured steps are kept; the kernels with the boundary
+// conditions run while an admitted case has boundary faces.
+static int pool_update_active(theseus_pool *p) {
   int hwm = 0, bnd = 0;
   for (const PoolCase &c : p->cases)
     if (c.active) {
@@ -1435,16 +1487,11 @@ static int pool_update_active(theseus_pool *p, hipStream_t s) {
       bnd |= c.boundary ? 1 : 0;
     }
   const int gran = 1024;
-  const int next = std::min(p->cap, (hwm + gran - 1) / gran * gran);
-  if (next == p->active && bnd == p->bnd) return THESEUS_OK;
-  HIP_TRY(hipStreamSynchronize(s));
-  if (p->exec) {
-    HIP_TRY(hipGraphExecDestroy(p->exec));
-    p->exec = nullptr;
-  }
-  if (p->graph) {
-    HIP_TRY(hipGraphDestroy(p->graph));
-    p->graph = nullptr;
+  int next = 0;
+  if (hwm > 0) {
+    next = gran;
+    while (next < hwm) next = (int)std::min<long long>(p->cap, ((long long)next * 5 / 4 + gran - 1) / gran * gran);
+    next = std::min(next, p->cap);
   }
   p->active = next;
   p->bnd = bnd;
@@ -1661,15 +1708,32 @@ int theseus_pool_create(const theseus_pool_desc *desc, theseus_pool **out) {
       tune_cache()[key] = {p->tile, p->xcd};
     }
   }
+  // the slot reset runs on the pool's own stream: finish it before any call on another
+  {
+    const hipError_t se = hipStreamSynchronize(s);
+    if (se != hipSuccess) {
+      theseus_pool_destroy(p);
+      return fail(THESEUS_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
+    }
+  }
   *out = p;
   return THESEUS_OK;
 }
 
 int theseus_pool_destroy(theseus_pool *p) {
   if (!p) return THESEUS_OK;
-  if (p->exec) hipGraphExecDestroy(p->exec);
-  if (p->graph) hipGraphDestroy(p->graph);
+  for (auto &kv : p->graphs) {
+    hipGraphExecDestroy(kv.second.second);
+    hipGraphDestroy(kv.second.first);
+  }
   if (p->capture_stream) hipStreamDestroy(p->capture_stream);
+  for (PoolStaging &b : p->staging) {
+    if (b.done) {
+      hipEventSynchronize(b.done);
+      hipEventDestroy(b.done);
+    }
+    if (b.host) hipHostFree(b.host);
+  }
   hipFree(p->d_J);
   hipFree(p->d_adj);
   hipFree(p->d_nbr);
@@ -1742,25 +1806,34 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
   const int start = pool_find_range(p, ne);
   if (start < 0) return fail(THESEUS_ERR_CAPACITY, "no free range of " + std::to_string(ne) + " elements");
   hipStream_t s = (hipStream_t)stream;
-  // geometry and connectivity at the slot offset; boundary codes point into
-  // the case's rows of the pool's boundary table
-  std::vector<int> nbr((size_t)ne * 2 * dim), owner(ne, cid);
+  // geometry and connectivity at the slot offset, staged in pinned memory;
+  // boundary codes point into the case's rows of the pool's boundary table
+  const size_t nJ = ne, nA = (size_t)ne * dim * dim, nB = (size_t)d.num_bc * THESEUS_BC_DATA_STRIDE;
+  const size_t nN = (size_t)ne * 2 * dim, nT = d.num_bc;
+  char *host = nullptr;
+  hipEvent_t staged = nullptr;
+  if ((rc = pool_stage(p, (nJ + nA + nB) * sizeof(double) + (nN + nT) * sizeof(int), &host, &staged))) return rc;
+  double *hJ = reinterpret_cast<double *>(host), *hA = hJ + nJ, *hB = hA + nA;
+  int *hN = reinterpret_cast<int *>(hB + nB), *hT = hN + nN;
+  std::copy(d.elem_jac, d.elem_jac + nJ, hJ);
+  std::copy(d.elem_adj, d.elem_adj + nA, hA);
+  std::fill(hB, hB + nB, 0.0);
+  if (d.bc_data) std::copy(d.bc_data, d.bc_data + nB, hB);
+  if (nT) std::copy(d.bc_type, d.bc_type + nT, hT);
   const int boff = cid * THESEUS_POOL_MAX_BC;
-  for (size_t f = 0; f < nbr.size(); ++f) {
+  for (size_t f = 0; f < nN; ++f) {
     const int v = d.face_neighbor[f];
-    nbr[f] = v >= 0 ? v + start : -(boff - (v + 1) + 1);
+    hN[f] = v >= 0 ? v + start : -(boff - (v + 1) + 1);
   }
   if (d.num_bc > 0) {
-    std::vector<double> bdat((size_t)d.num_bc * THESEUS_BC_DATA_STRIDE, 0.0);
-    if (d.bc_data) std::copy(d.bc_data, d.bc_data + bdat.size(), bdat.begin());
-    HIP_TRY(hipMemcpyAsync(p->d_bc_type + boff, d.bc_type, d.num_bc * sizeof(int), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(p->d_bc_data + (size_t)boff * THESEUS_BC_DATA_STRIDE, bdat.data(), bdat.size() * sizeof(double),
+    HIP_TRY(hipMemcpyAsync(p->d_bc_type + boff, hT, nT * sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(p->d_bc_data + (size_t)boff * THESEUS_BC_DATA_STRIDE, hB, nB * sizeof(double),
                            hipMemcpyHostToDevice, s));
   }
-  HIP_TRY(hipMemcpyAsync(p->d_J + start, d.elem_jac, ne * sizeof(double), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(p->d_adj + (size_t)start * dim * dim, d.elem_adj, (size_t)ne * dim * dim * sizeof(double),
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(p->d_nbr + (size_t)start * 2 * dim, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_J + start, hJ, nJ * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_adj + (size_t)start * dim * dim, hA, nA * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_nbr + (size_t)start * 2 * dim, hN, nN * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(staged, s));
   const long long nd = (long long)ne * p->ndof, off = (long long)start * p->ndof;
   for (int q = 0; q < p->neq; ++q)
     HIP_TRY(hipMemcpyAsync(p->d_U + q * p->Ntot + off, u0 + q * nd, nd * sizeof(double), hipMemcpyDefault, s));
@@ -1787,25 +1860,26 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
   p->sk->stability(a, (ne + p->sk->epb - 1) / p->sk->epb, s);
   if (dim == 2) hipLaunchKernelGGL(k_pref<2>, dim3(1), dim3(64), 0, s, p->d_U + off, p->Ntot, p->gas, p->d_case_pref + cid);
   else hipLaunchKernelGGL(k_pref<3>, dim3(1), dim3(64), 0, s, p->d_U + off, p->Ntot, p->gas, p->d_case_pref + cid);
+  hipLaunchKernelGGL(k_case_dt, dim3(1), dim3(64), 0, s, p->d_rates, cfl, p->d_case_dt + cid);
+  hipLaunchKernelGGL(k_fill_i32, dim3((ne + 255) / 256), dim3(256), 0, s, p->d_elem_case + start, ne, cid);
   HIP_TRY(hipGetLastError());
-  unsigned long long r[3];
-  HIP_TRY(hipMemcpyAsync(r, p->d_rates, sizeof r, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  double v[3];
-  std::memcpy(v, r, sizeof v);
-  const double total = std::max(v[0], v[1]) + v[2];
-  if (!(total > 0) || !std::isfinite(total)) return fail(THESEUS_ERR_NONFINITE, "stability estimate of the case is not positive and finite");
-  const double dt = cfl / total;
-  HIP_TRY(hipMemcpyAsync(p->d_case_dt + cid, &dt, sizeof dt, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(p->d_elem_case + start, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  double dt = 0.0;
+  if (dt_out) {
+    // the caller asked for dt: wait for it, and refuse a case whose estimate is not usable
+    HIP_TRY(hipMemcpyAsync(&dt, p->d_case_dt + cid, sizeof dt, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!(dt > 0)) {
+      pool_reset_slots(p, start, ne, s);
+      return fail(THESEUS_ERR_NONFINITE, "stability estimate of the case is not positive and finite");
+    }
+  }
   for (int e = 0; e < ne; ++e) p->slot_used[start + e] = 1;
   p->cases[cid].start = start;
   p->cases[cid].ne = ne;
   p->cases[cid].active = true;
   p->cases[cid].boundary = has_boundary;
   {
-    const int rc = pool_update_active(p, s);
+    const int rc = pool_update_active(p);
     if (rc) return rc;
   }
   if (case_id) *case_id = cid;
@@ -1816,14 +1890,23 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
 int theseus_pool_step(theseus_pool *p, void *stream) {
   if (!p) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null pool");
   if (p->active == 0) return THESEUS_OK;  // no case admitted
-  if (!p->exec) {
+  auto key = std::make_pair(p->active, p->bnd);
+  auto it = p->graphs.find(key);
+  if (it == p->graphs.end()) {
     hipStream_t cs = p->capture_stream;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
     HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     p->enqueue_step(cs);
-    HIP_TRY(hipStreamEndCapture(cs, &p->graph));
-    HIP_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    HIP_TRY(hipStreamEndCapture(cs, &g));
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      hipGraphDestroy(g);
+      return fail(THESEUS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    it = p->graphs.emplace(key, std::make_pair(g, x)).first;
   }
-  HIP_TRY(hipGraphLaunch(p->exec, (hipStream_t)stream));
+  HIP_TRY(hipGraphLaunch(it->second.second, (hipStream_t)stream));
   return THESEUS_OK;
 }
 
@@ -1862,7 +1945,7 @@ int theseus_pool_release(theseus_pool *p, int case_id, void *stream) {
   if (rc) return rc;
   for (int e = 0; e < c.ne; ++e) p->slot_used[c.start + e] = 0;
   c = PoolCase();
-  return pool_update_active(p, (hipStream_t)stream);
+  return pool_update_active(p);
 }
 
 }  // extern "C"

@@ -60,19 +60,21 @@ class Pool:
         keys = ("capacity_elements", "free_elements", "largest_free_range", "active_cases", "tile", "xcd_remap")
         return {k: x.value for k, x in zip(keys, v)}
 
-    def admit(self, mesh, u0, cfl: float, stream=None):
+    def admit(self, mesh, u0, cfl: float, stream=None, want_dt: bool = True):
         """Admit one case; returns (case_id, dt).  Raises CapacityError when
-        the pool has no free range or case slot for it."""
+        the pool has no free range or case slot for it.  With want_dt False
+        the admission is only enqueued on `stream` (dt is computed on the
+        device) and dt is returned as None."""
         desc, keep = build_descriptor(self.rt, mesh)
         cid, dt = C.c_int(), C.c_double()
         ptr = u0.data_ptr() if hasattr(u0, "data_ptr") else u0.ctypes.data
         rc = self._lib.theseus_pool_admit(self._h, C.byref(desc), C.c_void_p(ptr), C.c_double(cfl), C.byref(cid),
-                                          C.byref(dt), _stream(stream))
+                                          C.byref(dt) if want_dt else None, _stream(stream))
         del keep
         if rc == A.ERR_CAPACITY:
             raise CapacityError(rc, self._lib.theseus_last_error().decode())
         A.check(self._lib, rc)
-        return cid.value, dt.value
+        return cid.value, (dt.value if want_dt else None)
 
     def step(self, stream=None) -> None:
         A.check(self._lib, self._lib.theseus_pool_step(self._h, _stream(stream)))
@@ -221,18 +223,27 @@ def percentile(values, q):
 
 
 class Server:
-    """Continuous batching over one pool per polynomial order."""
+    """Continuous batching over one pool per polynomial order.
+
+    The host loop never blocks on the device while work is in flight: each
+    pool's steps go out in batches of at most ``max_batch_steps`` (never past
+    a case's last step), each batch followed by an event; the loop polls the
+    events, admits arrivals the moment a range is free (admission is only
+    enqueued on the pool's stream, behind the batches already there), and
+    keeps up to two batches queued per pool so the device does not wait for
+    the host between them.  A case joins the batches enqueued after its
+    admission; a finished case is read and released when its last batch's
+    event has completed."""
 
     def __init__(self, orders=(2, 3, 4), hbm_budget_bytes: int = 64 << 30, max_cases: int = 256,
                  precision=A.PRECISION_MIXED, cfl: float = 0.5, keep_results: bool = False, tile: int = -1,
-                 xcd_remap: int = -1, max_batch_steps: int = 8):
+                 xcd_remap: int = -1, max_batch_steps: int = 8, max_inflight: int = 2):
         import torch
 
         self.torch = torch
         self.cfl = cfl
-        # steps enqueued per pool between host synchronisations (never past a
-        # case's last step): fewer round trips, admissions wait at most this long
         self.max_batch_steps = max(1, int(max_batch_steps))
+        self.max_inflight = max(1, int(max_inflight))
         self.keep_results = keep_results
         budget = int(hbm_budget_bytes) // len(orders)
         self.pools = {p: Pool(euler_runtime(p, precision), budget, max_cases, tile, xcd_remap) for p in orders}
@@ -247,12 +258,41 @@ class Server:
         torch = self.torch
         waiting = deque(sorted(requests, key=lambda r: r.arrival_s))
         queued: list[Request] = []
-        active = {p: {} for p in self.pools}  # order -> {case_id: (request, steps_done)}
+        # order -> {case_id: [request, steps completed, steps enqueued]}
+        active = {p: {} for p in self.pools}
+        inflight = {p: deque() for p in self.pools}  # order -> (event, steps, case ids) per batch
         done: list[Request] = []
         states = {}
         t0 = time.perf_counter()
         dof_rhs = 0.0
         while (waiting or queued or any(active[p] for p in active)) and time.perf_counter() - t0 < time_limit_s:
+            progressed = False
+            # completed batches: count their steps, hand back finished cases
+            for p, q in inflight.items():
+                while q and q[0][0].query():
+                    _, k, cids = q.popleft()
+                    progressed = True
+                    cases = active[p]
+                    for cid in cids:
+                        rec = cases[cid]
+                        rec[1] += k
+                        dof_rhs += 3 * k * rec[0].dofs
+                    for cid in [c for c in cids if cases[c][1] >= cases[c][0].steps]:
+                        r = cases.pop(cid)[0]
+                        if self.keep_results:
+                            out = torch.empty(r.dofs, dtype=torch.float64, device="cuda")
+                            self.pools[p].read(cid, out, self.streams[p])
+                            r.result = out.cpu().numpy()
+                        else:
+                            out = states.get((p, r.dofs))
+                            if out is None:
+                                out = states[(p, r.dofs)] = torch.empty(r.dofs, dtype=torch.float64, device="cuda")
+                            self.pools[p].read(cid, out, self.streams[p])
+                        self.pools[p].release(cid, self.streams[p])
+                        r.done_s = time.perf_counter() - t0
+                        done.append(r)
+                        if log:
+                            log(f"case {r.rid} p={r.order} n={r.n} done in {r.done_s - r.arrival_s:.3f} s")
             now = time.perf_counter() - t0
             while waiting and waiting[0].arrival_s <= now:
                 queued.append(waiting.popleft())
@@ -263,53 +303,38 @@ class Server:
                 with torch.cuda.stream(self.streams[r.order]):
                     u0 = self.factory.state(pool.rt, r.n, r.amplitude, r.phases)
                 try:
-                    cid, dt = pool.admit(self.factory.mesh(r.n), u0, self.cfl, self.streams[r.order])
+                    cid, _ = pool.admit(self.factory.mesh(r.n), u0, self.cfl, self.streams[r.order], want_dt=False)
                 except CapacityError:
                     still.append(r)
                     continue
                 r.admitted_s = time.perf_counter() - t0
-                r.dt = dt
-                active[r.order][cid] = [r, 0]
+                active[r.order][cid] = [r, 0, 0]
+                progressed = True
             queued = still
-            if not any(active[p] for p in active):
-                if waiting:
-                    time.sleep(max(0.0, min(waiting[0].arrival_s - (time.perf_counter() - t0), 0.01)))
-                continue
-            batch = {}
+            # keep each pool's queue of batches topped up
             for p, cases in active.items():
-                if cases:
-                    k = min(self.max_batch_steps, min(rec[0].steps - rec[1] for rec in cases.values()))
-                    for _ in range(k):
-                        self.pools[p].step(self.streams[p])
-                    batch[p] = k
-            for p, cases in active.items():
-                if not cases:
+                if not cases or len(inflight[p]) >= self.max_inflight:
                     continue
-                self.streams[p].synchronize()
-                finished = []
-                k = batch[p]
-                for cid, rec in cases.items():
-                    rec[1] += k
-                    dof_rhs += 3 * k * rec[0].dofs
-                    if rec[1] >= rec[0].steps:
-                        finished.append(cid)
-                for cid in finished:
-                    r = cases.pop(cid)[0]
-                    if self.keep_results:
-                        out = torch.empty(r.dofs, dtype=torch.float64, device="cuda")
-                        self.pools[p].read(cid, out, self.streams[p])
-                        r.result = out.cpu().numpy()
-                    else:
-                        out = states.get((p, r.dofs))
-                        if out is None:
-                            out = states[(p, r.dofs)] = torch.empty(r.dofs, dtype=torch.float64, device="cuda")
-                        self.pools[p].read(cid, out, self.streams[p])
-                    self.pools[p].release(cid, self.streams[p])
-                    r.done_s = time.perf_counter() - t0
-                    done.append(r)
-                    if log:
-                        log(f"case {r.rid} p={r.order} n={r.n} done in {r.done_s - r.arrival_s:.3f} s")
+                k = min(self.max_batch_steps, min(rec[0].steps - rec[2] for rec in cases.values()))
+                if k <= 0:
+                    continue  # a case ends with the batches in flight: wait for them
+                st = self.streams[p]
+                for _ in range(k):
+                    self.pools[p].step(st)
+                for rec in cases.values():
+                    rec[2] += k
+                ev = torch.cuda.Event()
+                ev.record(st)
+                inflight[p].append((ev, k, list(cases.keys())))
+                progressed = True
+            if not progressed:
+                if not any(active[p] for p in active) and waiting:
+                    time.sleep(max(0.0, min(waiting[0].arrival_s - (time.perf_counter() - t0), 0.01)))
+                else:
+                    time.sleep(2e-4)
         makespan = time.perf_counter() - t0
+        for p, st in self.streams.items():
+            st.synchronize()
         tts = [r.done_s - r.arrival_s for r in done]
         return {
             "cases_done": len(done),
