@@ -398,6 +398,22 @@ THESEUS_API int theseus_pool_integral_measures(theseus_pool *p, int case_id, the
 /* Free a case's slots for later admissions (asynchronous on `stream`). */
 THESEUS_API int theseus_pool_release(theseus_pool *p, int case_id, void *stream);
 
+/* Pools of different polynomial order in one launch: up to three pools of
+ * orders 2, 3 and 4 (one each; 3-D Euler, one flux pairing -- compiled for
+ * Ranocha + Lax-Friedrichs -- and one precision) are advanced by one kernel
+ * launch per SSP-RK3 stage, the workgroups of each order running that
+ * order's code (128 lanes: four elements per workgroup at p = 2, two at
+ * p = 3, one at p = 4).  Each pool keeps its element array, admission and
+ * release; theseus_pool_group_step replaces theseus_pool_step for the
+ * member pools (a captured graph per combination of their launch ranges, kept
+ * until the group is destroyed), on the stream their other calls use.  A
+ * case's result is that of theseus_pool_step.  The group does not own the
+ * pools: destroy it first. */
+typedef struct theseus_pool_group theseus_pool_group; /* opaque */
+THESEUS_API int theseus_pool_group_create(theseus_pool *const *pools, int n, theseus_pool_group **out);
+THESEUS_API int theseus_pool_group_step(theseus_pool_group *g, void *stream);
+THESEUS_API int theseus_pool_group_destroy(theseus_pool_group *g);
+
 #ifdef __cplusplus
 }
 #endif

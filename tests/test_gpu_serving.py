@@ -206,3 +206,81 @@ def test_async_admission_across_launch_ranges():
     pool.close()
     assert info["active_cases"] == 1 and info["free_elements"] == 4608 - 64, info
     assert np.array_equal(alone, mixed), np.abs(alone - mixed).max()
+
+
+@pytest.mark.parametrize("precision", [A.PRECISION_FP64, A.PRECISION_MIXED])
+def test_group_launch_equals_per_order_pools(precision):
+    """Pools of orders 2, 3 and 4 stepped by one launch per stage end
+    bitwise where the same pools stepped one launch each end (p = 2 at four
+    elements per workgroup in both)."""
+    import torch
+
+    from theseus_amd.serving import CaseFactory, Pool, PoolGroup, euler_runtime
+
+    _say(f"pool group vs per-order pools precision={precision}")
+    f = CaseFactory()
+    cases = {2: [(4, 2.0, (0.3, 1.1, 2.0)), (3, 0.7, (2.0, 0.1, 0.4))],
+             3: [(3, 1.5, (0.2, 0.4, 0.6)), (4, 3.3, (1.0, 1.0, 1.0))],
+             4: [(2, 1.0, (0.5, 0.0, 0.1)), (3, 2.5, (0.0, 0.9, 0.3))]}
+    tile = {2: 1, 3: 0, 4: 0}
+
+    def run(grouped):
+        pools = {p: Pool(euler_runtime(p, precision), 256 << 20, 8, tile=tile[p], xcd_remap=1) for p in cases}
+        ids = {p: [pools[p].admit(f.mesh(n), f.state(pools[p].rt, n, a, ph), 0.5)[0] for n, a, ph in cases[p]]
+               for p in cases}
+        group = PoolGroup([pools[p] for p in cases]) if grouped else None
+        for _ in range(5):
+            if grouped:
+                group.step()
+            else:
+                for p in cases:
+                    pools[p].step()
+        torch.cuda.synchronize()
+        out = {}
+        for p in cases:
+            for (n, _, _), cid in zip(cases[p], ids[p]):
+                buf = torch.empty(5 * n ** 3 * (p + 1) ** 3, dtype=torch.float64, device="cuda")
+                pools[p].read(cid, buf)
+                out[(p, cid)] = buf.cpu().numpy()
+        if group is not None:
+            group.close()
+        for pool in pools.values():
+            pool.close()
+        return out
+
+    alone = run(False)
+    grouped = run(True)
+    assert alone.keys() == grouped.keys()
+    for key in alone:
+        assert np.isfinite(alone[key]).all()
+        assert np.array_equal(alone[key], grouped[key]), (key, np.abs(alone[key] - grouped[key]).max())
+
+
+def test_group_rejects_mismatched_pools():
+    from theseus_amd.serving import Pool, PoolGroup, euler_runtime
+
+    _say("pool group argument checks")
+    a = Pool(euler_runtime(2, A.PRECISION_MIXED), 64 << 20, 4, tile=0, xcd_remap=1)
+    b = Pool(euler_runtime(2, A.PRECISION_MIXED), 64 << 20, 4, tile=0, xcd_remap=1)
+    c = Pool(euler_runtime(3, A.PRECISION_FP64), 64 << 20, 4, tile=0, xcd_remap=1)
+    d = Pool(euler_runtime(5, A.PRECISION_MIXED), 64 << 20, 4, tile=0, xcd_remap=1)
+    try:
+        for pools, msg in (([a, b], "one pool per order"), ([a, c], "precision"), ([d], "order 2, 3 or 4")):
+            with pytest.raises(A.TheseusError, match=msg):
+                PoolGroup(pools)
+    finally:
+        for p in (a, b, c, d):
+            p.close()
+
+
+def test_server_single_launch_drains_a_poisson_stream():
+    from theseus_amd.serving import Server, make_requests
+
+    _say("case server stream, one launch for all orders")
+    reqs = make_requests(num=24, rate_per_s=400.0, n_range=(3, 6), steps=8)
+    srv = Server(hbm_budget_bytes=1 << 30, max_cases=8, keep_results=True, single_launch=True)
+    res = srv.run(reqs, time_limit_s=120.0)
+    srv.close()
+    _say(f"  {res}")
+    assert res["cases_done"] == 24
+    assert all(r.result is not None and np.isfinite(r.result).all() for r in reqs)

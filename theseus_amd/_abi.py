@@ -75,6 +75,9 @@ EXPORTED_SYMBOLS = (
     "theseus_pool_read",
     "theseus_pool_integral_measures",
     "theseus_pool_release",
+    "theseus_pool_group_create",
+    "theseus_pool_group_step",
+    "theseus_pool_group_destroy",
 )
 
 
@@ -229,8 +232,12 @@ def load(path: Path | str | None = None) -> C.CDLL:
     lib.theseus_pool_read.argtypes = [vp, C.c_int, vp, vp]
     lib.theseus_pool_release.argtypes = [vp, C.c_int, vp]
     lib.theseus_pool_integral_measures.argtypes = [vp, C.c_int, C.POINTER(IntegralMeasures), vp]
+    lib.theseus_pool_group_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
+    lib.theseus_pool_group_step.argtypes = [vp, vp]
+    lib.theseus_pool_group_destroy.argtypes = [vp]
     for name in ("theseus_pool_create", "theseus_pool_destroy", "theseus_pool_info", "theseus_pool_admit",
-                 "theseus_pool_step", "theseus_pool_read", "theseus_pool_release", "theseus_pool_integral_measures"):
+                 "theseus_pool_step", "theseus_pool_read", "theseus_pool_release", "theseus_pool_integral_measures",
+                 "theseus_pool_group_create", "theseus_pool_group_step", "theseus_pool_group_destroy"):
         getattr(lib, name).restype = C.c_int
     if path is None:
         _lib = lib

@@ -47,6 +47,15 @@ const TileKernels *find_tiles(int dim, int p, int vol, int surf) {
   auto it = tile_table().find({dim, p, vol, surf});
   return it == tile_table().end() ? nullptr : &it->second;
 }
+std::map<std::tuple<int, int>, OrdersKernels> &orders_table() {
+  static std::map<std::tuple<int, int>, OrdersKernels> t;
+  return t;
+}
+void register_orders(int vol, int surf, OrdersKernels k) { orders_table()[{vol, surf}] = k; }
+const OrdersKernels *find_orders(int vol, int surf) {
+  auto it = orders_table().find({vol, surf});
+  return it == orders_table().end() ? nullptr : &it->second;
+}
 std::map<std::tuple<int, int>, LteKernels> &lte_table() {
   static std::map<std::tuple<int, int>, LteKernels> t;
   return t;
@@ -1394,25 +1403,30 @@ struct theseus_pool {
     }
   }
 
-  // the three SSP-RK3 stages of every case (MFEM RK3SSPSolver, src/Simulation.cpp:265-268)
-  void enqueue_step(hipStream_t s) const {
+  // stage st of the SSP-RK3 step of every case (MFEM RK3SSPSolver,
+  // src/Simulation.cpp:265-268): u -> A, (u, A) -> B, (u, B) -> u
+  RHSArgs stage_args(int st) const {
     double *u = d_U, *A = d_stage[0], *B = d_stage[1];
-    RHSArgs a = args(u, A);
-    a.mode = 1;
-    launch(a, s);
-    a = args(A, B);
-    a.mode = 2;
-    a.X = u;
-    a.cx = 3. / 4;
-    a.cy = 1. / 4;
-    launch(a, s);
-    a = args(B, u);
-    a.mode = 2;
-    a.X = u;
-    a.cx = 1. / 3;
-    a.cy = 2. / 3;
-    launch(a, s);
+    RHSArgs a = args(st == 0 ? u : (st == 1 ? A : B), st == 0 ? A : (st == 1 ? B : u));
+    a.mode = st == 0 ? 1 : 2;
+    if (st > 0) {
+      a.X = u;
+      a.cx = st == 1 ? 3. / 4 : 1. / 3;
+      a.cy = st == 1 ? 1. / 4 : 2. / 3;
+    }
+    return a;
   }
+
+  void enqueue_step(hipStream_t s) const {
+    for (int st = 0; st < 3; ++st) launch(stage_args(st), s);
+  }
+};
+
+// Pools of orders 2, 3 and 4 stepped by one launch per stage (k_rhs_orders):
+// a captured step per combination of the pools' launch ranges.
+struct GroupGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 static size_t pool_bytes_per_element(const theseus_pool_desc &d, int ndof, int nfp) {
@@ -1907,6 +1921,118 @@ int theseus_pool_step(theseus_pool *p, void *stream) {
     it = p->graphs.emplace(key, std::make_pair(g, x)).first;
   }
   HIP_TRY(hipGraphLaunch(it->second.second, (hipStream_t)stream));
+  return THESEUS_OK;
+}
+
+}  // extern "C"
+
+struct theseus_pool_group {
+  theseus_pool *pools[3] = {nullptr, nullptr, nullptr};  // orders 2, 3, 4
+  const OrdersKernels *ok = nullptr;
+  int pm = 0;
+  BasisArgs *d_basis = nullptr;  // [3]: the bases of orders 2, 3, 4
+  hipStream_t capture_stream = nullptr;
+  std::map<std::array<int, 4>, GroupGraph> graphs;  // (active of p = 2, 3, 4; bnd)
+};
+
+extern "C" {
+
+int theseus_pool_group_create(theseus_pool *const *pools, int n, theseus_pool_group **out) {
+  if (!pools || !out || n < 1 || n > 3) return fail(THESEUS_ERR_INVALID_ARGUMENT, "one to three pools and an output");
+  *out = nullptr;
+  theseus_pool_group *g = new theseus_pool_group();
+  const theseus_pool *first = nullptr;
+  for (int i = 0; i < n; ++i) {
+    theseus_pool *p = pools[i];
+    const char *why = nullptr;
+    if (!p) why = "null pool";
+    else if (p->cd.dim != 3 || p->cd.flow_model != THESEUS_FLOW_EULER) why = "grouped pools are 3-D Euler";
+    else if (p->cd.order < 2 || p->cd.order > 4) why = "grouped pools have order 2, 3 or 4";
+    else if (g->pools[p->cd.order - 2]) why = "at most one pool per order";
+    else if (first && (p->cd.volume_flux != first->cd.volume_flux || p->cd.surface_flux != first->cd.surface_flux ||
+                       p->pm != first->pm))
+      why = "grouped pools share their flux pairing and precision";
+    if (why) {
+      delete g;
+      return fail(THESEUS_ERR_INVALID_ARGUMENT, why);
+    }
+    g->pools[p->cd.order - 2] = p;
+    if (!first) first = p;
+  }
+  g->ok = find_orders(first->cd.volume_flux, first->cd.surface_flux);
+  if (!g->ok) {
+    delete g;
+    return fail(THESEUS_ERR_UNSUPPORTED, "the one-launch kernel is compiled for Ranocha + Lax-Friedrichs only");
+  }
+  g->pm = first->pm;
+  BasisArgs bases[3];
+  std::memset(bases, 0, sizeof bases);
+  for (int i = 0; i < 3; ++i)
+    if (g->pools[i]) bases[i] = g->pools[i]->basis;
+  int rc = upload<BasisArgs>(&g->d_basis, bases, 3);
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  const hipError_t e = hipStreamCreate(&g->capture_stream);  // default priority
+  if (e != hipSuccess) {
+    theseus_pool_group_destroy(g);
+    return fail(THESEUS_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return THESEUS_OK;
+}
+
+int theseus_pool_group_destroy(theseus_pool_group *g) {
+  if (!g) return THESEUS_OK;
+  for (auto &kv : g->graphs) {
+    if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+    if (kv.second.graph) hipGraphDestroy(kv.second.graph);
+  }
+  if (g->capture_stream) hipStreamDestroy(g->capture_stream);
+  hipFree(g->d_basis);
+  delete g;
+  return THESEUS_OK;
+}
+
+int theseus_pool_group_step(theseus_pool_group *g, void *stream) {
+  if (!g) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null group");
+  static constexpr int kEpb[3] = {4, 2, 1};  // elements per workgroup of orders 2, 3, 4 in k_rhs_orders
+  std::array<int, 4> key = {0, 0, 0, 0};
+  for (int i = 0; i < 3; ++i)
+    if (g->pools[i]) {
+      key[i] = g->pools[i]->active;
+      key[3] |= g->pools[i]->bnd;
+    }
+  if (key[0] + key[1] + key[2] == 0) return THESEUS_OK;  // no case admitted
+  auto it = g->graphs.find(key);
+  if (it == g->graphs.end()) {
+    GroupGraph gg;
+    RHSArgs host[3][3];  // [stage][order]
+    std::memset(host, 0, sizeof host);
+    int nb[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i)
+      if (g->pools[i] && key[i] > 0) {
+        if (key[i] % kEpb[i]) return fail(THESEUS_ERR_UNSUPPORTED, "launch range not a multiple of the workgroup's elements");
+        nb[i] = key[i] / kEpb[i];
+        for (int st = 0; st < 3; ++st) host[st][i] = g->pools[i]->stage_args(st);
+      }
+    const int off[2] = {(nb[0] + 7) / 8 * 8, (nb[0] + 7) / 8 * 8 + (nb[1] + 7) / 8 * 8};
+    hipStream_t cs = g->capture_stream;
+    const OrdersLaunch launch = g->ok->euler[g->pm][key[3]];
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      for (int st = 0; st < 3; ++st) launch(host[st], g->d_basis, off, nb, cs);
+      e = hipStreamEndCapture(cs, &gg.graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      if (gg.graph) hipGraphDestroy(gg.graph);
+      return fail(THESEUS_ERR_DEVICE, std::string("capturing the grouped step: ") + hipGetErrorString(e));
+    }
+    it = g->graphs.emplace(key, gg).first;
+  }
+  HIP_TRY(hipGraphLaunch(it->second.exec, (hipStream_t)stream));
   return THESEUS_OK;
 }
 

@@ -1,5 +1,6 @@
 // Kernel instantiations: Ranocha volume + Lax-Friedrichs face (3-D Taylor-Green
-// workload), plus the tile-shape variants the serving pool tunes over.
+// workload), plus the tile-shape variants the serving pool tunes over and
+// the kernel that advances pools of orders 2, 3 and 4 in one launch.
 #include "registry.hpp"
 
 namespace thx {
@@ -18,5 +19,7 @@ Orders<3> orders3;
 TileRegistrar<3, 2, 5, 2> tiles2;
 TileRegistrar<3, 2, 5, 3> tiles3;
 TileRegistrar<3, 2, 5, 4> tiles4;
+// orders 2, 3 and 4 in one launch
+OrdersRegistrar<2, 5> orders234;
 }  // namespace
 }  // namespace thx

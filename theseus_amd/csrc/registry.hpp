@@ -38,6 +38,15 @@ struct TileKernels {
   int tpb[2] = {0, 0};
 };
 
+// 3-D Euler over orders 2, 3 and 4 in one launch (k_rhs_orders):
+// [precision][bnd]; a[i] are the arguments of order i + 2 (its basis is
+// basis[i], device memory).
+using OrdersLaunch = void (*)(const RHSArgs a[3], const BasisArgs *basis, const int off[2], const int nb[3],
+                              hipStream_t);
+struct OrdersKernels {
+  OrdersLaunch euler[2][2] = {};
+};
+
 // Euler with the tabulated LTE gas and the LTEEC flux
 struct LteKernels {
   RHSLaunch rhs = nullptr;
@@ -49,6 +58,8 @@ void register_flux(int dim, int p, int vol, int surf, FluxKernels k);
 void register_lte(int dim, int p, LteKernels k);
 const LteKernels *find_lte(int dim, int p);
 void register_tiles(int dim, int p, int vol, int surf, TileKernels k);
+void register_orders(int vol, int surf, OrdersKernels k);
+const OrdersKernels *find_orders(int vol, int surf);
 const TileKernels *find_tiles(int dim, int p, int vol, int surf);
 void register_shape(int dim, int p, ShapeKernels k);
 const FluxKernels *find_flux(int dim, int p, int vol, int surf);
@@ -107,6 +118,25 @@ struct TileRegistrar {
     k.tpb[0] = Shape<DIM, P>::TPB;
     k.tpb[1] = Shape<DIM, P, E1>::TPB;
     register_tiles(DIM, P, VOL, SURF, k);
+  }
+};
+
+template <typename R, int VOL, int SURF, bool BND>
+void launch_rhs_orders(const RHSArgs a[3], const BasisArgs *basis, const int off[2], const int nb[3], hipStream_t s) {
+  const RHSCore a2 = a[0], a3 = a[1], a4 = a[2];
+  hipLaunchKernelGGL((k_rhs_orders<R, VOL, SURF, BND>), dim3(off[1] + nb[2]), dim3(128), 0, s, a2, a3, a4, basis,
+                     off[0], off[1], nb[0], nb[1], nb[2]);
+}
+
+template <int VOL, int SURF>
+struct OrdersRegistrar {
+  OrdersRegistrar() {
+    OrdersKernels k;
+    k.euler[0][0] = &launch_rhs_orders<double, VOL, SURF, false>;
+    k.euler[1][0] = &launch_rhs_orders<float, VOL, SURF, false>;
+    k.euler[0][1] = &launch_rhs_orders<double, VOL, SURF, true>;
+    k.euler[1][1] = &launch_rhs_orders<float, VOL, SURF, true>;
+    register_orders(VOL, SURF, k);
   }
 };
 

@@ -55,7 +55,9 @@ struct BasisArgs {
   double Vinv[MAXNP * MAXNP];   // inverse 1-D Legendre Vandermonde: mode a = sum_i Vinv[a*Np+i] u_i
 };
 
-struct RHSArgs {
+// Everything a kernel launch needs but the basis (RHSArgs adds it): small
+// enough that three of them fit in one launch's kernel arguments.
+struct RHSCore {
   const double *U;    // stage state (L-vector)
   double *out;        // mode 0: dU; mode 1: U + dt dU; mode 2: cx X + cy (U + dt dU)
   const double *X;    // RK base state for mode 2
@@ -87,15 +89,18 @@ struct RHSArgs {
   const double *case_pref;
   int xcd_remap;      // block order (logical_block)
   lte::View lte;      // tabulated LTE gas (LTEEC kernels)
+};
+
+struct RHSArgs : RHSCore {
   BasisArgs basis;
 };
 
 template <typename R>
-__device__ __forceinline__ const GasC<R> &gas_of(const RHSArgs &a);
+__device__ __forceinline__ const GasC<R> &gas_of(const RHSCore &a);
 template <>
-__device__ __forceinline__ const GasC<double> &gas_of<double>(const RHSArgs &a) { return a.gas; }
+__device__ __forceinline__ const GasC<double> &gas_of<double>(const RHSCore &a) { return a.gas; }
 template <>
-__device__ __forceinline__ const GasC<float> &gas_of<float>(const RHSArgs &a) { return a.gas32; }
+__device__ __forceinline__ const GasC<float> &gas_of<float>(const RHSCore &a) { return a.gas32; }
 
 // Node index of in-face point t on local face (d, side)
 template <int DIM, int Np>
@@ -148,7 +153,7 @@ __device__ __forceinline__ int logical_block(int remap) {
 // the precision of the flux arithmetic; 0 on lanes without an element),
 // written and synchronised by the caller.
 template <int DIM, int P>
-__device__ __forceinline__ double blending_alpha(const RHSArgs &a, double *buf, const double *sVi, int node, int i,
+__device__ __forceinline__ double blending_alpha(const RHSCore &a, double *buf, const double *sVi, int node, int i,
                                                  int j, int k, bool lane_ok, bool valid) {
   using S = Shape<DIM, P>;
   constexpr int Np = S::Np, ND = S::NDOF, NP2 = S::NPOW2;
@@ -421,7 +426,7 @@ struct MetaRegs {
 };
 
 template <int DIM, int EPB, int TPB>
-__device__ __forceinline__ void meta_load_index(const RHSArgs &a, int ebase_blk, MetaRegs<DIM, EPB, TPB> &m) {
+__device__ __forceinline__ void meta_load_index(const RHSCore &a, int ebase_blk, MetaRegs<DIM, EPB, TPB> &m) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < MetaRegs<DIM, EPB, TPB>::NN; ++r) {
@@ -434,7 +439,7 @@ __device__ __forceinline__ void meta_load_index(const RHSArgs &a, int ebase_blk,
 }
 
 template <int DIM, int EPB, int TPB>
-__device__ __forceinline__ void meta_load_own(const RHSArgs &a, int ebase_blk, MetaRegs<DIM, EPB, TPB> &m) {
+__device__ __forceinline__ void meta_load_own(const RHSCore &a, int ebase_blk, MetaRegs<DIM, EPB, TPB> &m) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < MetaRegs<DIM, EPB, TPB>::NA; ++r) {
@@ -448,7 +453,7 @@ __device__ __forceinline__ void meta_load_own(const RHSArgs &a, int ebase_blk, M
 }
 
 template <int DIM, int EPB, int TPB>
-__device__ __forceinline__ void meta_load_normals(const RHSArgs &a, int ebase_blk, MetaRegs<DIM, EPB, TPB> &m) {
+__device__ __forceinline__ void meta_load_normals(const RHSCore &a, int ebase_blk, MetaRegs<DIM, EPB, TPB> &m) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < MetaRegs<DIM, EPB, TPB>::NN; ++r) {
@@ -535,15 +540,65 @@ struct RhsWaves {
   static constexpr int value = fits ? 4 : 1;
 };
 
-// Fused volume + face kernel: the Euler right-hand side, or the second
+// Workgroup LDS of the fused kernel.  RhsLds holds it as one object (the
+// multi-order kernel overlays the orders' in a union); RhsLdsRef names the
+// arrays wherever they live, so the single-order kernel keeps them as
+// separate LDS variables.
+template <typename R, int DIM, int P, int EPBX>
+struct RhsLdsDims {
+  using S = Shape<DIM, P, EPBX>;
+  using SC = RhsWgScratch<R, DIM, P, EPBX>;
+  static constexpr int Np = S::Np, NQ = S::NEQ, EPB = S::EPB;
+  static constexpr int NDS = StagedIndex<Np, S::NDOF>::size;
+  static constexpr bool FP64 = sizeof(R) == sizeof(double);
+  static constexpr int PE = FP64 ? 1 : EPB, PN = FP64 ? 1 : NDS;
+};
+
+template <typename R, int DIM, int P, int EPBX>
+struct RhsLds {
+  using T = RhsLdsDims<R, DIM, P, EPBX>;
+  R sq[T::EPB][T::NQ][T::NDS];
+  R sp[T::EPB][T::NDS];
+  R sir[T::EPB][T::NDS];
+  R spd_[T::PE][T::PN];
+  double sraw[T::SC::DOUBLES];
+  R sD2[T::Np * T::Np];
+  double sVi[T::Np * T::Np];
+  ElemMeta<DIM, T::EPB> sm;
+  int sblend;
+  double sprefe[T::PE];
+};
+
+template <typename R, int DIM, int P, int EPBX>
+struct RhsLdsRef {
+  using T = RhsLdsDims<R, DIM, P, EPBX>;
+  R (&sq)[T::EPB][T::NQ][T::NDS];
+  R (&sp)[T::EPB][T::NDS];
+  R (&sir)[T::EPB][T::NDS];
+  R (&spd_)[T::PE][T::PN];
+  double (&sraw)[T::SC::DOUBLES];
+  R (&sD2)[T::Np * T::Np];
+  double (&sVi)[T::Np * T::Np];
+  ElemMeta<DIM, T::EPB> &sm;
+  int &sblend;
+  double (&sprefe)[T::PE];
+  __device__ static RhsLdsRef of(RhsLds<R, DIM, P, EPBX> &l) {
+    return RhsLdsRef{l.sq, l.sp, l.sir, l.spd_, l.sraw, l.sD2, l.sVi, l.sm, l.sblend, l.sprefe};
+  }
+};
+
+// Fused volume + face kernel body: the Euler right-hand side, or the second
 // Navier-Stokes pass when VISC.  R is the type of the flux arithmetic and of
 // the LDS staging (double, or float in the mixed mode); the state, the RK
 // combination and the blending indicator are FP64 in both.  BND: the mesh
 // has boundary faces (a periodic mesh runs the variant without the
-// boundary-condition code).
-template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX = 0, bool BND = false>
-__global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB))
-__attribute__((amdgpu_waves_per_eu(RhsWaves<R, DIM, P, VOL, SURF, VISC, EPBX, BND>::value))) k_rhs(RHSArgs a) {
+// boundary-condition code).  `blk` is the workgroup's logical block (its
+// first element is blk * EPB); the body runs on TPB lanes.  B is the basis
+// (a.basis, passed apart so that a caller may hand over a copy of `a`
+// without it).
+template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX, bool BND>
+__device__ __forceinline__ void rhs_body(const RHSCore &a, const BasisArgs &B, const int blk,
+                                         RhsLdsRef<R, DIM, P, EPBX> L) {
   using S = Shape<DIM, P, EPBX>;
   using SC = RhsWgScratch<R, DIM, P, EPBX>;
   using VP = VolPlan<DIM, P, EPBX>;
@@ -556,19 +611,19 @@ __attribute__((amdgpu_waves_per_eu(RhsWaves<R, DIM, P, VOL, SURF, VISC, EPBX, BN
 
   using SX = StagedIndex<Np, ND>;
   constexpr int NDS = SX::size;
-  __shared__ R sq[EPB][NQ][NDS];
-  __shared__ R sp[EPB][NDS];
-  __shared__ R sir[EPB][NDS];
-  __shared__ R spd_[FP64 ? 1 : EPB][FP64 ? 1 : NDS];  // p - p_ref (mixed precision)
-  __shared__ double sraw[SC::DOUBLES];  // flux rows (as R) or indicator sweeps (FP64), see RhsWgScratch
-  __shared__ R sD2[Np * Np];
-  __shared__ double sVi[Np * Np];
-  __shared__ ElemMeta<DIM, EPB> sm;
-  __shared__ int sblend;
+  auto &sq = L.sq;
+  auto &sp = L.sp;
+  auto &sir = L.sir;
+  auto &spd_ = L.spd_;  // p - p_ref (mixed precision)
+  auto &sraw = L.sraw;  // flux rows (as R) or indicator sweeps (FP64), see RhsWgScratch
+  auto &sD2 = L.sD2;
+  auto &sVi = L.sVi;
+  auto &sm = L.sm;
+  auto &sblend = L.sblend;
   // each element's background pressure (mixed mode): lanes evaluate face
   // points of the other elements of the workgroup, which in the serving
   // pool may belong to another case
-  __shared__ double sprefe[FP64 ? 1 : EPB];
+  auto &sprefe = L.sprefe;
   auto sscr = [&](int row, int q) -> R & { return reinterpret_cast<R *>(sraw)[row * NQ + q]; };
   double *const sind = sraw + (lane_ok_pre(threadIdx.x, ND, EPB) ? (threadIdx.x / ND) * SC::IND_D : 0);
   // pressure for the momentum fluxes: p itself in FP64, p - p_ref in the mixed mode
@@ -586,7 +641,7 @@ __attribute__((amdgpu_waves_per_eu(RhsWaves<R, DIM, P, VOL, SURF, VISC, EPBX, BN
   const int node = tid - el * ND;
   const bool lane_ok = el < EPB;
   const int elc = lane_ok ? el : 0;
-  const int ebase_blk = logical_block(a.xcd_remap) * EPB;
+  const int ebase_blk = blk * EPB;
   const int e = ebase_blk + el;
   // serving pool: the case owning this element (free slots are skipped)
   const int ecase = a.elem_case ? a.elem_case[min(ebase_blk + elc, a.ne - 1)] : 0;
@@ -629,7 +684,7 @@ __attribute__((amdgpu_waves_per_eu(RhsWaves<R, DIM, P, VOL, SURF, VISC, EPBX, BN
   }
   static_assert(Np * Np <= TPB, "one basis entry per lane");
   const int kb = min(tid, Np * Np - 1);
-  const double bD2 = a.basis.Dhat2[kb], bVi = a.basis.Vinv[kb];
+  const double bD2 = B.Dhat2[kb], bVi = B.Vinv[kb];
 
   meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
   if (tid < Np * Np) {
@@ -773,7 +828,7 @@ __attribute__((amdgpu_waves_per_eu(RhsWaves<R, DIM, P, VOL, SURF, VISC, EPBX, BN
       if (valid) {
         const int L = face_index<DIM, Np>(dir, i, j, k);
         const int pos = dir == 0 ? i : (dir == 1 ? j : k);
-        const R den = R(Je * a.basis.w[pos]);
+        const R den = R(Je * B.w[pos]);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           const R fm = pos > 0 ? sscr(el * NI + L * (Np - 1) + pos - 1, q) : R(0);
@@ -822,7 +877,7 @@ __attribute__((amdgpu_waves_per_eu(RhsWaves<R, DIM, P, VOL, SURF, VISC, EPBX, BN
   //      neighbour traces from the registers loaded at the top, which
   //      arrived while the volume terms were computed
   {
-    const double w0 = a.basis.w[0];
+    const double w0 = B.w[0];
 #pragma unroll
     for (int r = 0; r < NFR; ++r) {
       const int fi = tid + r * TPB;
@@ -965,6 +1020,68 @@ __attribute__((amdgpu_waves_per_eu(RhsWaves<R, DIM, P, VOL, SURF, VISC, EPBX, BN
       else o = a.cx * xb[q] + a.cy * (qo + dt * r);
       a.out[q * Ntot + ebase] = o;
     }
+  }
+}
+
+template <typename R, int DIM, int P, int VOL, int SURF, bool VISC, int EPBX = 0, bool BND = false>
+__global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB))
+__attribute__((amdgpu_waves_per_eu(RhsWaves<R, DIM, P, VOL, SURF, VISC, EPBX, BND>::value))) k_rhs(RHSArgs a) {
+  using T = RhsLdsDims<R, DIM, P, EPBX>;
+  __shared__ R sq[T::EPB][T::NQ][T::NDS];
+  __shared__ R sp[T::EPB][T::NDS];
+  __shared__ R sir[T::EPB][T::NDS];
+  __shared__ R spd_[T::PE][T::PN];
+  __shared__ double sraw[T::SC::DOUBLES];
+  __shared__ R sD2[T::Np * T::Np];
+  __shared__ double sVi[T::Np * T::Np];
+  __shared__ ElemMeta<DIM, T::EPB> sm;
+  __shared__ int sblend;
+  __shared__ double sprefe[T::PE];
+  rhs_body<R, DIM, P, VOL, SURF, VISC, EPBX, BND>(
+      a, a.basis, logical_block(a.xcd_remap),
+      RhsLdsRef<R, DIM, P, EPBX>{sq, sp, sir, spd_, sraw, sD2, sVi, sm, sblend, sprefe});
+}
+
+// Several orders in one launch (3-D Euler, 128 lanes per workgroup: p = 2
+// with four elements per workgroup, p = 3 with two, p = 4 with one): blocks
+// [0, off[0]) run order 2 on args[0], [off[0], off[1]) order 3 on args[1],
+// [off[1], gridDim.x) order 4 on args[2]; each order's range starts on a
+// multiple of 8 blocks so its XCD-aware block order stays within the range,
+// and its trailing padding blocks return at once.  The three orders' LDS
+// share one union; their arguments are kernel arguments, the bases
+// (basis[0..2]) device memory.  Its register count is the largest of the
+// three orders' code paths: 128 in FP64, as the single-order kernels; 96 in
+// the mixed mode (5 waves per SIMD, where the single-order kernels take 80
+// and run 6) and 128 with boundary conditions -- the smallest counts that
+// need no scratch memory.
+template <typename R, int VOL, int SURF, bool BND>
+__global__ void __launch_bounds__(128)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(double) || BND ? 4 : 5))) k_rhs_orders(RHSCore a2, RHSCore a3, RHSCore a4, const BasisArgs *basis,
+                                                    int off0, int off1, int nb0, int nb1, int nb2) {
+  static_assert(Shape<3, 2, 4>::TPB == 128 && Shape<3, 3>::TPB == 128 && Shape<3, 4>::TPB == 128, "128 lanes");
+  __shared__ union U {
+    RhsLds<R, 3, 2, 4> p2;
+    RhsLds<R, 3, 3, 0> p3;
+    RhsLds<R, 3, 4, 0> p4;
+  } lds;
+  const int b = blockIdx.x;
+  auto remap = [](int lb, int nb, int remap_on) {
+    return (remap_on && (nb & 7) == 0) ? (lb & 7) * (nb >> 3) + (lb >> 3) : lb;
+  };
+  if (b < off0) {
+    if (b >= nb0) return;
+    rhs_body<R, 3, 2, VOL, SURF, false, 4, BND>(a2, basis[0], remap(b, nb0, a2.xcd_remap),
+                                                RhsLdsRef<R, 3, 2, 4>::of(lds.p2));
+  } else if (b < off1) {
+    const int lb = b - off0;
+    if (lb >= nb1) return;
+    rhs_body<R, 3, 3, VOL, SURF, false, 0, BND>(a3, basis[1], remap(lb, nb1, a3.xcd_remap),
+                                                RhsLdsRef<R, 3, 3, 0>::of(lds.p3));
+  } else {
+    const int lb = b - off1;
+    if (lb >= nb2) return;
+    rhs_body<R, 3, 4, VOL, SURF, false, 0, BND>(a4, basis[2], remap(lb, nb2, a4.xcd_remap),
+                                                RhsLdsRef<R, 3, 4, 0>::of(lds.p4));
   }
 }
 

@@ -105,6 +105,33 @@ class Pool:
             pass
 
 
+class PoolGroup:
+    """Pools of orders 2, 3 and 4 advanced by one kernel launch per stage
+    (theseus_pool_group); each pool keeps its own admission and release."""
+
+    def __init__(self, pools):
+        self._lib = A.load()
+        self.pools = list(pools)
+        arr = (C.c_void_p * len(self.pools))(*[p._h.value for p in self.pools])
+        h = C.c_void_p()
+        A.check(self._lib, self._lib.theseus_pool_group_create(arr, len(self.pools), C.byref(h)))
+        self._h = h
+
+    def step(self, stream=None) -> None:
+        A.check(self._lib, self._lib.theseus_pool_group_step(self._h, _stream(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.theseus_pool_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _stream(stream):
     if stream is None:
         return None
@@ -225,19 +252,22 @@ def percentile(values, q):
 class Server:
     """Continuous batching over one pool per polynomial order.
 
-    The host loop never blocks on the device while work is in flight: each
-    pool's steps go out in batches of at most ``max_batch_steps`` (never past
-    a case's last step), each batch followed by an event; the loop polls the
-    events, admits arrivals the moment a range is free (admission is only
-    enqueued on the pool's stream, behind the batches already there), and
-    keeps up to two batches queued per pool so the device does not wait for
-    the host between them.  A case joins the batches enqueued after its
-    admission; a finished case is read and released when its last batch's
-    event has completed."""
+    The pools advance either on a stream each (a launch per pool and stage,
+    the launches of different orders overlapping on the device), or, with
+    ``single_launch``, together: one launch per stage runs every order
+    (PoolGroup) on one stream.  The host loop never blocks on the device
+    while work is in flight: steps go out in batches of at most
+    ``max_batch_steps`` (never past a case's last step), each followed by an
+    event; the loop polls the events, admits arrivals the moment a range is
+    free (admission is only enqueued on the stream, behind the batches
+    already there), and keeps up to ``max_inflight`` batches queued so the
+    device does not wait for the host between them.  A case joins the
+    batches enqueued after its admission; a finished case is read and
+    released once its last batch's event has completed."""
 
     def __init__(self, orders=(2, 3, 4), hbm_budget_bytes: int = 64 << 30, max_cases: int = 256,
                  precision=A.PRECISION_MIXED, cfl: float = 0.5, keep_results: bool = False, tile: int = -1,
-                 xcd_remap: int = -1, max_batch_steps: int = 8, max_inflight: int = 2):
+                 xcd_remap: int = -1, max_batch_steps: int = 4, max_inflight: int = 2, single_launch: bool = False):
         import torch
 
         self.torch = torch
@@ -247,48 +277,66 @@ class Server:
         self.keep_results = keep_results
         budget = int(hbm_budget_bytes) // len(orders)
         self.pools = {p: Pool(euler_runtime(p, precision), budget, max_cases, tile, xcd_remap) for p in orders}
-        self.streams = {p: torch.cuda.Stream() for p in orders}  # default priority
+        self.group = None
+        if single_launch:
+            self.group = PoolGroup([self.pools[p] for p in orders])
+            self.lanes = [tuple(orders)]
+        else:
+            self.lanes = [(p,) for p in orders]
+        self.lane_of = {p: lane for lane in self.lanes for p in lane}
+        self.lane_stream = {lane: torch.cuda.Stream() for lane in self.lanes}  # default priority
+        self.streams = {p: self.lane_stream[self.lane_of[p]] for p in orders}
         self.factory = CaseFactory()
 
     def close(self):
+        if self.group is not None:
+            self.group.close()
         for pool in self.pools.values():
             pool.close()
+
+    def _step(self, lane, stream):
+        if self.group is not None:
+            self.group.step(stream)
+        else:
+            self.pools[lane[0]].step(stream)
 
     def run(self, requests, time_limit_s: float = 600.0, log=None) -> dict:
         torch = self.torch
         waiting = deque(sorted(requests, key=lambda r: r.arrival_s))
         queued: list[Request] = []
-        # order -> {case_id: [request, steps completed, steps enqueued]}
-        active = {p: {} for p in self.pools}
-        inflight = {p: deque() for p in self.pools}  # order -> (event, steps, case ids) per batch
+        # lane -> {(order, case_id): [request, steps completed, steps enqueued]}
+        active = {lane: {} for lane in self.lanes}
+        inflight = {lane: deque() for lane in self.lanes}  # lane -> (event, steps, case keys) per batch
         done: list[Request] = []
         states = {}
         t0 = time.perf_counter()
         dof_rhs = 0.0
-        while (waiting or queued or any(active[p] for p in active)) and time.perf_counter() - t0 < time_limit_s:
+        while (waiting or queued or any(active.values())) and time.perf_counter() - t0 < time_limit_s:
             progressed = False
             # completed batches: count their steps, hand back finished cases
-            for p, q in inflight.items():
+            for lane, q in inflight.items():
+                st = self.lane_stream[lane]
                 while q and q[0][0].query():
-                    _, k, cids = q.popleft()
+                    _, k, keys = q.popleft()
                     progressed = True
-                    cases = active[p]
-                    for cid in cids:
-                        rec = cases[cid]
+                    cases = active[lane]
+                    for key in keys:
+                        rec = cases[key]
                         rec[1] += k
                         dof_rhs += 3 * k * rec[0].dofs
-                    for cid in [c for c in cids if cases[c][1] >= cases[c][0].steps]:
-                        r = cases.pop(cid)[0]
+                    for key in [c for c in keys if cases[c][1] >= cases[c][0].steps]:
+                        p, cid = key
+                        r = cases.pop(key)[0]
                         if self.keep_results:
                             out = torch.empty(r.dofs, dtype=torch.float64, device="cuda")
-                            self.pools[p].read(cid, out, self.streams[p])
+                            self.pools[p].read(cid, out, st)
                             r.result = out.cpu().numpy()
                         else:
                             out = states.get((p, r.dofs))
                             if out is None:
                                 out = states[(p, r.dofs)] = torch.empty(r.dofs, dtype=torch.float64, device="cuda")
-                            self.pools[p].read(cid, out, self.streams[p])
-                        self.pools[p].release(cid, self.streams[p])
+                            self.pools[p].read(cid, out, st)
+                        self.pools[p].release(cid, st)
                         r.done_s = time.perf_counter() - t0
                         done.append(r)
                         if log:
@@ -300,40 +348,41 @@ class Server:
             still = []
             for r in queued:
                 pool = self.pools[r.order]
-                with torch.cuda.stream(self.streams[r.order]):
+                st = self.streams[r.order]
+                with torch.cuda.stream(st):
                     u0 = self.factory.state(pool.rt, r.n, r.amplitude, r.phases)
                 try:
-                    cid, _ = pool.admit(self.factory.mesh(r.n), u0, self.cfl, self.streams[r.order], want_dt=False)
+                    cid, _ = pool.admit(self.factory.mesh(r.n), u0, self.cfl, st, want_dt=False)
                 except CapacityError:
                     still.append(r)
                     continue
                 r.admitted_s = time.perf_counter() - t0
-                active[r.order][cid] = [r, 0, 0]
+                active[self.lane_of[r.order]][(r.order, cid)] = [r, 0, 0]
                 progressed = True
             queued = still
-            # keep each pool's queue of batches topped up
-            for p, cases in active.items():
-                if not cases or len(inflight[p]) >= self.max_inflight:
+            # keep each lane's queue of batches topped up
+            for lane, cases in active.items():
+                if not cases or len(inflight[lane]) >= self.max_inflight:
                     continue
                 k = min(self.max_batch_steps, min(rec[0].steps - rec[2] for rec in cases.values()))
                 if k <= 0:
                     continue  # a case ends with the batches in flight: wait for them
-                st = self.streams[p]
+                st = self.lane_stream[lane]
                 for _ in range(k):
-                    self.pools[p].step(st)
+                    self._step(lane, st)
                 for rec in cases.values():
                     rec[2] += k
                 ev = torch.cuda.Event()
                 ev.record(st)
-                inflight[p].append((ev, k, list(cases.keys())))
+                inflight[lane].append((ev, k, list(cases.keys())))
                 progressed = True
             if not progressed:
-                if not any(active[p] for p in active) and waiting:
+                if not any(active.values()) and waiting:
                     time.sleep(max(0.0, min(waiting[0].arrival_s - (time.perf_counter() - t0), 0.01)))
                 else:
                     time.sleep(2e-4)
         makespan = time.perf_counter() - t0
-        for p, st in self.streams.items():
+        for st in self.lane_stream.values():
             st.synchronize()
         tts = [r.done_s - r.arrival_s for r in done]
         return {
