@@ -1096,8 +1096,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(double) || BND ? 4 : 5)))
 // -0.5 n.Fv lifted with the face weight, and n.Fv at every point of the
 // element's faces for the neighbours' halves (n the canonical area vector).
 // ---------------------------------------------------------------------------
+// Occupancy requested for the gradient pass: 5 waves per SIMD for FP64 3-D
+// p = 3 on periodic meshes (at most 96 VGPRs; the LDS allows 5.5), where
+// the allocator would otherwise take 116 and run 4; elsewhere the default.
+// With the boundary code it would spill.
+template <typename R, int DIM, int P, int EPBX, bool BND>
+struct GradWaves {
+  static constexpr bool fits = sizeof(R) == sizeof(double) && DIM == 3 && P == 3 && EPBX == 0 && !BND;
+  static constexpr int value = fits ? 5 : 1;
+};
+
 template <typename R, int DIM, int P, int EPBX = 0, bool BND = false>
-__global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB)) k_ns_grad(RHSArgs a) {
+__global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB))
+__attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND>::value))) k_ns_grad(RHSArgs a) {
   using S = Shape<DIM, P, EPBX>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NV = VisComp<DIM>::N;
