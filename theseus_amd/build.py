@@ -28,6 +28,9 @@ HIPFLAGS = [
     # packed FP32 VALU ops stay off: some of their forms have given wrong
     # results on gfx950 now and then, with no error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core accumulators in ordinary VGPRs: the gradient pass's 4x4
+    # products are consumed at once, and copies out of AGPRs cost a wave
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

@@ -1096,14 +1096,89 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(double) || BND ? 4 : 5)))
 // -0.5 n.Fv lifted with the face weight, and n.Fv at every point of the
 // element's faces for the neighbours' halves (n the canonical area vector).
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Matrix-core line contractions of the gradient pass (FP32, 3-D, p = 3, one
+// element per wave, lane = node i + 4j + 16k).  v_mfma_f32_4x4x1f32 computes
+// 16 independent 4x4 outer products per wave: lane t receives
+// out[r] += a(lane 4(t/4) + r) * b(lane t).  With b(t) = Op[t%4][l] at step
+// l (l = 0..3) and a(4b + r) = field r along line b at position l, the four
+// steps give lane t the contraction sum_l Op[t%4][l] f_r(line b, l) for four
+// fields r at position t%4 of line b = t/4 -- in the same order of fused
+// multiply-adds as the per-lane loop, so bitwise the same values.  Lines in x
+// are numbered so that lane t holds node t; lines in y and z put the node of
+// lane t elsewhere, and one ds_bpermute per value brings it home.
+// ---------------------------------------------------------------------------
+template <typename R, int DIM, int P, int EPBX>
+struct GradMfma {
+  static constexpr bool on = sizeof(R) == sizeof(float) && DIM == 3 && P == 3 && Shape<DIM, P, EPBX>::NDOF == 64 &&
+                             Shape<DIM, P, EPBX>::TPB == 64 * Shape<DIM, P, EPBX>::EPB;
+};
+
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+// lane holding node t's value after a y / z line contraction
+__device__ __forceinline__ int mfma_src_y(int t) { return 4 * (t & 3) + 16 * (t >> 4) + ((t >> 2) & 3); }
+__device__ __forceinline__ int mfma_src_z(int t) { return 4 * (t & 15) + (t >> 4); }
+__device__ __forceinline__ float lane_fetch(int src, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src * 4, __float_as_int(v)));
+}
+
+// Lines of direction DIR: lane t's base node and stride (block b = t/4 is the
+// line, numbered so that in x lane t holds node t).
+template <int DIR>
+__device__ __forceinline__ int mfma_line_base(int b) {
+  return DIR == 0 ? 4 * b : (DIR == 1 ? (b & 3) + 16 * (b >> 2) : b);
+}
+constexpr int mfma_line_stride(int dir) { return dir == 0 ? 1 : (dir == 1 ? 4 : 16); }
+
+// sum_l Op[pos][l] * F[r][node at position l of the line] for the fields
+// r = 0..3 of direction DIR, returned at the lane of each node.
+template <int DIR>
+__device__ __forceinline__ void line_contract_dir(const float (*F)[64], const float op[4], int t, float out[4]) {
+  const int b = t >> 2, r = t & 3, base = mfma_line_base<DIR>(b);
+  mfma_f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int l = 0; l < 4; ++l)
+    c = __builtin_amdgcn_mfma_f32_4x4x1f32(F[r][base + mfma_line_stride(DIR) * l], op[l], c, 0, 0, 0);
+  const int src = DIR == 0 ? t : (DIR == 1 ? mfma_src_y(t) : mfma_src_z(t));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = DIR == 0 ? c[q] : lane_fetch(src, c[q]);
+}
+
+// All three directions of five fields: out[d][r]; the fifth field's three
+// directions share one product set (rows 0, 1, 2 of its blocks).  Every
+// lane of the wave must call it.
+__device__ __forceinline__ void line_contract_mfma5(const float (*F)[64], const float op[4], int t, float (*out)[5]) {
+  float o[4];
+  line_contract_dir<0>(F, op, t, o);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[0][q] = o[q];
+  line_contract_dir<1>(F, op, t, o);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[1][q] = o[q];
+  line_contract_dir<2>(F, op, t, o);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[2][q] = o[q];
+  const int b = t >> 2, r = t & 3;
+  const int b4 = r == 0 ? mfma_line_base<0>(b) : (r == 1 ? mfma_line_base<1>(b) : mfma_line_base<2>(b));
+  const int s4 = r == 0 ? 1 : (r == 1 ? 4 : 16);
+  mfma_f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int l = 0; l < 4; ++l) c = __builtin_amdgcn_mfma_f32_4x4x1f32(r < 3 ? F[4][b4 + s4 * l] : 0.f, op[l], c, 0, 0, 0);
+  out[0][4] = c[0];
+  out[1][4] = lane_fetch(mfma_src_y(t), c[1]);
+  out[2][4] = lane_fetch(mfma_src_z(t), c[2]);
+}
+
 // Occupancy requested for the gradient pass: 5 waves per SIMD for FP64 3-D
 // p = 3 on periodic meshes (at most 96 VGPRs; the LDS allows 5.5), where
-// the allocator would otherwise take 116 and run 4; elsewhere the default.
-// With the boundary code it would spill.
+// the allocator would otherwise take 116 and run 4; 8 for its FP32 form
+// with the matrix-core contractions (64 VGPRs, as without them); elsewhere
+// the default.  With the boundary code either would spill.
 template <typename R, int DIM, int P, int EPBX, bool BND>
 struct GradWaves {
   static constexpr bool fits = sizeof(R) == sizeof(double) && DIM == 3 && P == 3 && EPBX == 0 && !BND;
-  static constexpr int value = fits ? 5 : 1;
+  static constexpr int value = fits ? 5 : (GradMfma<R, DIM, P, EPBX>::on && !BND ? 8 : 1);  // 64 VGPRs
 };
 
 template <typename R, int DIM, int P, int EPBX = 0, bool BND = false>
@@ -1244,24 +1319,38 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND>::value))) k_n
   R hv[NVD];  // the own side's share -0.5 n.Fv of the central viscous face flux, lifted
 #pragma unroll
   for (int c = 0; c < NVD; ++c) hv[c] = R(0);
+  constexpr bool MF = GradMfma<R, DIM, P, EPBX>::on && !BND;  // with the boundary code it would cost a wave
+  // the line contractions D.S of all directions on the matrix cores (every lane of the wave)
+  float drefm[MF ? 3 : 1][MF ? 5 : 1];
+  float opD[4];  // matrix-core operand: row node%4 of D (then of Dhat)
+  if constexpr (MF) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) opD[l] = sD[(node & 3) * 4 + l];
+    line_contract_mfma5(reinterpret_cast<const float (*)[64]>(sS[elc]), opD, node, drefm);
+  }
   if (valid) {
     // volume gradient (AssembleGradVolumePointKernel): acc[dd] = sum_ref (D.S)_ref adj(ref,dd),
     // one reference direction at a time, in the reference's ref = 0,1,2 order
     R acc[DIM][NQ];
-#pragma unroll 1
+#pragma unroll MF ? DIM : 1
     for (int dir = 0; dir < DIM; ++dir) {
       R dref[NQ];
+      if constexpr (MF) {
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) dref[c] = R(0);
-      const int mi = dir == 0 ? i : (dir == 1 ? j : k);
-      const int st = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
-      const int base = node - mi * st;
+        for (int c = 0; c < NQ; ++c) dref[c] = drefm[MF ? dir : 0][MF ? c : 0];
+      } else {
 #pragma unroll
-      for (int l = 0; l < Np; ++l) {
-        const R coef = sD[mi * Np + l];
-        const int sn_ = base + l * st;
+        for (int c = 0; c < NQ; ++c) dref[c] = R(0);
+        const int mi = dir == 0 ? i : (dir == 1 ? j : k);
+        const int st = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
+        const int base = node - mi * st;
 #pragma unroll
-        for (int c = 0; c < NQ; ++c) dref[c] += sS[el][c][sn_] * coef;
+        for (int l = 0; l < Np; ++l) {
+          const R coef = sD[mi * Np + l];
+          const int sn_ = base + l * st;
+#pragma unroll
+          for (int c = 0; c < NQ; ++c) dref[c] += sS[el][c][sn_] * coef;
+        }
       }
 #pragma unroll
       for (int dd = 0; dd < DIM; ++dd) {
@@ -1357,20 +1446,39 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND>::value))) k_n
     }
   }
   lds_sync();
+  // the divergence's line contractions on the matrix cores, one direction at
+  // a time (then summed in x, y, z order)
+  float dvm[MF ? 3 : 1][MF ? 4 : 1];
+  if constexpr (MF) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) opD[l] = sDh[(node & 3) * 4 + l];
+    const float(*G)[4][64] = reinterpret_cast<const float(*)[4][64]>(sG[elc]);
+    float gx[4], gy[4], gz[4];
+    line_contract_dir<0>(G[0], opD, node, gx);
+    line_contract_dir<1>(G[1], opD, node, gy);
+    line_contract_dir<2>(G[2], opD, node, gz);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dvm[0][c] = gx[c] + gy[c] + gz[c];
+  }
   if (valid) {
     R dv[NVD];
+    if constexpr (MF) {
 #pragma unroll
-    for (int c = 0; c < NVD; ++c) dv[c] = R(0);
+      for (int c = 0; c < NVD; ++c) dv[c] = dvm[0][MF ? c : 0];
+    } else {
 #pragma unroll
-    for (int dir = 0; dir < DIM; ++dir) {
-      const int mi = dir == 0 ? i : (dir == 1 ? j : k);
-      const int st = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
-      const int base = node - mi * st;
+      for (int c = 0; c < NVD; ++c) dv[c] = R(0);
 #pragma unroll
-      for (int l = 0; l < Np; ++l) {
-        const R coef = sDh[mi * Np + l];
+      for (int dir = 0; dir < DIM; ++dir) {
+        const int mi = dir == 0 ? i : (dir == 1 ? j : k);
+        const int st = dir == 0 ? 1 : (dir == 1 ? Np : Np * Np);
+        const int base = node - mi * st;
 #pragma unroll
-        for (int c = 0; c < NVD; ++c) dv[c] += coef * sG[el][dir][c][base + l * st];
+        for (int l = 0; l < Np; ++l) {
+          const R coef = sDh[mi * Np + l];
+#pragma unroll
+          for (int c = 0; c < NVD; ++c) dv[c] += coef * sG[el][dir][c][base + l * st];
+        }
       }
     }
 #pragma unroll
