@@ -565,6 +565,36 @@ __device__ __forceinline__ void face_llf(const GasC<R> &g, const Pt<R, DIM> &a, 
   for (int e = 0; e < DIM + 2; ++e) F[e] = R(0.5) * (f1[e] + f2[e]) - R(0.5) * lambda_max * (b.q[e] - a.q[e]);
 }
 
+// face_llf(a, b) evaluated from (own, oth) without ordering the operands:
+// F(a, b) = (f(a) + f(b))/2 - (lambda/2)(q_b - q_a) with lambda symmetric, so
+// with sg = +1 when own is a (the low side) and -1 when it is b,
+// F = (f(own) + f(oth))/2 - (sg lambda/2)(q_oth - q_own): the sum commutes
+// and the sign changes are exact, so the two elements of a face get the
+// same bits, as with ordered operands.  Explicit fused multiply-adds fix the
+// rounding on both sides.
+template <typename R, int DIM>
+__device__ __forceinline__ void face_llf_signed(const GasC<R> &g, const Pt<R, DIM> &own, const Pt<R, DIM> &oth,
+                                                const R *nor, R sg, R *F) {
+  R vd1 = R(0), vd2 = R(0), nn = R(0);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    vd1 += own.v[d] * nor[d];
+    vd2 += oth.v[d] * nor[d];
+    nn += nor[d] * nor[d];
+  }
+  const R nrm = dsqrt(nn);
+  const R lambda_max = dmax(dabs(vd1) + sound_speed(g, own) * nrm, dabs(vd2) + sound_speed(g, oth) * nrm);
+  const R hl = sg * (R(0.5) * lambda_max);
+  R f1[DIM + 2], f2[DIM + 2];
+  normal_flux(own, nor, f1);
+  normal_flux(oth, nor, f2);
+#pragma unroll
+  for (int e = 0; e < DIM + 2; ++e) {
+    const R avg = R(0.5) * (f1[e] + f2[e]);
+    F[e] = std::fma(-hl, oth.q[e] - own.q[e], avg);
+  }
+}
+
 // Compile-time flux selection (ids of include/theseus_rhs.h)
 template <int VOL, typename R, int DIM>
 __device__ __forceinline__ void volume_flux(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b, const R *met,

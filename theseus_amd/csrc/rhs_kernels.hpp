@@ -914,10 +914,17 @@ __device__ __forceinline__ void rhs_body(const RHSCore &a, const BasisArgs &B, c
             oth.p = R(o64.p);
             oth.pd = R(o64.p - sprefe[FP64 ? 0 : pe]);
           }
-          Pt<R, DIM> pl, pr;
-          pt_select(s != 0, own, oth, pl);
-          pt_select(s != 0, oth, own, pr);
-          surface_flux<SURF, R, DIM>(g, pl, pr, n, F);
+          // Lax-Friedrichs without ordering the operands (face_llf_signed):
+          // 78 selects fewer per lane; the FP64 Euler kernel keeps the
+          // ordered form, with which its 4-wave register budget holds
+          if constexpr (SURF == 5 && (VISC || !FP64)) {
+            face_llf_signed<R, DIM>(g, own, oth, n, s != 0 ? R(1) : R(-1), F);
+          } else {
+            Pt<R, DIM> pl, pr;
+            pt_select(s != 0, own, oth, pl);
+            pt_select(s != 0, oth, own, pr);
+            surface_flux<SURF, R, DIM>(g, pl, pr, n, F);
+          }
         } else {
           double qx[NQ];  // FP64 exterior state: the neighbour's trace, or the boundary's
 #pragma unroll
