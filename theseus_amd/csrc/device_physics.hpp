@@ -43,6 +43,15 @@ template <typename R>
 __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 template <>
 __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
+// Square root of a positive, finite argument (sound speeds, face area
+// magnitudes): the hardware v_sqrt_f32 (1 ulp) in FP32, one instruction for
+// the ~10 of the correctly rounded form; FP64 keeps IEEE sqrt (a
+// reciprocal-square-root + Newton form cost the FP64 kernel a scratch spill
+// at its 4-wave register budget).
+template <typename R>
+__device__ __forceinline__ R dsqrt_pos(R x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float dsqrt_pos<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <typename R>
 __device__ __forceinline__ R dlog(R x) { return ln_pos(x); }
 template <>
@@ -132,7 +141,7 @@ __device__ __forceinline__ void derive(const GasC<R> &g, Pt<R, DIM> &s) {
 
 template <typename R, int DIM>
 __device__ __forceinline__ R sound_speed(const GasC<R> &g, const Pt<R, DIM> &s) {
-  return dsqrt(g.gamma * s.p * s.ir);
+  return dsqrt_pos(g.gamma * s.p * s.ir);
 }
 
 // ---------------------------------------------------------------------------
@@ -572,17 +581,16 @@ __device__ __forceinline__ void face_llf(const GasC<R> &g, const Pt<R, DIM> &a, 
 // and the sign changes are exact, so the two elements of a face get the
 // same bits, as with ordered operands.  Explicit fused multiply-adds fix the
 // rounding on both sides.
+// nrm = |nor|, computed once per face by the caller.
 template <typename R, int DIM>
 __device__ __forceinline__ void face_llf_signed(const GasC<R> &g, const Pt<R, DIM> &own, const Pt<R, DIM> &oth,
-                                                const R *nor, R sg, R *F) {
-  R vd1 = R(0), vd2 = R(0), nn = R(0);
+                                                const R *nor, R nrm, R sg, R *F) {
+  R vd1 = R(0), vd2 = R(0);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     vd1 += own.v[d] * nor[d];
     vd2 += oth.v[d] * nor[d];
-    nn += nor[d] * nor[d];
   }
-  const R nrm = dsqrt(nn);
   const R lambda_max = dmax(dabs(vd1) + sound_speed(g, own) * nrm, dabs(vd2) + sound_speed(g, oth) * nrm);
   const R hl = sg * (R(0.5) * lambda_max);
   R f1[DIM + 2], f2[DIM + 2];

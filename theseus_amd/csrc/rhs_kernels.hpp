@@ -918,7 +918,10 @@ __device__ __forceinline__ void rhs_body(const RHSCore &a, const BasisArgs &B, c
           // 78 selects fewer per lane; the FP64 Euler kernel keeps the
           // ordered form, with which its 4-wave register budget holds
           if constexpr (SURF == 5 && (VISC || !FP64)) {
-            face_llf_signed<R, DIM>(g, own, oth, n, s != 0 ? R(1) : R(-1), F);
+            R nn = R(0);
+#pragma unroll
+            for (int dd = 0; dd < DIM; ++dd) nn += n[dd] * n[dd];
+            face_llf_signed<R, DIM>(g, own, oth, n, dsqrt_pos(nn), s != 0 ? R(1) : R(-1), F);
           } else {
             Pt<R, DIM> pl, pr;
             pt_select(s != 0, own, oth, pl);
