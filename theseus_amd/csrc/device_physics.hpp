@@ -39,19 +39,14 @@ struct Pt {
 // pressure fluctuation of a low-Mach flow from drowning in the rounding of
 // a pressure 1/(gamma Ma^2) times larger.
 
+// Square root: IEEE sqrt in FP64; in the mixed mode's FP32 the hardware
+// v_sqrt_f32 (1 ulp), one instruction for the ~10 of the correctly rounded
+// form.  (A reciprocal-square-root + Newton form in FP64 cost the FP64
+// kernel a scratch spill at its 4-wave register budget.)
 template <typename R>
 __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 template <>
-__device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
-// Square root of a positive, finite argument (sound speeds, face area
-// magnitudes): the hardware v_sqrt_f32 (1 ulp) in FP32, one instruction for
-// the ~10 of the correctly rounded form; FP64 keeps IEEE sqrt (a
-// reciprocal-square-root + Newton form cost the FP64 kernel a scratch spill
-// at its 4-wave register budget).
-template <typename R>
-__device__ __forceinline__ R dsqrt_pos(R x) { return sqrt(x); }
-template <>
-__device__ __forceinline__ float dsqrt_pos<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <typename R>
 __device__ __forceinline__ R dlog(R x) { return ln_pos(x); }
 template <>
@@ -141,7 +136,7 @@ __device__ __forceinline__ void derive(const GasC<R> &g, Pt<R, DIM> &s) {
 
 template <typename R, int DIM>
 __device__ __forceinline__ R sound_speed(const GasC<R> &g, const Pt<R, DIM> &s) {
-  return dsqrt_pos(g.gamma * s.p * s.ir);
+  return dsqrt(g.gamma * s.p * s.ir);
 }
 
 // ---------------------------------------------------------------------------

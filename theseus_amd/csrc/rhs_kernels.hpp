@@ -828,12 +828,13 @@ __device__ __forceinline__ void rhs_body(const RHSCore &a, const BasisArgs &B, c
       if (valid) {
         const int L = face_index<DIM, Np>(dir, i, j, k);
         const int pos = dir == 0 ? i : (dir == 1 ? j : k);
-        const R den = R(Je * B.w[pos]);
+        // one reciprocal per direction instead of a division per field
+        const R rden = drcp(R(Je * B.w[pos]));
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           const R fm = pos > 0 ? sscr(el * NI + L * (Np - 1) + pos - 1, q) : R(0);
           const R fp = pos < Np - 1 ? sscr(el * NI + L * (Np - 1) + pos, q) : R(0);
-          const R val = (fm - fp) / den;
+          const R val = (fm - fp) * rden;
           dfv[q] = dir == 0 ? val : dfv[q] + val;
         }
       }
@@ -921,7 +922,7 @@ __device__ __forceinline__ void rhs_body(const RHSCore &a, const BasisArgs &B, c
             R nn = R(0);
 #pragma unroll
             for (int dd = 0; dd < DIM; ++dd) nn += n[dd] * n[dd];
-            face_llf_signed<R, DIM>(g, own, oth, n, dsqrt_pos(nn), s != 0 ? R(1) : R(-1), F);
+            face_llf_signed<R, DIM>(g, own, oth, n, dsqrt(nn), s != 0 ? R(1) : R(-1), F);
           } else {
             Pt<R, DIM> pl, pr;
             pt_select(s != 0, own, oth, pl);
