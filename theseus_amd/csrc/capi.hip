@@ -236,6 +236,21 @@ __global__ void __launch_bounds__(256) k_measures_final(const double *partial, i
   if (tid < NMEAS) out[tid] = red[0][tid];
 }
 
+// rates[q] = max over the slots of rate q (one 64-lane workgroup).
+__global__ void __launch_bounds__(64) k_rates_fold(unsigned long long *rates) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    unsigned long long v = t < kRateSlots ? rates[3 + q * kRateSlots + t] : 0ull;
+    // non-negative doubles order like their bit patterns
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(v, off, 64);
+      v = o > v ? o : v;
+    }
+    if (t == 0) rates[q] = v;
+  }
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0ull;
@@ -701,7 +716,7 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
   if ((rc = upload(&h->d_J, d.elem_jac, ne)) || (rc = upload(&h->d_adj, d.elem_adj, (size_t)ne * dim * dim)) ||
       (rc = upload(&h->d_nbr, d.face_neighbor, (size_t)ne * 2 * dim)) ||
       (rc = upload_bc(&h->d_bc_type, &h->d_bc_data, d)) || (rc = upload<double>(&h->d_alpha, nullptr, ne)) ||
-      (rc = upload<unsigned long long>(&h->d_rates, nullptr, 4)) || (rc = upload<double>(&h->d_pref, nullptr, 1)) ||
+      (rc = upload<unsigned long long>(&h->d_rates, nullptr, kRateWords)) || (rc = upload<double>(&h->d_pref, nullptr, 1)) ||
       (rc = upload<double>(&h->d_stage[0], nullptr, M)) || (rc = upload<double>(&h->d_stage[1], nullptr, M))) {
     theseus_rhs_destroy(h);
     return rc;
@@ -896,7 +911,7 @@ static int enqueue_ssprk3(theseus_rhs *h, double *u, double dt, const double *dt
 
 // Zeroes the rate slots and launches the stability kernel (no host copy).
 static void enqueue_stability(theseus_rhs *h, const double *u, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, h->d_rates, 3);
+  hipLaunchKernelGGL(k_zero_u64, dim3((kRateWords + 63) / 64), dim3(64), 0, s, h->d_rates, kRateWords);
   StabArgs a;
   std::memset(&a, 0, sizeof(a));
   a.U = u;
@@ -916,6 +931,7 @@ static void enqueue_stability(theseus_rhs *h, const double *u, hipStream_t s) {
   a.w0 = h->basis.w[0];
   a.lte = h->ltev;
   h->sk->stability(a, h->nblocks(), s);
+  hipLaunchKernelGGL(k_rates_fold, dim3(1), dim3(64), 0, s, h->d_rates);
 }
 
 // The variable-step decision of src/Simulation.cpp:1136-1158,1240-1301 and
@@ -1686,7 +1702,7 @@ int theseus_pool_create(const theseus_pool_desc *desc, theseus_pool **out) {
       (rc = upload<double>(&p->d_bc_data, nullptr, (size_t)d.max_cases * THESEUS_POOL_MAX_BC * THESEUS_BC_DATA_STRIDE)) ||
       (rc = upload<double>(&p->d_case_dt, nullptr, d.max_cases)) ||
       (rc = upload<double>(&p->d_case_pref, nullptr, d.max_cases)) ||
-      (rc = upload<unsigned long long>(&p->d_rates, nullptr, 4)) || (rc = upload<double>(&p->d_U, nullptr, M)) ||
+      (rc = upload<unsigned long long>(&p->d_rates, nullptr, kRateWords)) || (rc = upload<double>(&p->d_U, nullptr, M)) ||
       (rc = upload<double>(&p->d_stage[0], nullptr, M)) || (rc = upload<double>(&p->d_stage[1], nullptr, M))) {
     theseus_pool_destroy(p);
     return rc;
@@ -1852,7 +1868,7 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
   for (int q = 0; q < p->neq; ++q)
     HIP_TRY(hipMemcpyAsync(p->d_U + q * p->Ntot + off, u0 + q * nd, nd * sizeof(double), hipMemcpyDefault, s));
   // stability estimate of the case alone (its element range), then dt and the background pressure
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p->d_rates, 3);
+  hipLaunchKernelGGL(k_zero_u64, dim3((kRateWords + 63) / 64), dim3(64), 0, s, p->d_rates, kRateWords);
   StabArgs a;
   std::memset(&a, 0, sizeof(a));
   a.U = p->d_U;
@@ -1872,6 +1888,7 @@ int theseus_pool_admit(theseus_pool *p, const theseus_case_desc *c, const double
   a.gas = p->gas;
   a.w0 = p->basis.w[0];
   p->sk->stability(a, (ne + p->sk->epb - 1) / p->sk->epb, s);
+  hipLaunchKernelGGL(k_rates_fold, dim3(1), dim3(64), 0, s, p->d_rates);
   if (dim == 2) hipLaunchKernelGGL(k_pref<2>, dim3(1), dim3(64), 0, s, p->d_U + off, p->Ntot, p->gas, p->d_case_pref + cid);
   else hipLaunchKernelGGL(k_pref<3>, dim3(1), dim3(64), 0, s, p->d_U + off, p->Ntot, p->gas, p->d_case_pref + cid);
   hipLaunchKernelGGL(k_case_dt, dim3(1), dim3(64), 0, s, p->d_rates, cfl, p->d_case_dt + cid);

@@ -1502,9 +1502,15 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND>::value))) k_n
 // rates reduced on the device; non-negative doubles are max-reduced through
 // their bit patterns with 64-bit integer atomics.
 // ---------------------------------------------------------------------------
+// Stability rates buffer: [0..2] the advective, surface and diffusive rate
+// (bit patterns of non-negative doubles), then kRateSlots partial maxima of
+// each, zeroed before and folded after each estimate.
+constexpr int kRateSlots = 64;
+constexpr int kRateWords = 3 + 3 * kRateSlots;
+
 struct StabArgs {
   const double *U;
-  unsigned long long *rates;  // [3] advective, surface, diffusive (bit patterns)
+  unsigned long long *rates;  // kRateWords: [3] results, then the slots (bit patterns)
   long long Ntot;
   int ne;             // elements estimated: [e0, e0 + ne)
   int e0;
@@ -1637,11 +1643,15 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
   surf = wave_max(surf);
   diff = wave_max(diff);
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(&a.rates[0], (unsigned long long)__double_as_longlong(adv));
-    atomicMax(&a.rates[1], (unsigned long long)__double_as_longlong(surf));
-    atomicMax(&a.rates[2], (unsigned long long)__double_as_longlong(diff));
+    // one of kRateSlots slots per rate (by block), so that the grid's
+    // atomics do not all queue on three addresses; k_rates_fold takes the max
+    unsigned long long *slot = a.rates + 3 + (blockIdx.x % kRateSlots);
+    atomicMax(slot, (unsigned long long)__double_as_longlong(adv));
+    atomicMax(slot + kRateSlots, (unsigned long long)__double_as_longlong(surf));
+    atomicMax(slot + 2 * kRateSlots, (unsigned long long)__double_as_longlong(diff));
   }
 }
+
 
 // ---------------------------------------------------------------------------
 // Integral measures (RHSOperator_impl.hpp:530-717): node-quadrature integrals
