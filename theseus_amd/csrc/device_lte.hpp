@@ -217,11 +217,6 @@ __device__ __forceinline__ double temperature_from_energy(const View &v, double 
   return __builtin_nan("");
 }
 
-// A scheduling fence: the compiler keeps the table evaluations before and
-// after it apart instead of interleaving all of them, which held so many
-// partial sums live that the pair flux spilled registers to scratch.
-__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
-
 enum class Q { pressure, gibbs };
 template <Q P>
 __device__ __forceinline__ double value(const Eval &s) {
@@ -233,17 +228,13 @@ __device__ __forceinline__ double density_secant(const Eval &a, const Eval &b, d
   const Cell &c = a.cell;
   const double r1 = a.rho, r2 = b.rho;
   const double h1 = evaluate<1, 0>(c, a.xi, eta);
-  sched_fence();
   const double h2 = evaluate<1, 0>(c, b.xi, eta);
-  sched_fence();
   const double mean_hr = 0.5 * (h1 + h2);
   const double div_hr = divided<0, 1, 0>(c, a.xi, eta, b.xi, eta);
-  sched_fence();
   if constexpr (P == Q::pressure) {
     return 0.5 * (r1 * r1 + r2 * r2) * div_hr + (r1 + r2) * mean_hr;
   } else {
     const double div_h = divided<0>(c, a.xi, eta, b.xi, eta);
-    sched_fence();
     return div_h + 0.5 * (r1 + r2) * div_hr + mean_hr;
   }
 }
@@ -252,23 +243,17 @@ __device__ __forceinline__ double beta_secant(const Eval &a, const Eval &b, doub
   const Cell &c = a.cell;
   const double mean_t = 0.5 * (a.T + b.T);
   const double h1 = evaluate<1, 0>(c, xi, a.eta);
-  sched_fence();
   const double h2 = evaluate<1, 0>(c, xi, b.eta);
-  sched_fence();
   const double mean_hr = 0.5 * (h1 + h2);
   const double div_hr = divided<1, 1, 0>(c, xi, a.eta, xi, b.eta);
-  sched_fence();
   const double hr_beta = mean_hr - mean_t * div_hr;
   if constexpr (P == Q::pressure) {
     return rho * rho * hr_beta;
   } else {
     const double g1 = evaluate<0, 0>(c, xi, a.eta);
-    sched_fence();
     const double g2 = evaluate<0, 0>(c, xi, b.eta);
-    sched_fence();
     const double mean_h = 0.5 * (g1 + g2);
     const double div_h = divided<1>(c, xi, a.eta, xi, b.eta);
-    sched_fence();
     return mean_h - mean_t * div_h + rho * hr_beta;
   }
 }
@@ -292,16 +277,12 @@ __device__ __forceinline__ Gradient discrete_gradient(const View &v, const Eval 
   if (same_cell) {
     if (dd) {
       const double s1 = density_secant<P>(a, b, a.eta);
-      sched_fence();
       const double s2 = density_secant<P>(a, b, b.eta);
-      sched_fence();
       g.density = 0.5 * (s1 * b1 + s2 * b2);
     }
     if (bd) {
       const double s1 = beta_secant<P>(a, b, a.xi, a.rho);
-      sched_fence();
       const double s2 = beta_secant<P>(a, b, b.xi, b.rho);
-      sched_fence();
       g.beta = 0.5 * (s1 + s2);
     }
   } else if (dd || bd) {
@@ -335,9 +316,7 @@ __device__ __forceinline__ Gradient discrete_gradient(const View &v, const Eval 
 __device__ __forceinline__ void thermodynamic_means(const View &v, const Eval &l, const Eval &r, double &rho_mean,
                                                     double &e_mean) {
   const Gradient p = discrete_gradient<Q::pressure>(v, l, r);
-  sched_fence();
   const Gradient g = discrete_gradient<Q::gibbs>(v, l, r);
-  sched_fence();
   rho_mean = p.density / g.density;
   e_mean = g.beta - p.beta / rho_mean;
 }
