@@ -196,10 +196,15 @@ typedef struct theseus_lte_table {
 } theseus_lte_table;
 
 /* MakeRHSOperator for the LTEHelmholtzGas with the LTEEC flux
- * (LTEGasModel.hpp:339, LTEECFlux.hpp:93-107): FP64 Euler, volume_flux =
- * THESEUS_VOLUME_LTEEC, surface_flux = THESEUS_SURFACE_LTEEC.  A state
- * outside the table (or whose temperature recovery does not converge)
- * yields non-finite values instead of the reference's trap. */
+ * (LTEGasModel.hpp:339, LTEECFlux.hpp:93-107): FP64, volume_flux =
+ * THESEUS_VOLUME_LTEEC, surface_flux = THESEUS_SURFACE_LTEEC.  Euler, or
+ * Navier-Stokes (flow_model THESEUS_FLOW_NAVIER_STOKES, periodic meshes):
+ * BR1 on the gas's entropy variables ((g - |u|^2/2)/T, u/T, -1/T), viscosity
+ * and conductivity bilinear from the table's mu and lambda planes
+ * (LTETransport.hpp:32-65, the build without SUTHERLAND), bulk-viscosity
+ * ratio mu_bulk from the descriptor; the descriptor's mu and Pr are unused.
+ * A state outside the table (or whose temperature recovery does not
+ * converge) yields non-finite values instead of the reference's trap. */
 THESEUS_API int theseus_rhs_create_lte(const theseus_case_desc *desc, const theseus_lte_table *table,
                                        theseus_rhs **out);
 

@@ -257,6 +257,45 @@ int oracle_lte_flux(void *h, int face, int dim, const double *q1, const double *
   }
 }
 
+// The tabulated gas at one conservative state q (oracle_lte.hpp ViscousGas):
+// out = T, p, mu, lambda, cp, cv, the entropy variables [dim+2], and the
+// primitive gradient [dim+2] that the entropy-variable gradient dS converts to.
+int oracle_lte_gas_point(void *h, int dim, double mu_bulk, const double *q, const double *dS, double *out) {
+  try {
+    const lte::ViscousGas g{static_cast<lte::Table *>(h), dim, mu_bulk};
+    const lte::Eval s = g.state(q);
+    const int neq = dim + 2;
+    out[0] = s.T;
+    out[1] = s.pressure();
+    out[2] = g.viscosity(q);
+    out[3] = g.thermal_conductivity(q);
+    out[4] = s.cp();
+    out[5] = s.cv();
+    g.entropy_state(q, out + 6);
+    g.grad_entropy_to_grad_prim(q, dS, out + 6 + neq);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// NavierStokesFlux.hpp:160-290 with the tabulated gas; F as oracle_viscous_flux
+int oracle_lte_viscous_flux(void *h, int dim, double mu_bulk, const double *q, const double *dx, const double *dy,
+                            const double *dz, double *F) {
+  try {
+    const lte::ViscousGas g{static_cast<lte::Table *>(h), dim, mu_bulk};
+    double f[MAXEQ][MAXDIM] = {{0}};
+    ComputeViscousFluxKernel(g, q, dx, dy, dz, f);
+    for (int e = 0; e < MAXEQ; ++e)
+      for (int d = 0; d < MAXDIM; ++d) F[e * MAXDIM + d] = f[e][d];
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
 // The case's gas becomes the table (kept alive by the caller) with the LTEEC flux.
 void oracle_case_set_lte(void *c, void *t) { static_cast<Case *>(c)->lte = static_cast<lte::Table *>(t); }
 

@@ -141,6 +141,11 @@ struct Eval {
   double gibbs_density_derivative() const { return 2 * H<1, 0>() + rho * H<2, 0>(); }
   double gibbs_temperature_derivative() const { return H<0, 1>() + rho * H<1, 1>(); }
   double cv() const { return -T * H<0, 2>(); }
+  // LTEHelmholtzEOS.hpp:76-80
+  double cp() const {
+    const double m = H<1, 1>();
+    return cv() + T * rho * rho * m * m / pressure_density_derivative();
+  }
   double sound_speed_squared() const {
     const double m = H<1, 1>();
     return pressure_density_derivative() - rho * rho * m * m / H<0, 2>();
@@ -317,6 +322,54 @@ inline Eval evaluate_state(const Table &t, const double *q, int dim) {
   if (!s.valid) throw std::runtime_error("Helmholtz state outside the table");
   return s;
 }
+
+// LTEHelmholtzEOS.hpp:99-110: viscosity (plane 4) or thermal conductivity
+// (plane 5), bilinear in the evaluated cell's coordinates
+inline double transport(const Table &t, const Eval &s, int prop) {
+  const double *p = t.planes.data() + t.index(prop, s.cell.i, s.cell.j);
+  return p[0] * (1 - s.xi) * (1 - s.eta) + p[1] * s.xi * (1 - s.eta) + p[t.nx] * (1 - s.xi) * s.eta +
+         p[t.nx + 1] * s.xi * s.eta;
+}
+
+// The tabulated gas on the viscous path, LTEGasModel<LTEHelmholtzGasEOS,
+// LTETransport> (LTEGasModel.hpp:339): viscosity and conductivity from the
+// table (LTETransport.hpp:32-65, the build without SUTHERLAND), the constant
+// bulk-viscosity ratio, the entropy variables ((g - |u|^2/2)/T, u/T, -1/T)
+// and their gradient conversion (LTEHelmholtzGasEOS.hpp:133-171).  The same
+// members as the ideal gas, so the Navier-Stokes assembly takes either.
+struct ViscousGas {
+  const Table *t = nullptr;
+  int dim = 3;
+  double mu_bulk = 2.0 / 3.0;
+
+  Eval state(const double *q) const { return evaluate_state(*t, q, dim); }
+  double velocity(const double *q, int d) const { return q[1 + d] / q[0]; }
+  double viscosity(const double *q) const { return transport(*t, state(q), 4); }
+  double thermal_conductivity(const double *q) const { return transport(*t, state(q), 5); }
+  double bulk_viscosity(const double *) const { return mu_bulk; }
+  // LTEHelmholtzGasEOS.hpp:133-142: the energy slot of the primitive gradient is grad T
+  void grad_temperature(const double *, const double *, const double *grad_p, double *grad_t) const {
+    for (int d = 0; d < dim; ++d) grad_t[d] = grad_p[d];
+  }
+  // LTEHelmholtzGasEOS.hpp:144-157 (kinetic energy density / rho, :40-50)
+  void entropy_state(const double *q, double *E) const {
+    const Eval s = state(q);
+    const double beta = 1 / s.T;
+    double m2 = 0;
+    for (int d = 0; d < dim; ++d) m2 += q[1 + d] * q[1 + d];
+    const double kinetic = 0.5 * m2 / q[0] / q[0];
+    E[0] = beta * (s.gibbs() - kinetic);
+    for (int d = 0; d < dim; ++d) E[1 + d] = beta * velocity(q, d);
+    E[dim + 1] = -beta;
+  }
+  // LTEHelmholtzGasEOS.hpp:159-171
+  void grad_entropy_to_grad_prim(const double *q, const double *dE, double *dP) const {
+    const double T = state(q).T;
+    dP[0] = 0.0;
+    for (int d = 0; d < dim; ++d) dP[1 + d] = T * dE[1 + d] + T * velocity(q, d) * dE[dim + 1];
+    dP[dim + 1] = T * T * dE[dim + 1];
+  }
+};
 
 // LTEECFlux.hpp:11-85 (volume: mean metric row; face: the normal, with
 // Lax-Friedrichs dissipation, DissipationOperator.hpp:15-28)

@@ -613,7 +613,8 @@ inline void EulerRHS(const Case &c, const double *U, double *dudt, const double 
 // ---------------------------------------------------------------------------
 // BR1 Navier-Stokes operator (NSOperator_impl.hpp:10-1105).
 // ---------------------------------------------------------------------------
-inline void NSRHS(const Case &c, const double *U, double *dudt, const double *alpha) {
+template <class G>
+inline void NSRHS(const Case &c, const G &gas, const double *U, double *dudt, const double *alpha) {
   const size_t N = c.ntot();
   const int neq = c.neq, dim = c.dim, dof = c.ndof, Np = c.Np;
   const double w0 = c.basis.w[0];
@@ -629,7 +630,7 @@ inline void NSRHS(const Case &c, const double *U, double *dudt, const double *al
   for (long pt = 0; pt < (long)N; ++pt) {
     double q[MAXEQ], s[MAXEQ];
     for (int k = 0; k < neq; ++k) q[k] = U[k * N + pt];
-    c.gas.entropy_state(q, s);
+    gas.entropy_state(q, s);
     for (int k = 0; k < neq; ++k) S[k * N + pt] = s[k];
   }
   // GradOperator (NSOperator_impl.hpp:194-577)
@@ -725,7 +726,7 @@ inline void NSRHS(const Case &c, const double *U, double *dudt, const double *al
         q[k] = U[k * N + pt];
         dS[k] = grad[dd][k * N + pt];
       }
-      c.gas.grad_entropy_to_grad_prim(q, dS, dP);
+      gas.grad_entropy_to_grad_prim(q, dS, dP);
       for (int k = 0; k < neq; ++k) grad[dd][k * N + pt] = dP[k];
     }
   // MultCNS_Volume (NSOperator_impl.hpp:844-1073)
@@ -743,7 +744,7 @@ inline void NSRHS(const Case &c, const double *U, double *dudt, const double *al
           q[k] = U[k * N + eb + pt];
           for (int dd = 0; dd < dim; ++dd) dq[dd][k] = grad[dd][k * N + eb + pt];
         }
-        ComputeViscousFluxKernel(c.gas, q, dq[0], dq[1], dq[2], pf);
+        ComputeViscousFluxKernel(gas, q, dq[0], dq[1], dq[2], pf);
         for (int dir = 0; dir < dim; ++dir) {
           const double *row = c.adj_row(e, dir);
           for (int qq = 0; qq < neq; ++qq) {
@@ -801,8 +802,8 @@ inline void NSRHS(const Case &c, const double *U, double *dudt, const double *al
         }
       }
       CaseFaceFlux(c, qm, qp, n, pf);
-      ComputeViscousFluxKernel(c.gas, qm, gm[0], gm[1], gm[2], vm);
-      ComputeViscousFluxKernel(c.gas, qp, gp[0], gp[1], gp[2], vp);
+      ComputeViscousFluxKernel(gas, qm, gm[0], gm[1], gm[2], vm);
+      ComputeViscousFluxKernel(gas, qp, gp[0], gp[1], gp[2], vp);
       for (int k = 0; k < neq; ++k)
         for (int dd = 0; dd < dim; ++dd) {
           const double avg = 0.5 * (vm[k][dd] + vp[k][dd]);
@@ -851,7 +852,15 @@ inline void RHS(const Case &c, const double *U, double *dudt, std::vector<double
     BlendingCoefficient(c, U, alpha.data(), nullptr);
     ap = alpha.data();
   }
-  if (c.flow == THESEUS_FLOW_NAVIER_STOKES) NSRHS(c, U, dudt, ap);
+  if (c.flow == THESEUS_FLOW_NAVIER_STOKES) {
+    if (c.lte) {
+      for (int nb : c.nbr)
+        if (nb < 0) throw std::runtime_error("the tabulated gas with Navier-Stokes needs a periodic mesh");
+      NSRHS(c, lte::ViscousGas{c.lte, c.dim, c.gas.mu_bulk}, U, dudt, ap);
+    } else {
+      NSRHS(c, c.gas, U, dudt, ap);
+    }
+  }
   else EulerRHS(c, U, dudt, ap);
   if (alpha_out) *alpha_out = alpha;
 }
@@ -883,10 +892,22 @@ inline void EstimateStability(const Case &c, const double *U, double out[3]) {
       if (viscous) {
         // StabilityEstimate.hpp:23-38
         const double density = q[0];
-        const double shear = c.gas.viscosity(q);
+        double shear, conductivity, gamma, cp;
+        if (c.lte) {  // LTETransport.hpp:32-65; gamma = cp/cv of the table (LTEHelmholtzEOS.hpp:71-91)
+          const lte::Eval s = lte::evaluate_state(*c.lte, q, dim);
+          shear = lte::transport(*c.lte, s, 4);
+          conductivity = lte::transport(*c.lte, s, 5);
+          cp = s.cp();
+          gamma = cp / s.cv();
+        } else {
+          shear = c.gas.viscosity(q);
+          conductivity = c.gas.thermal_conductivity(q);
+          gamma = c.gas.gamma;
+          cp = c.gas.cp;
+        }
         const double longitudinal = (2.0 - c.gas.bulk_viscosity(q)) * shear;
         const double momentum_diffusivity = std::max(shear, longitudinal) / density;
-        const double thermal_diffusivity = c.gas.thermal_conductivity(q) * c.gas.gamma / (density * c.gas.cp);
+        const double thermal_diffusivity = conductivity * gamma / (density * cp);
         const double eff = std::max(momentum_diffusivity, thermal_diffusivity);
         diff = std::max(diff, c.diff_scale * eff * metric_square_sum);
       }

@@ -187,8 +187,10 @@ inline void ComputeInviscidFluxKernel(const Gas &g, const double *q, double F[MA
   }
 }
 
-// NavierStokesFlux.hpp:389-519 (Cartesian: axisymmetric=false)
-inline void ComputeViscousFluxKernel(const Gas &g, const double *q, const double *dx, const double *dy,
+// NavierStokesFlux.hpp:160-290 (Cartesian: axisymmetric=false); G is the
+// ideal gas or the tabulated one (oracle_lte.hpp ViscousGas)
+template <class G>
+inline void ComputeViscousFluxKernel(const G &g, const double *q, const double *dx, const double *dy,
                                      const double *dz, double vf[MAXEQ][MAXDIM]) {
   const int dim = g.dim;
   for (int qq = 0; qq < MAXEQ; qq++)

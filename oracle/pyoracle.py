@@ -276,6 +276,10 @@ def _lte_sigs(lib):
     lib.oracle_lte_flux.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
     lib.oracle_lte_flux.restype = C.c_int
     lib.oracle_case_set_lte.argtypes = [vp, vp]
+    lib.oracle_lte_gas_point.argtypes = [vp, C.c_int, C.c_double, _dp, _dp, _dp]
+    lib.oracle_lte_gas_point.restype = C.c_int
+    lib.oracle_lte_viscous_flux.argtypes = [vp, C.c_int, C.c_double] + [_dp] * 5
+    lib.oracle_lte_viscous_flux.restype = C.c_int
     lib._lte_ready = True
     return lib
 
@@ -331,6 +335,31 @@ class LteTable:
         if self._lib.oracle_lte_flux(self._h, int(face), dim, _p(q1), _p(q2), _p(d1), _p(d2), _p(F)) != 0:
             raise RuntimeError(self._lib.oracle_last_error().decode())
         return F
+
+    def gas_point(self, dim, q, dS=None, mu_bulk=2.0 / 3.0):
+        """The tabulated gas at the conservative state q: dict of T, p, mu,
+        lambda, cp, cv, the entropy variables S and the primitive gradient dP
+        that the entropy-variable gradient dS converts to."""
+        neq = dim + 2
+        q = arr(q)
+        dS = np.zeros(neq) if dS is None else arr(dS)
+        out = np.zeros(6 + 2 * neq)
+        if self._lib.oracle_lte_gas_point(self._h, dim, mu_bulk, _p(q), _p(dS), _p(out)) != 0:
+            raise RuntimeError(self._lib.oracle_last_error().decode())
+        keys = ("T", "p", "mu", "lambda", "cp", "cv")
+        r = {k: out[i] for i, k in enumerate(keys)}
+        r["S"] = out[6:6 + neq]
+        r["dP"] = out[6 + neq:]
+        return r
+
+    def viscous_flux(self, dim, q, dx, dy, dz, mu_bulk=2.0 / 3.0):
+        """Physical viscous flux [dim+2][dim] of the tabulated gas from the
+        primitive gradients (energy slot = grad T)."""
+        q, dx, dy, dz = arr(q), arr(dx), arr(dy), arr(dz)
+        F = np.zeros(15)
+        if self._lib.oracle_lte_viscous_flux(self._h, dim, mu_bulk, _p(q), _p(dx), _p(dy), _p(dz), _p(F)) != 0:
+            raise RuntimeError(self._lib.oracle_last_error().decode())
+        return F.reshape(5, 3)[: dim + 2, :dim]
 
     def attach(self, case: "OracleCase"):
         """Make the oracle case use this table with the LTEEC flux."""

@@ -205,18 +205,19 @@ def _lte_arrays(nx=4, ny=4):
     ("blending", A.ERR_UNSUPPORTED),
     ("ideal_flux", A.ERR_INVALID_ARGUMENT),
     ("bad_grid", A.ERR_INVALID_ARGUMENT),
-    ("ns", A.ERR_UNSUPPORTED),
+    ("ns_with_boundaries", A.ERR_UNSUPPORTED),
 ])
 def test_create_lte_rejects_invalid_inputs(mutate, code):
-    """theseus_rhs_create_lte validates before any device work."""
+    """theseus_rhs_create_lte validates before any device work; Navier-Stokes
+    with the tabulated gas is limited to periodic meshes."""
     lib = A.load()
     rt = RunTime(dim=3, order=3, flow_model=A.FLOW_EULER, volume_flux=A.VOLUME_LTEEC,
                  surface_flux=A.SURFACE_LTEEC, blending=mutate == "blending")
     if mutate == "ideal_flux":
         rt.volume_flux = A.VOLUME_RANOCHA
-    if mutate == "ns":
+    if mutate == "ns_with_boundaries":
         rt.flow_model = A.FLOW_NAVIER_STOKES
-    mesh = box_mesh(3, (2, 2, 2), 0.0, 1.0)
+    mesh = box_mesh(3, (2, 2, 2), 0.0, 1.0, periodic=mutate != "ns_with_boundaries")
     d, keep = build_descriptor(rt, mesh)
     arrays = [np.ascontiguousarray(a) for a in _lte_arrays()]
     if mutate == "bad_grid":
@@ -227,3 +228,5 @@ def test_create_lte_rejects_invalid_inputs(mutate, code):
     h = C.c_void_p()
     assert lib.theseus_rhs_create_lte(C.byref(d), C.byref(t), C.byref(h)) == code
     assert not h.value and lib.theseus_last_error().decode()
+    if mutate == "ns_with_boundaries":
+        assert "periodic" in lib.theseus_last_error().decode()

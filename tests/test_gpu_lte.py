@@ -84,3 +84,92 @@ def test_lte_stability_and_steps_parity(oracle, table):
     op.close()
     err = rel_l2(res, ref)
     assert err <= 1e-10, (err, per_field_rel(res, ref, 5))
+
+
+# ---------------------------------------------------------------------------
+# Navier-Stokes with the tabulated gas (BR1; viscosity and conductivity from
+# the table's mu and lambda planes, LTETransport.hpp:32-65)
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def viscous_table(table):
+    """The workload's table with viscosity and conductivity scaled by 1e3
+    (Reynolds number ~300 instead of ~3e5), so that the viscous terms are a
+    sizeable part of the right-hand side."""
+    import dataclasses
+
+    planes = table.planes.copy()
+    planes[4:6] *= 1e3
+    return dataclasses.replace(table, planes=planes)
+
+
+def _gpu_rhs(rt, mesh, table, U):
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+
+    op = MakeRHSOperator(rt, mesh, lte_table=table)
+    u = torch.from_numpy(U).cuda()
+    du = torch.empty_like(u)
+    op.Mult(u, du)
+    torch.cuda.synchronize()
+    got = du.cpu().numpy()
+    op.close()
+    return got
+
+
+@pytest.mark.parametrize("dim,n,order", [(3, 2, 3), (3, 3, 2), (3, 2, 4), (3, 5, 1), (2, 3, 3), (2, 4, 2), (2, 3, 4)])
+def test_lte_navier_stokes_rhs_parity(oracle, viscous_table, dim, n, order):
+    """Full RHS relative L2 <= 1e-12; the viscous part alone (Navier-Stokes
+    minus Euler, on both sides) within 1e-10 of its own norm."""
+    import dataclasses
+
+    from theseus_amd import _abi as A
+    from theseus_amd.cases import c4_lte_tgv
+
+    _say(f"case lte navier-stokes rhs dim={dim} n={n} p={order}")
+    rt, mesh, U, _ = c4_lte_tgv(n=n, order=order, table=viscous_table, viscous=True, dim=dim)
+    rt_eu = dataclasses.replace(rt, flow_model=A.FLOW_EULER)
+    ref = _oracle(oracle, rt, mesh, viscous_table).rhs(U)
+    ref_eu = _oracle(oracle, rt_eu, mesh, viscous_table).rhs(U)
+    got = _gpu_rhs(rt, mesh, viscous_table, U)
+    got_eu = _gpu_rhs(rt_eu, mesh, viscous_table, U)
+    assert np.isfinite(got).all()
+    err = rel_l2(got, ref)
+    assert err <= 1e-12, (err, per_field_rel(got, ref, dim + 2))
+    vis_ref, vis_got = ref - ref_eu, got - got_eu
+    assert np.abs(vis_ref).max() > 5e-4 * np.abs(ref).max()  # the viscous terms matter here
+    verr = rel_l2(vis_got, vis_ref)
+    assert verr <= 1e-10, (verr, per_field_rel(vis_got, vis_ref, dim + 2))
+
+
+def test_lte_navier_stokes_stability_and_steps_parity(oracle, viscous_table):
+    """The stability estimate (with the tabulated diffusivity) to 1e-12 and
+    the state after 20 SSP-RK3 steps to 1e-10."""
+    import torch
+
+    from theseus_amd.cases import c4_lte_tgv
+    from theseus_amd.operator import MakeRHSOperator
+
+    _say("case lte navier-stokes steps n=2 p=3")
+    rt, mesh, U, _ = c4_lte_tgv(n=2, order=3, table=viscous_table, viscous=True)
+    ora = _oracle(oracle, rt, mesh, viscous_table)
+    rates = ora.stability(U)
+    assert rates[2] > 0
+    op = MakeRHSOperator(rt, mesh, lte_table=viscous_table)
+    u = torch.from_numpy(U).cuda()
+    s = op.EstimateStability(u)
+    got = np.array([s.advective_rate, s.surface_rate, s.diffusive_rate])
+    np.testing.assert_allclose(got, rates, rtol=1e-12, atol=0)
+    dt = 0.5 / (max(rates[0], rates[1]) + rates[2])
+    ref = U.copy()
+    for _ in range(20):
+        ref = ora.step(ref, dt)
+    t = 0.0
+    for _ in range(20):
+        t = op.Step(u, t, dt)
+    torch.cuda.synchronize()
+    res = u.cpu().numpy()
+    op.close()
+    err = rel_l2(res, ref)
+    assert err <= 1e-10, (err, per_field_rel(res, ref, 5))

@@ -301,3 +301,202 @@ def test_lte_operator_conserves_entropy_in_the_volume(small_table):
         prod += Wall[nidx] * (wv @ r[:, nidx])
     scale = np.abs(Wall) @ np.abs(r).sum(axis=0)
     assert prod <= 1e-10 * scale, (prod, scale)
+
+
+# ---------------------------------------------------------------------------
+# The tabulated gas on the viscous (Navier-Stokes BR1) path
+# ---------------------------------------------------------------------------
+
+def test_lte_viscous_gas_matches_reference_known_answers():
+    """helmholtz_gas_tests.cpp:103-165 on the polynomial table
+    (helmholtz_fixture.hpp: mu = 2 + r + T, lambda = 4 + 2r + T): at
+    rho = 0.75 + 0.125k, T = 3 + 0.25k, u = (3, -2, 1) the recovered
+    temperature, the pressure, the tabulated viscosity and conductivity, the
+    entropy variables, and the primitive gradient of dS = (0, 1, 2, 3, 0.5)
+    = (0, 2.5T, ., ., 0.5T^2), all to 1e-10 (1e-12 for the density slot).
+    cp = cv + T rho^2 H_rhoT^2 / p_rho with cv = 2T, H_rhoT = 1/4 and
+    p_rho = 4r + 3r^2 + rT/2 closes the stability's gamma."""
+    t = polynomial_table()
+    for k in range(5):
+        r, T = 0.75 + 0.125 * k, 3 + 0.25 * k
+        q = np.array([r, 3 * r, -2 * r, r, r * (2 * r + r * r / 2 + T * T + 7)])
+        v = t.gas_point(3, q, dS=[0, 1, 2, 3, 0.5])
+
+        def near(x, want, tol=1e-10):
+            assert np.isfinite(x) and abs(x - want) <= tol * max(1.0, abs(want)), (k, x, want)
+
+        near(v["T"], T)
+        near(v["p"], r * r * (2 + r + T / 4))
+        near(v["mu"], 2 + r + T)
+        near(v["lambda"], 4 + 2 * r + T)
+        near(v["S"][0], (4 * r + 1.5 * r * r + r * T / 2 - T * T - 7) / T)
+        near(v["S"][1], 3 / T)
+        near(v["S"][2], -2 / T)
+        near(v["S"][4], -1 / T)
+        near(v["dP"][0], 0.0, 1e-12)
+        near(v["dP"][1], 2.5 * T)
+        near(v["dP"][2], T * (2 - 2 * 0.5))
+        near(v["dP"][3], T * (3 + 0.5))
+        near(v["dP"][4], 0.5 * T * T)
+        near(v["cv"], 2 * T)
+        near(v["cp"], 2 * T + T * r * r / 16 / (4 * r + 3 * r * r + r * T / 2))
+
+
+def test_lte_viscous_flux_equals_the_ideal_gas_flux_with_the_same_transport():
+    """NavierStokesFlux.hpp:160-290 takes grad T from the gas: the tabulated
+    gas passes the energy slot through (LTEHelmholtzGasEOS.hpp:133-142), the
+    ideal gas forms it from grad p and grad rho (EOS.hpp:159-173).  With the
+    ideal gas's conductivity mu cp/Pr set to the table's lambda and grad p =
+    rho R grad T + (p/rho) grad rho, both give the same stress and energy flux
+    (the ideal-gas flux is itself pinned by normal_viscous_face_tests.cpp)."""
+    from oracle import pyoracle as O
+
+    t = polynomial_table()
+    rng = np.random.default_rng(7)
+    gamma, R = 1.4, 1.0
+    cp = gamma / (gamma - 1) * R
+    for dim in (2, 3):
+        for _ in range(5):
+            r, T = rng.uniform(0.6, 1.9), rng.uniform(2.2, 4.8)
+            u = rng.uniform(-1, 1, dim)
+            q = np.zeros(dim + 2)
+            q[0] = r
+            q[1:dim + 1] = r * u
+            q[dim + 1] = r * (2 * r + r * r / 2 + T * T + 0.5 * u @ u)
+            v = t.gas_point(dim, q)
+            rows = [rng.uniform(-1, 1, dim + 2) for _ in range(dim)]
+            for row in rows:
+                row[0] = rng.uniform(-1, 1)  # a density gradient the tabulated path ignores
+            lte_rows = [row.copy() for row in rows]
+            for row in lte_rows:
+                row[0] = 0.0
+            while len(lte_rows) < 3:
+                lte_rows.append(np.zeros(dim + 2))
+            Fl = t.viscous_flux(dim, q, *lte_rows)
+            p_id = (gamma - 1) * (q[dim + 1] - 0.5 * r * u @ u)
+            id_rows = []
+            for row in rows:
+                g = row.copy()
+                g[dim + 1] = r * R * row[dim + 1] + p_id / r * row[0]
+                id_rows.append(g)
+            while len(id_rows) < 3:
+                id_rows.append(np.zeros(dim + 2))
+            Pr = v["mu"] * cp / v["lambda"]
+            Fi = O.viscous_flux(dim, gamma, Pr, R, v["mu"], 2.0 / 3.0, q, *id_rows)
+            np.testing.assert_allclose(Fl, Fi, rtol=1e-12, atol=1e-12 * np.abs(Fi).max())
+
+
+def ideal_gas_table(rho_lo, rho_hi, T_lo, T_hi, R, gamma, mu, Pr, n=400):
+    """A table of the calorically perfect gas: H = cv T (1 - ln T) + R T ln rho
+    (so p = rho^2 H_rho = rho R T, e = H - T H_T = cv T), constant viscosity and
+    conductivity mu cp / Pr, energy axis cv T with the exact inverse guess."""
+    cv = R / (gamma - 1)
+    rho = np.linspace(rho_lo, rho_hi, n)
+    T = np.linspace(T_lo, T_hi, n)
+    rr, TT = np.meshgrid(rho, T)  # [ny][nx]
+    planes = np.zeros((6, n, n))
+    planes[0] = cv * TT * (1 - np.log(TT)) + R * TT * np.log(rr)
+    planes[1] = R * TT / rr
+    planes[2] = -cv * np.log(TT) + R * np.log(rr)
+    planes[3] = R / rr
+    planes[4] = mu
+    planes[5] = mu * (cv + R) / Pr
+    e = cv * T
+    inv = np.repeat((e / cv)[:, None], n, axis=1)
+    return LteTable(arrays=(rho, T, e, planes.reshape(-1), inv.reshape(-1)))
+
+
+def _tgv_state(mesh, order, R):
+    """A Taylor-Green state with a density perturbation (rho, u, p all vary)."""
+    from theseus_amd.mesh import gll_nodes
+
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    x, y, z = X[0], X[1], X[2]
+    rho = 1.0 + 0.2 * np.sin(x) * np.cos(z)
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    w = 0.3 * np.sin(z) * np.cos(x)
+    p = 8.0 + (np.cos(2.0 * x) + np.cos(2.0 * y)) * (np.cos(2.0 * z) + 2) / 16.0
+    E = p / 0.4 + 0.5 * rho * (u * u + v * v + w * w)
+    U = np.stack([rho, rho * u, rho * v, rho * w, E]).reshape(-1)
+    return U, rho, p / (rho * R)
+
+
+def test_lte_navier_stokes_viscous_terms_match_the_ideal_gas_on_an_ideal_gas_table():
+    """The viscous part of the tabulated gas's Navier-Stokes RHS (NS minus
+    Euler, both LTEEC) equals the ideal gas's (NS minus Euler, both Ranocha/LLF)
+    on a table of that ideal gas with its constant viscosity and conductivity:
+    the entropy variables of the two gases differ by the factor R and a
+    constant, so the BR1 gradients agree, and both conversions give the same
+    velocity and temperature gradients.  Left over: the table's interpolation
+    in the temperature recovery.  The stability estimate's diffusive rate
+    agrees too (gamma = cp/cv of the interpolant)."""
+    from oracle.pyoracle import OracleCase
+    from theseus_amd import _abi as A
+    from theseus_amd.config import RunTime
+    from theseus_amd.mesh import box_mesh
+    from theseus_amd.operator import build_descriptor
+
+    R, gamma, mu, Pr = 1.0, 1.4, 0.05, 0.71
+    mesh = box_mesh(3, 2, 0.0, 2 * np.pi, periodic=True)
+    U, rho, T = _tgv_state(mesh, 3, R)
+    table = ideal_gas_table(rho.min() * 0.95, rho.max() * 1.05, T.min() * 0.95, T.max() * 1.05, R, gamma, mu, Pr)
+
+    def case(flow, vol, surf, lte):
+        rt = RunTime(dim=3, order=3, flow_model=flow, volume_flux=vol, surface_flux=surf, gamma=gamma, Pr=Pr,
+                     R_gas=R, mu=mu, blending=False)
+        desc, keep = build_descriptor(rt, mesh)
+        c = OracleCase(desc)
+        c._keep = keep
+        if lte:
+            table.attach(c)
+        return c
+
+    ns_i = case(A.FLOW_NAVIER_STOKES, A.VOLUME_RANOCHA, A.SURFACE_LLF, False)
+    eu_i = case(A.FLOW_EULER, A.VOLUME_RANOCHA, A.SURFACE_LLF, False)
+    ns_t = case(A.FLOW_NAVIER_STOKES, A.VOLUME_LTEEC, A.SURFACE_LTEEC, True)
+    eu_t = case(A.FLOW_EULER, A.VOLUME_LTEEC, A.SURFACE_LTEEC, True)
+    vis_i = ns_i.rhs(U) - eu_i.rhs(U)
+    vis_t = ns_t.rhs(U) - eu_t.rhs(U)
+    scale = np.abs(vis_i).max()
+    assert scale > 1e-2
+    assert np.abs(vis_i.reshape(5, -1)[0]).max() <= 1e-12 * scale  # no viscous mass flux
+    err = np.abs(vis_t - vis_i).max() / scale
+    assert err <= 1e-8, err
+    # cv = -T H_TT comes from the interpolant's second derivative, O(h^2): here ~5e-8
+    ri, rt_ = ns_i.stability(U), ns_t.stability(U)
+    np.testing.assert_allclose(rt_[2], ri[2], rtol=1e-6)
+
+
+def test_lte_navier_stokes_free_stream_and_conservation(small_table):
+    """Uniform flow gives dU = 0 (the viscous terms vanish with the
+    gradients); on a periodic box the weighted sum of dU vanishes with the
+    viscous terms included (central viscous face flux)."""
+    import dataclasses
+
+    from theseus_amd import _abi as A
+    from theseus_amd.mesh import gll_nodes
+
+    from theseus_amd.cases import c4_lte_tgv
+    from theseus_amd.operator import build_descriptor
+    from oracle.pyoracle import OracleCase
+
+    rt, mesh, U, _ = c4_lte_tgv(n=2, order=3, table=small_table)
+    rt = dataclasses.replace(rt, flow_model=A.FLOW_NAVIER_STOKES)
+    desc, keep = build_descriptor(rt, mesh)
+    c = OracleCase(desc)
+    c._keep = keep
+    t = LteTable(arrays=(small_table.rho, small_table.T, small_table.e, small_table.planes.reshape(-1) * 1.0,
+                         small_table.inv.reshape(-1)))
+    t.attach(c)
+    N = mesh.num_elements * 64
+    q = U.reshape(5, -1)[:, 1].copy()
+    r = c.rhs(np.repeat(q, N))
+    assert np.abs(r).max() < 1e-9 * np.abs(q).max()
+    r = c.rhs(U).reshape(5, -1)
+    _, w = gll_nodes(3)
+    W = np.einsum("i,j,k->kji", w, w, w).reshape(-1)
+    Wall = np.tile(W, mesh.num_elements) * np.repeat(mesh.elem_jac, 64)
+    for k in range(5):
+        assert abs(r[k] @ Wall) <= 1e-11 * (np.abs(r[k]) @ Wall + 1e-300), k

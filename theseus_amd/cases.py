@@ -198,36 +198,44 @@ def lte_taylor_green(mesh: BoxMesh, order: int, table, rho0: float = 0.1, T0: fl
     """Taylor-Green vortex of the tabulated LTE gas: rho = rho0, velocity
     amplitude U = mach * c(rho0, T0) with c^2 = gamma_eff R T of the synthetic
     gas, pressure p0 + rho0 U^2/16 (cos 2x + cos 2y)(cos 2z + 2) with
-    p0 = rho0 R T0; T = p/(rho R) (the synthetic gas's pressure H_rho rho^2 =
-    rho R T) and the energy from the table's interpolant."""
+    p0 = rho0 R T0 (on a 2-D mesh the planar vortex, p0 + rho0 U^2/4
+    (cos 2x + cos 2y)); T = p/(rho R) (the synthetic gas's pressure
+    H_rho rho^2 = rho R T) and the energy from the table's interpolant."""
     from .lte import R_AIR, SyntheticGas, _energy_at
 
     gas = SyntheticGas()
     x01, _ = gll_nodes(order)
     X = mesh.node_coordinates(x01)
-    x, y, z = X[0], X[1], X[2]
+    x, y = X[0], X[1]
+    z = X[2] if mesh.dim == 3 else np.zeros_like(x)
     gam = 1.0 + R_AIR / gas.cv(T0)
     U = mach * np.sqrt(gam * R_AIR * T0)
     u = U * np.sin(x) * np.cos(y) * np.cos(z)
     v = -U * np.cos(x) * np.sin(y) * np.cos(z)
     w = np.zeros_like(x)
     rho = np.full_like(x, rho0)
-    p = rho0 * R_AIR * T0 + rho0 * U * U / 16.0 * (np.cos(2.0 * x) + np.cos(2.0 * y)) * (np.cos(2.0 * z) + 2)
+    if mesh.dim == 3:
+        p = rho0 * R_AIR * T0 + rho0 * U * U / 16.0 * (np.cos(2.0 * x) + np.cos(2.0 * y)) * (np.cos(2.0 * z) + 2)
+    else:
+        p = rho0 * R_AIR * T0 + rho0 * U * U / 4.0 * (np.cos(2.0 * x) + np.cos(2.0 * y))
     T = p / (rho0 * R_AIR)
     i = int(np.searchsorted(table.rho, rho0, side="right") - 1)
     e = _energy_at(table, min(i, table.nx - 2), rho0, T)
     E = rho * e + 0.5 * rho * (u * u + v * v + w * w)
-    return _pack(rho, [rho * u, rho * v, rho * w], E)
+    mom = [rho * u, rho * v, rho * w] if mesh.dim == 3 else [rho * u, rho * v]
+    return _pack(rho, mom, E)
 
 
-def c4_lte_tgv(n: int = 24, order: int = 3, table=None):
+def c4_lte_tgv(n: int = 24, order: int = 3, table=None, viscous: bool = False, dim: int = 3):
     """3-D LTE Taylor-Green vortex, periodic [0,2pi]^3, LTEEC flux, the
     synthetic dissociating-gas table (256 x 256, log rho in [-4,1], log T in
-    [2.3,4.2]), rho0 = 0.1, T0 = 3500 K, amplitude Mach 0.3."""
+    [2.3,4.2]), rho0 = 0.1, T0 = 3500 K, amplitude Mach 0.3.  viscous: the
+    Navier-Stokes form (BR1, viscosity and conductivity from the table's
+    mu and lambda planes); dim = 2: the planar vortex on [0,2pi]^2."""
     from .lte import synthetic_table
 
     table = table if table is not None else synthetic_table()
-    mesh = box_mesh(3, n, 0.0, 2 * np.pi, periodic=True)
-    rt = RunTime(dim=3, order=order, flow_model=A.FLOW_EULER, volume_flux=A.VOLUME_LTEEC,
-                 surface_flux=A.SURFACE_LTEEC, blending=False)
+    mesh = box_mesh(dim, n, 0.0, 2 * np.pi, periodic=True)
+    rt = RunTime(dim=dim, order=order, flow_model=A.FLOW_NAVIER_STOKES if viscous else A.FLOW_EULER,
+                 volume_flux=A.VOLUME_LTEEC, surface_flux=A.SURFACE_LTEEC, blending=False)
     return rt, mesh, lte_taylor_green(mesh, order, table), table

@@ -457,12 +457,18 @@ static int enqueue_rhs(theseus_rhs *h, RHSArgs a, hipStream_t s) {
     else hipLaunchKernelGGL(k_pref<3>, dim3(1), dim3(64), 0, s, a.U, h->Ntot, h->gas, h->d_pref);
     h->pref_set = true;
   }
-  if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
+  if (h->lk) {
+    if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
+      h->lk->nodes(a, nb, s);
+      timed(h, 1, s, [&] { h->lk->ns_grad(a, nb, s); });
+      timed(h, 0, s, [&] { h->lk->ns(a, nb, s); });
+    } else {
+      timed(h, 1, s, [&] { h->lk->nodes(a, nb, s); });
+      timed(h, 0, s, [&] { h->lk->rhs(a, nb, s); });
+    }
+  } else if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
     timed(h, 1, s, [&] { h->sk->ns_grad[h->pm][h->has_boundary](a, nb, s); });
     timed(h, 0, s, [&] { h->fk->ns[h->pm][h->has_boundary](a, nb, s); });
-  } else if (h->lk) {
-    timed(h, 1, s, [&] { h->lk->nodes(a, nb, s); });
-    timed(h, 0, s, [&] { h->lk->rhs(a, nb, s); });
   } else {
     timed(h, 0, s, [&] { h->fk->euler[h->pm][h->has_boundary](a, nb, s); });
   }
@@ -645,8 +651,8 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     return fail(THESEUS_ERR_INVALID_ARGUMENT, "Navier-Stokes needs Pr > 0 and mu >= 0");
   const LteKernels *lk = nullptr;
   if (lt) {
-    if (d.flow_model != THESEUS_FLOW_EULER || d.precision != THESEUS_PRECISION_FP64 || d.blending)
-      return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas is implemented for FP64 Euler without blending");
+    if (d.precision != THESEUS_PRECISION_FP64 || d.blending)
+      return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas is implemented in FP64 without blending");
     if (d.volume_flux != THESEUS_VOLUME_LTEEC || d.surface_flux != THESEUS_SURFACE_LTEEC)
       return fail(THESEUS_ERR_INVALID_ARGUMENT, "the tabulated LTE gas uses the LTEEC volume and surface fluxes");
     if (lt->nx < 2 || lt->ny < 2 || !lt->rho_grid || !lt->T_grid || !lt->e_grid || !lt->planes || !lt->inverse)
@@ -677,6 +683,8 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     for (int b = 0; b < d.num_bc; ++b)
       if (d.bc_type[b] != THESEUS_BC_SUPERSONIC_OUTFLOW)
         return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas supports supersonic-outflow boundaries only");
+  if (lt && d.flow_model == THESEUS_FLOW_NAVIER_STOKES && has_boundary)
+    return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas with Navier-Stokes is implemented for periodic meshes");
   const int ne = d.num_elements, dim = d.dim;
 
   int device = 0;

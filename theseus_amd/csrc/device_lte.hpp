@@ -142,6 +142,18 @@ struct Eval {
   __device__ double g_rho() const { return 2 * H<1, 0>() + rho * H<2, 0>(); }
   __device__ double g_T() const { return H<0, 1>() + rho * H<1, 1>(); }
   __device__ double cv() const { return -T * H<0, 2>(); }
+  __device__ double cp() const {  // LTEHelmholtzEOS.hpp:76-80
+    const double m = H<1, 1>();
+    return cv() + T * rho * rho * m * m / p_rho();
+  }
+  // LTEHelmholtzEOS.hpp:99-110: viscosity (plane 4) or thermal conductivity
+  // (plane 5), bilinear in the cell's coordinates
+  template <int PROP>
+  __device__ double transport() const {
+    const double *p = cell.H + PROP * cell.plane;
+    return p[0] * (1 - xi) * (1 - eta) + p[1] * xi * (1 - eta) + p[cell.stride] * (1 - xi) * eta +
+           p[cell.stride + 1] * xi * eta;
+  }
   __device__ double c2() const {
     const double m = H<1, 1>();
     return p_rho() - rho * rho * m * m / H<0, 2>();
@@ -379,6 +391,46 @@ __global__ void __launch_bounds__(256) k_lte_nodes(const double *U, long long Nt
   const int j = find_cell(v.T, v.ny, T);
   v.node_T[t] = T;
   v.node_cell[t] = (i < 0 || j < 0) ? -1 : (i | (j << 16));
+}
+
+// ---- the viscous path (LTEGasModel<LTEHelmholtzGasEOS, LTETransport>,
+//      LTEGasModel.hpp:339; LTETransport.hpp:32-65 without SUTHERLAND)
+
+// A node's quantities for the viscous flux: velocity, recovered temperature,
+// tabulated viscosity and conductivity
+template <int DIM>
+struct ViscNode {
+  double u[DIM];
+  double T, mu, kappa;
+};
+
+// Entropy variables ((g - |u|^2/2)/T, u/T, -1/T) of a node whose temperature
+// and cell were recovered (LTEHelmholtzGasEOS.hpp:144-157; the kinetic term
+// is the kinetic energy density over rho, :40-50)
+template <int DIM>
+__device__ __forceinline__ void entropy_state(const View &v, const double *q, double T, int cell, double *E) {
+  const Eval s = at_packed(v, q[0], T, cell);
+  const double beta = 1 / s.T;
+  double m2 = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) m2 += q[1 + d] * q[1 + d];
+  const double kinetic = 0.5 * m2 / q[0] / q[0];
+  E[0] = beta * (s.gibbs() - kinetic);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) E[1 + d] = beta * (q[1 + d] / q[0]);
+  E[DIM + 1] = -beta;
+}
+
+template <int DIM>
+__device__ __forceinline__ ViscNode<DIM> visc_node(const View &v, const double *q, double T, int cell) {
+  const Eval s = at_packed(v, q[0], T, cell);
+  ViscNode<DIM> n;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) n.u[d] = q[1 + d] / q[0];
+  n.T = T;
+  n.mu = s.transport<4>();
+  n.kappa = s.transport<5>();
+  return n;
 }
 
 // A point's evaluated state from its conservative vector (LTEHelmholtzGasEOS.hpp:60-77)

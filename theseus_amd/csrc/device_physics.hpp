@@ -711,15 +711,47 @@ struct VisComp {
   __host__ __device__ static constexpr int en(int d) { return NTAU + d; }
 };
 
-// Physical viscous flux, NavierStokesFlux.hpp:389-519 (Cartesian); writes the
-// unique components in VisComp order.
+// Physical viscous flux, NavierStokesFlux.hpp:160-290 (Cartesian), from the
+// transport coefficients, the velocity u, the temperature gradient gt and the
+// primitive gradients dP[dir][]; writes the unique components in VisComp order.
+template <typename R, int DIM>
+__device__ __forceinline__ void viscous_flux_core(R mu, R kappa, R mu_bulk, const R *u, const R *gt,
+                                                  const R (*dP)[DIM + 2], R *out) {
+  R div_vel = R(0);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) div_vel += dP[d][1 + d];
+  R tau[DIM][DIM];
+#pragma unroll
+  for (int dir = 0; dir < DIM; ++dir)
+#pragma unroll
+    for (int mom = 0; mom < DIM; ++mom) {
+      if (mom == dir) tau[mom][dir] = mu * (R(2) * dP[dir][1 + mom] - mu_bulk * div_vel);
+      else tau[mom][dir] = mu * (dP[dir][1 + mom] + dP[mom][1 + dir]);
+    }
+#pragma unroll
+  for (int a = 0; a < DIM; ++a)
+#pragma unroll
+    for (int b = a; b < DIM; ++b) out[VisComp<DIM>::tau(a, b)] = tau[a][b];
+#pragma unroll
+  for (int dir = 0; dir < DIM; ++dir) {
+    R ef = kappa * gt[dir];
+#pragma unroll
+    for (int mom = 0; mom < DIM; ++mom) ef += u[mom] * tau[mom][dir];
+    out[VisComp<DIM>::en(dir)] = ef;
+  }
+}
+
+// The ideal gas: constant viscosity, kappa = mu cp / Pr (Transport.hpp:19-54)
+// and grad T from the pressure and density gradients (EOS.hpp:159-173,
+// fac = (1/(gamma-1)) / (cv rho), cv = cp/gamma).  The same arithmetic as
+// viscous_flux_core, written out: computing grad T and div u in one loop
+// keeps the FP64 gradient kernel of the 3-D p = 3 case within its registers.
 template <typename R, int DIM>
 __device__ __forceinline__ void viscous_flux(const GasC<R> &g, const NodeThermo<R, DIM> &t, const R (*dP)[DIM + 2],
                                              R *out) {
   const R mu = g.mu;
   const R kappa = g.mu * g.cp * g.PrInverse;
   const R mu_bulk = g.mu_bulk;
-  // grad_temperature (EOS.hpp:159-173): fac = (1/(gamma-1)) / (cv rho), cv = cp/gamma
   const R pressor = t.p * t.ir;
   const R fac = g.gammaM1Inverse * g.gamma / g.cp * t.ir;
   R gt[DIM];

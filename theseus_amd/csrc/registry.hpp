@@ -51,6 +51,8 @@ struct OrdersKernels {
 struct LteKernels {
   RHSLaunch rhs = nullptr;
   RHSLaunch nodes = nullptr;  // temperature recovery per node (k_lte_nodes), before rhs
+  RHSLaunch ns_grad = nullptr;  // Navier-Stokes pass 1 of the tabulated gas (after nodes)
+  RHSLaunch ns = nullptr;       // Navier-Stokes pass 2 (after ns_grad)
   int tpb = 0, epb = 0;
 };
 
@@ -140,9 +142,13 @@ struct OrdersRegistrar {
   }
 };
 
-template <int DIM, int P>
+template <int DIM, int P, bool VISC = false>
 void launch_rhs_lte(const RHSArgs &a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_rhs_lte<DIM, P>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+  hipLaunchKernelGGL((k_rhs_lte<DIM, P, VISC>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+}
+template <int DIM, int P>
+void launch_ns_grad_lte(const RHSArgs &a, int nb, hipStream_t s) {
+  hipLaunchKernelGGL((k_ns_grad<double, DIM, P, 0, false, true>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
 }
 
 template <int DIM>
@@ -157,6 +163,8 @@ struct LteRegistrar {
     LteKernels k;
     k.rhs = &launch_rhs_lte<DIM, P>;
     k.nodes = &launch_lte_nodes<DIM>;
+    k.ns_grad = &launch_ns_grad_lte<DIM, P>;
+    k.ns = &launch_rhs_lte<DIM, P, true>;
     k.tpb = Shape<DIM, P>::TPB;
     k.epb = Shape<DIM, P>::EPB;
     register_lte(DIM, P, k);

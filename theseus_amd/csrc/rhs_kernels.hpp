@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "bc_device.hpp"
 #include "device_lte.hpp"
 #include "device_physics.hpp"
@@ -1186,15 +1188,40 @@ __device__ __forceinline__ void line_contract_mfma5(const float (*F)[64], const 
 // the allocator would otherwise take 116 and run 4; 8 for its FP32 form
 // with the matrix-core contractions (64 VGPRs, as without them); elsewhere
 // the default.  With the boundary code either would spill.
-template <typename R, int DIM, int P, int EPBX, bool BND>
+template <typename R, int DIM, int P, int EPBX, bool BND, bool LTE = false>
 struct GradWaves {
-  static constexpr bool fits = sizeof(R) == sizeof(double) && DIM == 3 && P == 3 && EPBX == 0 && !BND;
+  static constexpr bool fits = sizeof(R) == sizeof(double) && DIM == 3 && P == 3 && EPBX == 0 && !BND && !LTE;
   static constexpr int value = fits ? 5 : (GradMfma<R, DIM, P, EPBX>::on && !BND ? 8 : 1);  // 64 VGPRs
 };
 
-template <typename R, int DIM, int P, int EPBX = 0, bool BND = false>
+// The tabulated gas on the BR1 path: the primitive gradient (0, grad u, grad T)
+// from the entropy-variable gradient (LTEHelmholtzGasEOS.hpp:159-171) and the
+// viscous flux with the node's tabulated viscosity and conductivity, grad T
+// taken from the energy slot (grad_temperature, :133-142)
+template <typename R, int DIM>
+__device__ __forceinline__ void grad_entropy_to_grad_prim(const lte::ViscNode<DIM> &t, const R *, const R *dE,
+                                                          R *dP) {
+  const R T = t.T;
+  dP[0] = R(0);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) dP[1 + d] = T * dE[1 + d] + T * t.u[d] * dE[DIM + 1];
+  dP[DIM + 1] = T * T * dE[DIM + 1];
+}
+template <typename R, int DIM>
+__device__ __forceinline__ void viscous_flux(const GasC<R> &g, const lte::ViscNode<DIM> &t, const R (*dP)[DIM + 2],
+                                             R *out) {
+  R gt[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) gt[d] = dP[d][DIM + 1];
+  viscous_flux_core<R, DIM>(t.mu, t.kappa, g.mu_bulk, t.u, gt, dP, out);
+}
+
+// LTE: the tabulated gas (FP64, periodic meshes); its nodes' temperatures and
+// cells come from k_lte_nodes of the same state
+template <typename R, int DIM, int P, int EPBX = 0, bool BND = false, bool LTE = false>
 __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB))
-__attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND>::value))) k_ns_grad(RHSArgs a) {
+__attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND, LTE>::value))) k_ns_grad(RHSArgs a) {
+  static_assert(!LTE || (sizeof(R) == sizeof(double) && !BND), "the tabulated gas: FP64, periodic");
   using S = Shape<DIM, P, EPBX>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NV = VisComp<DIM>::N;
@@ -1253,17 +1280,29 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND>::value))) k_n
   double q64[NQ];
 #pragma unroll
   for (int c = 0; c < NQ; ++c) q64[c] = a.U[c * Ntot + ebase];
+  double Town = 0.0;
+  int cown = 0;
+  if constexpr (LTE) {
+    Town = a.lte.node_T[ebase];
+    cown = a.lte.node_cell[ebase];
+  }
   static_assert(Np * Np <= TPB, "one basis entry per lane");
   const int kb = min(tid, Np * Np - 1);
   const double bD = a.basis.D[kb], bDh = a.basis.Dhat[kb];
   meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
   double qn[NFR][NQ];
+  double Tn[LTE ? NFR : 1];
+  int cn[LTE ? NFR : 1];
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
     const uint32_t fc = kFaceTable<DIM, P, EPBX>.v[min(tid + r * TPB, NFT - 1)];
     const size_t nbase = (size_t)max(fnb[r], 0) * ND + ((fc >> 8) & 255u);
 #pragma unroll
     for (int c = 0; c < NQ; ++c) qn[r][c] = a.U[c * Ntot + nbase];
+    if constexpr (LTE) {
+      Tn[r] = a.lte.node_T[nbase];
+      cn[r] = a.lte.node_cell[nbase];
+    }
   }
   if (tid < Np * Np) {
     sD[tid] = R(bD);
@@ -1277,7 +1316,8 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND>::value))) k_n
   for (int c = 0; c < NQ; ++c) q[c] = R(q64[c]);
   if (valid) {
     R Sv[NQ];
-    entropy_state<R, DIM>(g, q, Sv);
+    if constexpr (LTE) lte::entropy_state<DIM>(a.lte, q, Town, cown, Sv);
+    else entropy_state<R, DIM>(g, q, Sv);
 #pragma unroll
     for (int c = 0; c < NQ; ++c) sS[el][c][node] = Sv[c];
   }
@@ -1299,7 +1339,8 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND>::value))) k_n
         R qr[NQ], Sn[NQ];
 #pragma unroll
         for (int c = 0; c < NQ; ++c) qr[c] = R(qn[r][c]);
-        entropy_state<R, DIM>(g, qr, Sn);
+        if constexpr (LTE) lte::entropy_state<DIM>(a.lte, qr, Tn[LTE ? r : 0], cn[LTE ? r : 0], Sn);
+        else entropy_state<R, DIM>(g, qr, Sn);
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {
           const R so = sS[pe][c][on];
@@ -1370,7 +1411,10 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND>::value))) k_n
         for (int c = 0; c < NQ; ++c) acc[dd][c] = dir == 0 ? dref[c] * ad : acc[dd][c] + dref[c] * ad;
       }
     }
-    const NodeThermo<R, DIM> th = node_thermo<R, DIM>(g, q);
+    using Thermo = std::conditional_t<LTE, lte::ViscNode<DIM>, NodeThermo<R, DIM>>;
+    Thermo th;
+    if constexpr (LTE) th = lte::visc_node<DIM>(a.lte, q, Town, cown);
+    else th = node_thermo<R, DIM>(g, q);
 
     // gradient = volume part / J + w * jump * n from the face through the
     // node in each direction (a node lies on at most one face per direction)
@@ -1549,7 +1593,10 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
 #pragma unroll
     for (int c = 0; c < NQ; ++c) s.q[c] = a.U[c * Ntot + ebase];
     derive(g, s);
-    const double c_s = a.lte.nx > 0 ? sqrt(lte::state_eval<DIM>(a.lte, s.q).c2()) : sound_speed(g, s);
+    const bool tab = a.lte.nx > 0;
+    lte::Eval ev;
+    if (tab) ev = lte::state_eval<DIM>(a.lte, s.q);
+    const double c_s = tab ? sqrt(ev.c2()) : sound_speed(g, s);
     const double invJ = 1.0 / a.J[e];
     const double *adj_e = a.adj + (size_t)e * DIM * DIM;
     double dsum = 0.0, msum = 0.0;
@@ -1567,12 +1614,19 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
     }
     adv = a.adv_scale * dsum;
     if (a.viscous) {
+      // StabilityEstimate.hpp:23-38; the tabulated gas takes viscosity and
+      // conductivity from the table and gamma = cp/cv from the interpolant
       const double rho = s.q[0];
-      const double shear = g.mu;
+      double shear = g.mu, kappa = g.mu * g.cp * g.PrInverse, gam = g.gamma, cp = g.cp;
+      if (tab) {
+        shear = ev.transport<4>();
+        kappa = ev.transport<5>();
+        cp = ev.cp();
+        gam = cp / ev.cv();
+      }
       const double longitudinal = (2.0 - g.mu_bulk) * shear;
       const double md = fmax(shear, longitudinal) / rho;
-      const double kappa = g.mu * g.cp * g.PrInverse;
-      const double td = kappa * g.gamma / (rho * g.cp);
+      const double td = kappa * gam / (rho * cp);
       diff = a.diff_scale * fmax(md, td) * msum;
     }
     // face rates: each node on a face takes the max over the faces it touches

@@ -5,14 +5,18 @@
 // once per node per stage by k_lte_nodes (inverse-table guess + Newton)
 // and staged in LDS with the table cell, so the pair fluxes only evaluate
 // the interpolant; each GLL-line pair is evaluated once; the neighbours' face
-// traces come with their recovered temperatures and cells.
+// traces come with their recovered temperatures and cells.  With VISC it is
+// the Navier-Stokes pass 2 of the tabulated gas (periodic meshes): the
+// viscous volume divergence with the own half of the central viscous face
+// flux, and the neighbours' n.Fv, come from k_ns_grad<..., LTE> as for the
+// ideal gas (DGSEMIntegrator.hpp:261-316, 486-622).
 #pragma once
 
 #include "rhs_kernels.hpp"
 
 namespace thx {
 
-template <int DIM, int P>
+template <int DIM, int P, bool VISC = false>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   using S = Shape<DIM, P>;
   using SC = RhsScratch<double, DIM, P>;
@@ -20,6 +24,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   constexpr int NFACE = 2 * DIM * NF;
   constexpr int NFR = (EPB * NFACE + TPB - 1) / TPB;
   constexpr int NPL = SC::NPL, NPAIR = SC::NPAIR;
+  constexpr int NVD = DIM + 1;  // viscous terms: momentum and energy
 
   __shared__ double sq[EPB][NQ][ND];
   __shared__ double sT[EPB][ND];     // recovered temperature
@@ -43,6 +48,9 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   const int e = ebase_blk + el;
   const bool valid = lane_ok && e < a.ne;
   const long long Ntot = a.Ntot;
+  const long long Gtot = (long long)a.ne * NFACE;
+  const double *Vd = static_cast<const double *>(a.Vd);
+  const double *Gf = static_cast<const double *>(a.Gf);
 
   // ---- loads (unconditional, clamped), as in k_rhs
   constexpr int NFT = EPB * NFACE;
@@ -158,6 +166,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   // state, loaded once the pair fluxes no longer hold registers
   double qn[NFR][NQ], Tn[NFR];
   int cn[NFR];
+  double gn[NFR][VISC ? NVD : 1];
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
     const int fi = min(tid + r * TPB, NFT - 1);
@@ -170,6 +179,12 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
     for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
     Tn[r] = tv.node_T[nbase];
     cn[r] = tv.node_cell[nbase];
+    if constexpr (VISC) {
+      // the neighbour's n.Fv at this point (the own side's half is in Vd)
+      const size_t gb_n = (size_t)max(fnb[r], 0) * NFACE + (f ^ 1) * NF + t;
+#pragma unroll
+      for (int c = 0; c < NVD; ++c) gn[r][c] = Gf[c * Gtot + gb_n];
+    }
   }
   double xb[NQ];
   {
@@ -222,6 +237,11 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
         double F[NQ];
         lte::lteec_flux<DIM, true>(tv, ql, qh, sl, sh, own_low ? po : pt, own_low ? pt : po, own_low ? co : ct,
                                    own_low ? ct : co, n, F);
+        if constexpr (VISC) {
+          // F* - 0.5 n.(Fv_L + Fv_R); the own side's -0.5 n.Fv is in Vd
+#pragma unroll
+          for (int c = 0; c < NVD; ++c) F[1 + c] -= interior ? 0.5 * gn[r][c] : 0.0;
+        }
         const double wown = 1.0 / (w0 * sm.J[pe]);
         const double cs = (interior && !s) ? wown : -wown;
 #pragma unroll
@@ -233,6 +253,11 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   double fsum[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) fsum[q] = 0.0;
+  if constexpr (VISC) {
+    // the viscous volume divergence with the own half of the face term
+#pragma unroll
+    for (int c = 0; c < NVD; ++c) fsum[1 + c] = Vd[c * Ntot + ebase];
+  }
   if (valid) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
