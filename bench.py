@@ -184,15 +184,16 @@ def serving_leg(num_cases, log, fp64=False):
     return out
 
 
-def lte_leg(log, steps=10):
-    """LTE Taylor-Green vortex, 24^3 hexes, p=3, LTEEC flux, the synthetic
-    256 x 256 Helmholtz table, FP64: DOF*RHS-evals/s over SSP-RK3 steps."""
+def _lte_run(viscous, steps):
+    """SSP-RK3 steps of the LTE Taylor-Green vortex (24^3, p=3): elapsed ms,
+    finiteness, and the per-kernel means (slot 0: the RHS pass; slot 1: the
+    temperature recovery for Euler, the gradient pass for Navier-Stokes)."""
     import torch
 
     from theseus_amd.cases import c4_lte_tgv
     from theseus_amd.operator import MakeRHSOperator
 
-    rt, mesh, U0, table = c4_lte_tgv(n=24, order=3)
+    rt, mesh, U0, table = c4_lte_tgv(n=24, order=3, viscous=viscous)
     op = MakeRHSOperator(rt, mesh, lte_table=table)
     u = torch.from_numpy(U0).cuda()
     s = op.EstimateStability(u)
@@ -213,14 +214,29 @@ def lte_leg(log, steps=10):
     k_ms, _ = op.kernel_timing(0)
     r_ms, _ = op.kernel_timing(1)
     op.close()
-    dof = U0.size
+    return ms, finite, k_ms, r_ms, U0.size
+
+
+def lte_leg(log, steps=10):
+    """LTE Taylor-Green vortex, 24^3 hexes, p=3, LTEEC flux, the synthetic
+    256 x 256 Helmholtz table, FP64: DOF*RHS-evals/s over SSP-RK3 steps; then
+    the same workload as Navier-Stokes (BR1, viscosity and conductivity from
+    the table's mu and lambda planes)."""
+    ms, finite, k_ms, r_ms, dof = _lte_run(False, steps)
     value = dof * 3 * steps / (ms * 1e-3)
     log(f"lte: {ms/steps:.3f} ms/step, {value:.3e} DOF*RHS/s, kernel {k_ms*1e3:.1f} us, "
         f"temperature recovery {r_ms*1e3:.1f} us")
+    ns_ms, ns_finite, p2_ms, p1_ms, _ = _lte_run(True, steps)
+    ns_value = dof * 3 * steps / (ns_ms * 1e-3)
+    log(f"lte navier-stokes: {ns_ms/steps:.3f} ms/step, {ns_value:.3e} DOF*RHS/s, pass 2 {p2_ms*1e3:.1f} us, "
+        f"pass 1 {p1_ms*1e3:.1f} us")
     return {"value": value, "unit": "DOF*RHS-evals/s", "ms_per_step": ms / steps, "steps": steps,
             "kernel_mean_ms": k_ms, "recovery_mean_ms": r_ms, "finite": finite, "dof": dof,
             "workload": "3-D LTE Taylor-Green vortex, 24^3 hexes, p=3, LTEEC, synthetic 256x256 Helmholtz table, "
-                        "rho0 0.1, T0 3500 K, Mach 0.3, FP64"}
+                        "rho0 0.1, T0 3500 K, Mach 0.3, FP64",
+            "navier_stokes": {"value": ns_value, "ms_per_step": ns_ms / steps, "pass2_mean_ms": p2_ms,
+                              "pass1_mean_ms": p1_ms, "finite": ns_finite,
+                              "workload": "the same, Navier-Stokes BR1 with the table's mu and lambda planes"}}
 
 
 # Whole-RHS bytes per DOF (node x equation) of the workload table's
@@ -324,6 +340,15 @@ def config_table(log, c2_row, lte_row, serving_row, cpu_seconds=8.0):
                      "cpu_sample": "8^3 p=3 (same rate per DOF)",
                      "roofline_frac_fp64": frac(lte_row["value"], "euler", 0), "roofline_frac_mixed": None})
         log(f"table: C4: GPU {lte_row['value']:.3e}, CPU {cpu:.3e}")
+        nsr = lte_row.get("navier_stokes")
+        if nsr is not None:
+            rt, mesh, U0, table = c4_lte_tgv(n=8, order=3, viscous=True)
+            cpu, cores, n = _cpu_rate(rt, mesh, U0, cpu_seconds, lte_table=table)
+            rows.append({"config": "C4 as Navier-Stokes (BR1, tabulated mu and lambda)", "dof": lte_row["dof"],
+                         "gpu_fp64": nsr["value"], "gpu_mixed": None, "cpu_fp64": cpu, "cpu_cores": cores,
+                         "cpu_steps": n, "cpu_sample": "8^3 p=3 (same rate per DOF)",
+                         "roofline_frac_fp64": frac(nsr["value"], "ns", 0), "roofline_frac_mixed": None})
+            log(f"table: C4 Navier-Stokes: GPU {nsr['value']:.3e}, CPU {cpu:.3e}")
     if serving_row is not None:
         rt = euler_runtime(3, A.PRECISION_FP64)
         mesh, U0 = tgv_case(rt, 16, 2.0, (0.1, 0.2, 0.3))
