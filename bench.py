@@ -477,6 +477,9 @@ def main():
     op.Mult(ua, r64)
     opm.Mult(ua, r32)
     torch.cuda.synchronize()
+    # the FP64 entropy-production check of both right-hand sides there
+    P64, M64 = op.EntropyProduction(ua, r64)
+    P32, _ = opm.EntropyProduction(ua, r32)
     mixed = {
         "value": value_m,
         "ms_per_step": msm / args.steps,
@@ -487,11 +490,15 @@ def main():
         "state_rel_l2_vs_fp64": float(torch.linalg.norm(uam - ua) / torch.linalg.norm(ua)),
         "state_after_steps": acc_steps,
         "kernels_mean_ms": {"ns_pass2_volume_face": m2_ms, "ns_pass1_br1_gradient": m1_ms},
+        "entropy_production": {"fp64": P64, "mixed": P32, "magnitude": M64,
+                               "mixed_minus_fp64_over_magnitude": (P32 - P64) / M64,
+                               "check": "sum w J S(u).dudt in FP64 at the same state; <= 0 is entropy-stable"},
         "arithmetic": "FP32 fluxes, gradients and LDS staging; FP64 state, stage combination and indicator",
     }
     log(f"mixed: {msm/args.steps:.3f} ms/step, {value_m:.4e} DOF*RHS/s ({value_m/value:.2f}x), "
         f"state err after {acc_steps} steps {mixed['state_rel_l2_vs_fp64_per_field']}, "
-        f"RHS err {mixed['rhs_rel_l2_vs_fp64_per_field']}")
+        f"RHS err {mixed['rhs_rel_l2_vs_fp64_per_field']}, entropy production fp64 {P64:.6e} mixed {P32:.6e} "
+        f"(magnitude {M64:.3e})")
 
     modes = [1, 2, 2]
     b2, b2_dof = algorithmic_bytes(mesh.num_elements, args.order, 3, 0)

@@ -271,6 +271,25 @@ THESEUS_API int theseus_rhs_integral_measures(theseus_rhs *h, const double *u,
  * must be ordered by the caller.  The handle's device must be current.  The
  * baseline of theseus_rhs_integral_measures_baseline is not updated. */
 THESEUS_API int theseus_rhs_integral_measures_noalloc(theseus_rhs *h, const double *u, double *out, void *stream);
+/* Entropy production of a right-hand side, computed in FP64 whatever the
+ * handle's precision: production = sum over elements e and nodes i of
+ * w_i J_e S(u_i) . dudt_i, magnitude = the same sum of |S . dudt|, with S
+ * the entropy variables ((gamma - s)/(gamma-1) - beta |v|^2/2, beta v, -beta),
+ * s = ln p - gamma ln rho, beta = rho/p (EOS.hpp:206-234), and w_i the GLL
+ * quadrature weights on [0,1]^dim.  It is d/dt of the total mathematical
+ * entropy sum w J (-rho s/(gamma-1)) under du/dt = dudt: <= 0 (up to
+ * rounding) for an entropy-stable discretisation on a periodic mesh, so it
+ * checks a mixed-precision right-hand side in FP64.  No counterpart in the
+ * reference.  u and dudt are device L-vectors in the same layout (file or
+ * aligned frames alike: the sum does not depend on the node order inside an
+ * element).  Ideal gas only (THESEUS_ERR_UNSUPPORTED for the tabulated gas).
+ * Per-workgroup partials and a fixed-order second pass (deterministic);
+ * synchronises `stream`. */
+typedef struct theseus_entropy_production {
+  double production, magnitude;
+} theseus_entropy_production;
+THESEUS_API int theseus_rhs_entropy_production(theseus_rhs *h, const double *u, const double *dudt,
+                                               theseus_entropy_production *out, void *stream);
 /* RHSOperator::GetIntegralMeasuresBaseline (RHSOperator.hpp:92); all zero
  * before the first theseus_rhs_integral_measures call. */
 THESEUS_API int theseus_rhs_integral_measures_baseline(const theseus_rhs *h, theseus_integral_measures *out);

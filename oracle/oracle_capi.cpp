@@ -173,6 +173,16 @@ int oracle_integral_measures(void *h, const double *U, double *out9) {
   }
 }
 
+int oracle_entropy_production(void *h, const double *U, const double *dU, double *out2) {
+  try {
+    EntropyProduction(*static_cast<Case *>(h), U, dU, out2);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
 int oracle_ssprk3_step(void *h, double *U, double dt) {
   try {
     SSPRK3Step(*static_cast<Case *>(h), U, dt);

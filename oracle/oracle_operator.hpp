@@ -990,6 +990,33 @@ struct Measures {
   double mass, ke, en, max_press, min_press, max_temp, min_temp, max_dens, min_dens;
 };
 
+// Entropy production of a right-hand side (the engine's FP64 check of a
+// mixed-precision RHS; no counterpart in the reference): sum over elements
+// and nodes of w_i J_e S(U_i) . dU_i and of its absolute values, S the ideal
+// gas's entropy variables (EOS.hpp:206-234), w the GLL weights on [0,1]^dim.
+inline void EntropyProduction(const Case &c, const double *U, const double *dU, double out[2]) {
+  if (c.lte) throw std::runtime_error("entropy production is implemented for the ideal gas");
+  const size_t N = c.ntot();
+  const int Np = c.Np, dim = c.dim, neq = c.neq;
+  double prod = 0.0, mag = 0.0;
+  double q[MAXEQ], S[MAXEQ];
+  for (int e = 0; e < c.ne; ++e)
+    for (int node = 0; node < c.ndof; ++node) {
+      c.gather(U, e, node, q);
+      c.gas.entropy_state(q, S);
+      double sd = 0.0;
+      for (int k = 0; k < neq; ++k) sd += S[k] * dU[k * N + (size_t)e * c.ndof + node];
+      const int i = node % Np, j = dim > 1 ? (node / Np) % Np : 0, k = dim > 2 ? node / (Np * Np) : 0;
+      double w = c.basis.w[i];
+      if (dim > 1) w *= c.basis.w[j];
+      if (dim > 2) w *= c.basis.w[k];
+      prod += w * c.J[e] * sd;
+      mag += w * c.J[e] * std::abs(sd);
+    }
+  out[0] = prod;
+  out[1] = mag;
+}
+
 inline Measures IntegralMeasures(const Case &c, const double *U) {
   Measures m{0.0, 0.0, 0.0, 0.0, 1e32, 0.0, 1e32, 0.0, 1e32};
   const int Np = c.Np, dim = c.dim;

@@ -57,6 +57,8 @@ def load():
     lib.oracle_estimate_stability.restype = C.c_int
     lib.oracle_integral_measures.argtypes = [C.c_void_p, _dp, _dp]
     lib.oracle_integral_measures.restype = C.c_int
+    lib.oracle_entropy_production.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    lib.oracle_entropy_production.restype = C.c_int
     lib.oracle_ssprk3_step.argtypes = [C.c_void_p, _dp, C.c_double]
     lib.oracle_ssprk3_step.restype = C.c_int
     _lib = lib
@@ -206,6 +208,14 @@ class OracleCase:
         if rc:
             raise RuntimeError(self._lib.oracle_last_error().decode())
         return dict(zip(self.MEASURES, out.tolist()))
+
+    def entropy_production(self, U, dU):
+        """(sum w J S(U).dU, sum w J |S(U).dU|) with the ideal gas's entropy variables."""
+        U, dU = arr(U), arr(dU)
+        out = np.zeros(2)
+        if self._lib.oracle_entropy_production(self._h, _p(U), _p(dU), _p(out)):
+            raise RuntimeError(self._lib.oracle_last_error().decode())
+        return out[0], out[1]
 
     def adaptive_steps(self, U, cfl, max_dt_growth, t_final, nsteps, t=0.0):
         """The reference's variable-step loop (src/Simulation.cpp:1136-1158,

@@ -61,6 +61,7 @@ EXPORTED_SYMBOLS = (
     "theseus_rhs_integral_measures",
     "theseus_rhs_integral_measures_baseline",
     "theseus_rhs_integral_measures_noalloc",
+    "theseus_rhs_entropy_production",
     "theseus_orient_elements",
     "theseus_orient_node",
     "theseus_rhs_step_ssprk3_cfl",
@@ -156,6 +157,11 @@ class IntegralMeasures(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class EntropyProduction(C.Structure):
+    """theseus_entropy_production: sum w J S.dU and sum w J |S.dU|."""
+    _fields_ = [("production", C.c_double), ("magnitude", C.c_double)]
+
+
 class Clock(C.Structure):
     """theseus_clock: the device-resident variable-step clock (64 bytes)."""
     _fields_ = [("t", C.c_double), ("dt", C.c_double), ("dt_real", C.c_double), ("t_final", C.c_double),
@@ -214,6 +220,9 @@ def load(path: Path | str | None = None) -> C.CDLL:
     lib.theseus_rhs_integral_measures_baseline.restype = C.c_int
     lib.theseus_rhs_integral_measures_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.theseus_rhs_integral_measures_noalloc.restype = C.c_int
+    lib.theseus_rhs_entropy_production.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(EntropyProduction),
+                                                   C.c_void_p]
+    lib.theseus_rhs_entropy_production.restype = C.c_int
     lib.theseus_orient_elements.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p]
     lib.theseus_orient_elements.restype = C.c_int

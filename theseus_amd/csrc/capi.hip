@@ -13,6 +13,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -236,6 +237,75 @@ __global__ void __launch_bounds__(256) k_measures_final(const double *partial, i
   if (tid < NMEAS) out[tid] = red[0][tid];
 }
 
+// Entropy production (theseus_rhs_entropy_production): one lane per node,
+// FP64; per-workgroup sums of w J S.dU and w J |S.dU| in a fixed tree order.
+struct GllWeights {
+  double w[MAXNP];  // GLL weights on [0,1]
+};
+template <int DIM>
+__global__ void __launch_bounds__(256) k_entropy_production(const double *U, const double *dU, long long Ntot,
+                                                             int ndof, int Np, const double *J, GasC<double> gas,
+                                                             GllWeights gw, double *partial) {
+  __shared__ double red[256][2];
+  const int tid = threadIdx.x;
+  const long long t = (long long)blockIdx.x * 256 + tid;
+  double pr = 0.0, mg = 0.0;
+  if (t < Ntot) {
+    double q[DIM + 2], S[DIM + 2];
+#pragma unroll
+    for (int c = 0; c < DIM + 2; ++c) q[c] = U[c * Ntot + t];
+    entropy_state<double, DIM>(gas, q, S);
+    double sd = 0.0;
+#pragma unroll
+    for (int c = 0; c < DIM + 2; ++c) sd += S[c] * dU[c * Ntot + t];
+    const int e = (int)(t / ndof);
+    const int node = (int)(t - (long long)e * ndof);
+    double w = gw.w[node % Np];
+    if (DIM > 1) w *= gw.w[(node / Np) % Np];
+    if (DIM > 2) w *= gw.w[node / (Np * Np)];
+    const double wj = w * J[e];
+    pr = wj * sd;
+    mg = wj * fabs(sd);
+  }
+  red[tid][0] = pr;
+  red[tid][1] = mg;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      red[tid][0] += red[tid + st][0];
+      red[tid][1] += red[tid + st][1];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    partial[2 * (size_t)blockIdx.x] = red[0][0];
+    partial[2 * (size_t)blockIdx.x + 1] = red[0][1];
+  }
+}
+
+// One workgroup of 256: thread t sums pairs t, t+256, ... in order, then a
+// fixed binary tree; out[0..1] = the totals.
+__global__ void __launch_bounds__(256) k_sum_pairs_final(const double *partial, int nb, double *out) {
+  __shared__ double red[256][2];
+  const int tid = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int k = tid; k < nb; k += 256) {
+    a += partial[2 * (size_t)k];
+    b += partial[2 * (size_t)k + 1];
+  }
+  red[tid][0] = a;
+  red[tid][1] = b;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      red[tid][0] += red[tid + st][0];
+      red[tid][1] += red[tid + st][1];
+    }
+    __syncthreads();
+  }
+  if (tid < 2) out[tid] = red[0][tid];
+}
+
 // rates[q] = max over the slots of rate q (one 64-lane workgroup).
 __global__ void __launch_bounds__(64) k_rates_fold(unsigned long long *rates) {
   const int t = threadIdx.x;
@@ -376,6 +446,7 @@ struct theseus_rhs {
   double *d_stage[2] = {nullptr, nullptr};
   unsigned long long *d_rates = nullptr;
   double *d_meas = nullptr;           // integral measures: [nblocks][9] partials, then the 9 results
+  double *d_ent = nullptr;            // entropy production: [nodes/256][2] partials, then the 2 results
   theseus_integral_measures diag0{};  // baseline: the first measures with nonzero mass
   bool has_boundary = false;
   int device = 0;  // the device current at create time, which holds the handle's arrays
@@ -859,6 +930,7 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_stage[1]);
   hipFree(h->d_rates);
   hipFree(h->d_meas);
+  hipFree(h->d_ent);
   hipFree(h->d_orient);
   hipFree(h->d_otab);
   hipFree(h->d_ual);
@@ -1053,6 +1125,37 @@ int theseus_rhs_integral_measures(theseus_rhs *h, const double *u, theseus_integ
                      (hipStream_t)stream);
   if (rc) return rc;
   if (h->diag0.mass == 0.0) h->diag0 = *out;
+  return THESEUS_OK;
+}
+
+int theseus_rhs_entropy_production(theseus_rhs *h, const double *u, const double *dudt,
+                                   theseus_entropy_production *out, void *stream) {
+  if (!h || !u || !dudt || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle, vector or output");
+  if (h->lk) return fail(THESEUS_ERR_UNSUPPORTED, "entropy production is implemented for the ideal gas");
+  const long long nbl = (h->Ntot + 255) / 256;
+  if (nbl > INT_MAX) return fail(THESEUS_ERR_INVALID_ARGUMENT, "too many nodes");
+  const int nb = (int)nbl;
+  if (!h->d_ent) {
+    const int rc = upload<double>(&h->d_ent, nullptr, 2 * ((size_t)nb + 1));
+    if (rc) return rc;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int ndof = h->ndof, Np = h->Np;
+  GllWeights gw;
+  std::memcpy(gw.w, h->basis.w, sizeof gw.w);
+  if (h->d.dim == 2)
+    hipLaunchKernelGGL(k_entropy_production<2>, dim3(nb), dim3(256), 0, s, u, dudt, h->Ntot, ndof, Np, h->d_J, h->gas,
+                       gw, h->d_ent);
+  else
+    hipLaunchKernelGGL(k_entropy_production<3>, dim3(nb), dim3(256), 0, s, u, dudt, h->Ntot, ndof, Np, h->d_J, h->gas,
+                       gw, h->d_ent);
+  hipLaunchKernelGGL(k_sum_pairs_final, dim3(1), dim3(256), 0, s, h->d_ent, nb, h->d_ent + 2 * (size_t)nb);
+  HIP_TRY(hipGetLastError());
+  double r[2];
+  HIP_TRY(hipMemcpyAsync(r, h->d_ent + 2 * (size_t)nb, sizeof r, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  out->production = r[0];
+  out->magnitude = r[1];
   return THESEUS_OK;
 }
 

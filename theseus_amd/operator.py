@@ -229,6 +229,19 @@ class RHSOperator:
         A.check(self._lib, self._lib.theseus_rhs_integral_measures_noalloc(
             self._h, C.c_void_p(u.data_ptr()), C.c_void_p(out.data_ptr()), _stream_ptr(stream)))
 
+    def EntropyProduction(self, u, dudt, stream=None) -> tuple[float, float]:
+        """FP64 entropy production of the right-hand side dudt at u:
+        (sum w J S(u).dudt, sum w J |S(u).dudt|), S the entropy variables
+        (EOS.hpp:206-234); <= 0 for an entropy-stable scheme on a periodic
+        mesh, whatever precision computed dudt (theseus_rhs_entropy_production)."""
+        self._check(u, "u")
+        self._check(dudt, "dudt")
+        r = A.EntropyProduction()
+        A.check(self._lib, self._lib.theseus_rhs_entropy_production(self._h, C.c_void_p(u.data_ptr()),
+                                                                    C.c_void_p(dudt.data_ptr()), C.byref(r),
+                                                                    _stream_ptr(stream)))
+        return r.production, r.magnitude
+
     def GetIntegralMeasuresBaseline(self) -> IntegralMeasures:
         """RHSOperator::GetIntegralMeasuresBaseline (RHSOperator.hpp:92)."""
         m = A.IntegralMeasures()
