@@ -184,16 +184,22 @@ def serving_leg(num_cases, log, fp64=False):
     return out
 
 
-def _lte_run(viscous, steps):
-    """SSP-RK3 steps of the LTE Taylor-Green vortex (24^3, p=3): elapsed ms,
-    finiteness, and the per-kernel means (slot 0: the RHS pass; slot 1: the
-    temperature recovery for Euler, the gradient pass for Navier-Stokes)."""
+def _lte_run(viscous, steps, mixed=False, total_steps=50):
+    """SSP-RK3 steps of the LTE Taylor-Green vortex (24^3, p=3): elapsed ms
+    of `steps` timed steps, finiteness, the per-kernel means (slot 0: the RHS
+    pass; slot 1: the temperature recovery for Euler, the gradient pass for
+    Navier-Stokes), and the state after `total_steps` steps (device tensor)."""
+    import dataclasses
+
     import torch
 
+    from theseus_amd import _abi as A
     from theseus_amd.cases import c4_lte_tgv
     from theseus_amd.operator import MakeRHSOperator
 
     rt, mesh, U0, table = c4_lte_tgv(n=24, order=3, viscous=viscous)
+    if mixed:
+        rt = dataclasses.replace(rt, precision=A.PRECISION_MIXED)
     op = MakeRHSOperator(rt, mesh, lte_table=table)
     u = torch.from_numpy(U0).cuda()
     s = op.EstimateStability(u)
@@ -210,23 +216,37 @@ def _lte_run(viscous, steps):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
-    finite = bool(torch.isfinite(u).all())
     k_ms, _ = op.kernel_timing(0)
     r_ms, _ = op.kernel_timing(1)
+    for _ in range(total_steps - 1 - steps):
+        t = op.Step(u, t, dt)
+    torch.cuda.synchronize()
+    finite = bool(torch.isfinite(u).all())
     op.close()
-    return ms, finite, k_ms, r_ms, U0.size
+    return ms, finite, k_ms, r_ms, U0.size, u
 
 
 def lte_leg(log, steps=10):
     """LTE Taylor-Green vortex, 24^3 hexes, p=3, LTEEC flux, the synthetic
-    256 x 256 Helmholtz table, FP64: DOF*RHS-evals/s over SSP-RK3 steps; then
-    the same workload as Navier-Stokes (BR1, viscosity and conductivity from
-    the table's mu and lambda planes)."""
-    ms, finite, k_ms, r_ms, dof = _lte_run(False, steps)
+    256 x 256 Helmholtz table, FP64: DOF*RHS-evals/s over SSP-RK3 steps; the
+    mixed-precision form on the same workload (speed and per-field state
+    error after 50 steps against FP64); then the workload as Navier-Stokes
+    (BR1, viscosity and conductivity from the table's mu and lambda planes)."""
+    import torch
+
+    ms, finite, k_ms, r_ms, dof, u64 = _lte_run(False, steps)
     value = dof * 3 * steps / (ms * 1e-3)
     log(f"lte: {ms/steps:.3f} ms/step, {value:.3e} DOF*RHS/s, kernel {k_ms*1e3:.1f} us, "
         f"temperature recovery {r_ms*1e3:.1f} us")
-    ns_ms, ns_finite, p2_ms, p1_ms, _ = _lte_run(True, steps)
+    mx_ms, mx_finite, mk_ms, _, _, umx = _lte_run(False, steps, mixed=True)
+    mx_value = dof * 3 * steps / (mx_ms * 1e-3)
+    a = umx.reshape(5, -1)
+    b = u64.reshape(5, -1)
+    mx_err = [float(torch.linalg.norm(a[q] - b[q]) / torch.linalg.norm(b[q])) for q in range(5)]
+    del u64, umx, a, b
+    log(f"lte mixed: {mx_ms/steps:.3f} ms/step ({mx_value/value:.2f}x), kernel {mk_ms*1e3:.1f} us, "
+        f"state err after 50 steps vs FP64 {mx_err}")
+    ns_ms, ns_finite, p2_ms, p1_ms, _, _ = _lte_run(True, steps)
     ns_value = dof * 3 * steps / (ns_ms * 1e-3)
     log(f"lte navier-stokes: {ns_ms/steps:.3f} ms/step, {ns_value:.3e} DOF*RHS/s, pass 2 {p2_ms*1e3:.1f} us, "
         f"pass 1 {p1_ms*1e3:.1f} us")
@@ -234,6 +254,11 @@ def lte_leg(log, steps=10):
             "kernel_mean_ms": k_ms, "recovery_mean_ms": r_ms, "finite": finite, "dof": dof,
             "workload": "3-D LTE Taylor-Green vortex, 24^3 hexes, p=3, LTEEC, synthetic 256x256 Helmholtz table, "
                         "rho0 0.1, T0 3500 K, Mach 0.3, FP64",
+            "mixed": {"value": mx_value, "ms_per_step": mx_ms / steps, "speedup_vs_fp64": mx_value / value,
+                      "kernel_mean_ms": mk_ms, "finite": mx_finite,
+                      "state_rel_l2_vs_fp64_per_field": mx_err, "state_after_steps": 50,
+                      "arithmetic": "FP32 pair and face fluxes (Itoh-Abe means on an FP32 copy of the table); "
+                                    "FP64 temperature recovery, nodal pressure and sound speed, stage combination"},
             "navier_stokes": {"value": ns_value, "ms_per_step": ns_ms / steps, "pass2_mean_ms": p2_ms,
                               "pass1_mean_ms": p1_ms, "finite": ns_finite,
                               "workload": "the same, Navier-Stokes BR1 with the table's mu and lambda planes"}}
@@ -335,10 +360,12 @@ def config_table(log, c2_row, lte_row, serving_row, cpu_seconds=8.0):
     if lte_row is not None:
         rt, mesh, U0, table = c4_lte_tgv(n=8, order=3)
         cpu, cores, n = _cpu_rate(rt, mesh, U0, cpu_seconds, lte_table=table)
+        mxr = lte_row.get("mixed")
         rows.append({"config": "C4 3-D LTE TGV 24^3, p=3, LTEEC", "dof": lte_row["dof"], "gpu_fp64": lte_row["value"],
-                     "gpu_mixed": None, "cpu_fp64": cpu, "cpu_cores": cores, "cpu_steps": n,
-                     "cpu_sample": "8^3 p=3 (same rate per DOF)",
-                     "roofline_frac_fp64": frac(lte_row["value"], "euler", 0), "roofline_frac_mixed": None})
+                     "gpu_mixed": mxr["value"] if mxr else None, "cpu_fp64": cpu, "cpu_cores": cores,
+                     "cpu_steps": n, "cpu_sample": "8^3 p=3 (same rate per DOF)",
+                     "roofline_frac_fp64": frac(lte_row["value"], "euler", 0),
+                     "roofline_frac_mixed": frac(mxr["value"], "euler", 1) if mxr else None})
         log(f"table: C4: GPU {lte_row['value']:.3e}, CPU {cpu:.3e}")
         nsr = lte_row.get("navier_stokes")
         if nsr is not None:

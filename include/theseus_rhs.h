@@ -196,8 +196,12 @@ typedef struct theseus_lte_table {
 } theseus_lte_table;
 
 /* MakeRHSOperator for the LTEHelmholtzGas with the LTEEC flux
- * (LTEGasModel.hpp:339, LTEECFlux.hpp:93-107): FP64, volume_flux =
- * THESEUS_VOLUME_LTEEC, surface_flux = THESEUS_SURFACE_LTEEC.  Euler, or
+ * (LTEGasModel.hpp:339, LTEECFlux.hpp:93-107): volume_flux =
+ * THESEUS_VOLUME_LTEEC, surface_flux = THESEUS_SURFACE_LTEEC, no blending.
+ * precision FP64, or MIXED: FP32 pair and face fluxes (the discrete-gradient
+ * means of the interpolant on an FP32 copy of the planes), with the
+ * temperature recovery, each node's pressure and sound speed, the stage
+ * combination and the stability estimate in FP64.  Euler, or
  * Navier-Stokes (flow_model THESEUS_FLOW_NAVIER_STOKES, periodic meshes):
  * BR1 on the gas's entropy variables ((g - |u|^2/2)/T, u/T, -1/T), viscosity
  * and conductivity bilinear from the table's mu and lambda planes

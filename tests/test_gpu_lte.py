@@ -173,3 +173,91 @@ def test_lte_navier_stokes_stability_and_steps_parity(oracle, viscous_table):
     op.close()
     err = rel_l2(res, ref)
     assert err <= 1e-10, (err, per_field_rel(res, ref, 5))
+
+
+# ---------------------------------------------------------------------------
+# The mixed-precision form of the tabulated gas: FP32 pair and face fluxes
+# (the Itoh-Abe means of the interpolant on an FP32 copy of the table);
+# temperature recovery, per-node pressure and sound speed, stage combination
+# in FP64.  Errors per field against the FP64 oracle, bounds set from
+# measured values with about 3x margin.
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("viscous", [False, True])
+@pytest.mark.parametrize("dim,n", [(3, 3), (2, 4)])
+def test_lte_mixed_rhs_and_steps_per_field(oracle, table, viscous_table, viscous, dim, n):
+    """RHS per field against the oracle (density and momenta <= 1e-4; the
+    energy RHS, a small difference of fluxes carrying the internal energy of
+    about 3.5e6 J/kg, <= 1e-2) and the state after 20 SSP-RK3 steps (<= 5e-4
+    per field; the 3-D z-momentum, zero at the start, <= 5e-3)."""
+    import dataclasses
+
+    import torch
+
+    from theseus_amd import _abi as A
+    from theseus_amd.cases import c4_lte_tgv
+    from theseus_amd.operator import MakeRHSOperator
+
+    tab = viscous_table if viscous else table
+    _say(f"case lte mixed dim={dim} n={n} p=3 viscous={viscous}")
+    rt, mesh, U, _ = c4_lte_tgv(n=n, order=3, table=tab, viscous=viscous, dim=dim)
+    rtm = dataclasses.replace(rt, precision=A.PRECISION_MIXED)
+    ora = _oracle(oracle, rt, mesh, tab)
+    ref = ora.rhs(U)
+    got = _gpu_rhs(rtm, mesh, tab, U)
+    nq = dim + 2
+    errs = per_field_rel(got, ref, nq)
+    _say(f"  RHS per-field rel L2 {errs}")
+    assert max(errs[:nq - 1]) <= 1e-4 and errs[nq - 1] <= 1e-2, errs
+    rates = ora.stability(U)
+    dt = 0.5 / (max(rates[0], rates[1]) + rates[2])
+    ref_u = U.copy()
+    for _ in range(20):
+        ref_u = ora.step(ref_u, dt)
+    op = MakeRHSOperator(rtm, mesh, lte_table=tab)
+    u = torch.from_numpy(U).cuda()
+    t = 0.0
+    for _ in range(20):
+        t = op.Step(u, t, dt)
+    torch.cuda.synchronize()
+    res = u.cpu().numpy()
+    op.close()
+    serr = per_field_rel(res, ref_u, nq)
+    _say(f"  state after 20 steps per-field rel L2 {serr}")
+    zmom = [3] if dim == 3 else []
+    assert max(e for q, e in enumerate(serr) if q not in zmom) <= 5e-4, serr
+    assert all(serr[q] <= 5e-3 for q in zmom), serr
+
+
+def test_lte_mixed_full_size_state_per_field(table):
+    """The LTE Taylor-Green workload at full size (24^3, p=3): the mixed
+    state after 50 SSP-RK3 steps against the FP64 engine's (itself within
+    1e-10 of the oracle over steps): <= 1e-5 per field but the z-momentum
+    (zero at the start: <= 5e-4)."""
+    import dataclasses
+
+    import torch
+
+    from theseus_amd import _abi as A
+    from theseus_amd.cases import c4_lte_tgv
+    from theseus_amd.operator import MakeRHSOperator
+
+    _say("case lte mixed full size 24^3 p=3, 50 steps")
+    rt, mesh, U, _ = c4_lte_tgv(n=24, order=3, table=table)
+    out = {}
+    dt = None
+    for name, r in (("fp64", rt), ("mixed", dataclasses.replace(rt, precision=A.PRECISION_MIXED))):
+        op = MakeRHSOperator(r, mesh, lte_table=table)
+        u = torch.from_numpy(U).cuda()
+        if dt is None:
+            dt = 0.5 / op.EstimateStability(u).TotalRate()
+        t = 0.0
+        for _ in range(50):
+            t = op.Step(u, t, dt)
+        torch.cuda.synchronize()
+        out[name] = u.cpu().numpy()
+        op.close()
+    serr = per_field_rel(out["mixed"], out["fp64"], 5)
+    _say(f"  state after 50 steps per-field rel L2 {serr}")
+    assert max(serr[0], serr[1], serr[2], serr[4]) <= 1e-5, serr
+    assert serr[3] <= 5e-4, serr

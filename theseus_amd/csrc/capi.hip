@@ -428,6 +428,7 @@ struct theseus_rhs {
   const LteKernels *lk = nullptr;  // tabulated LTE gas
   lte::View ltev;                  // its device arrays
   double *d_lte = nullptr;
+  float *d_lte32 = nullptr;   // mixed precision: the planes in FP32
   double *d_lteT = nullptr;   // per node: recovered temperature of the stage state
   int *d_ltecell = nullptr;   // per node: its table cell
   BasisArgs basis;
@@ -524,18 +525,19 @@ static int enqueue_rhs(theseus_rhs *h, RHSArgs a, hipStream_t s) {
       return fail(THESEUS_ERR_INVALID_ARGUMENT,
                   "mixed precision: the first evaluation of a handle fixes its background pressure and must not be "
                   "captured into a graph; evaluate or step once before capturing");
-    if (h->d.dim == 2) hipLaunchKernelGGL(k_pref<2>, dim3(1), dim3(64), 0, s, a.U, h->Ntot, h->gas, h->d_pref);
+    if (h->lk) h->lk->pref(a, 1, s);  // the tabulated gas's pressure (FP64 recovery)
+    else if (h->d.dim == 2) hipLaunchKernelGGL(k_pref<2>, dim3(1), dim3(64), 0, s, a.U, h->Ntot, h->gas, h->d_pref);
     else hipLaunchKernelGGL(k_pref<3>, dim3(1), dim3(64), 0, s, a.U, h->Ntot, h->gas, h->d_pref);
     h->pref_set = true;
   }
   if (h->lk) {
     if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
       h->lk->nodes(a, nb, s);
-      timed(h, 1, s, [&] { h->lk->ns_grad(a, nb, s); });
-      timed(h, 0, s, [&] { h->lk->ns(a, nb, s); });
+      timed(h, 1, s, [&] { h->lk->ns_grad[h->pm](a, nb, s); });
+      timed(h, 0, s, [&] { h->lk->ns[h->pm](a, nb, s); });
     } else {
       timed(h, 1, s, [&] { h->lk->nodes(a, nb, s); });
-      timed(h, 0, s, [&] { h->lk->rhs(a, nb, s); });
+      timed(h, 0, s, [&] { h->lk->rhs[h->pm](a, nb, s); });
     }
   } else if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
     timed(h, 1, s, [&] { h->sk->ns_grad[h->pm][h->has_boundary](a, nb, s); });
@@ -722,8 +724,7 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     return fail(THESEUS_ERR_INVALID_ARGUMENT, "Navier-Stokes needs Pr > 0 and mu >= 0");
   const LteKernels *lk = nullptr;
   if (lt) {
-    if (d.precision != THESEUS_PRECISION_FP64 || d.blending)
-      return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas is implemented in FP64 without blending");
+    if (d.blending) return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas is implemented without blending");
     if (d.volume_flux != THESEUS_VOLUME_LTEEC || d.surface_flux != THESEUS_SURFACE_LTEEC)
       return fail(THESEUS_ERR_INVALID_ARGUMENT, "the tabulated LTE gas uses the LTEEC volume and surface fluxes");
     if (lt->nx < 2 || lt->ny < 2 || !lt->rho_grid || !lt->T_grid || !lt->e_grid || !lt->planes || !lt->inverse)
@@ -834,6 +835,15 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     }
     h->ltev.node_T = h->d_lteT;
     h->ltev.node_cell = h->d_ltecell;
+    if (h->pm) {
+      // the mixed-precision form's FP32 copy of the planes
+      std::vector<float> p32(lt->planes, lt->planes + 6 * nx * ny);
+      if ((rc = upload(&h->d_lte32, p32.data(), p32.size()))) {
+        theseus_rhs_destroy(h);
+        return rc;
+      }
+      h->ltev.planes32 = h->d_lte32;
+    }
   }
   if (d.elem_orient) {
     const std::vector<Sym> syms = symmetries(dim);
@@ -924,6 +934,7 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_Fv);
   hipFree(h->d_pref);
   hipFree(h->d_lte);
+  hipFree(h->d_lte32);
   hipFree(h->d_lteT);
   hipFree(h->d_ltecell);
   hipFree(h->d_stage[0]);

@@ -48,11 +48,13 @@ struct OrdersKernels {
 };
 
 // Euler with the tabulated LTE gas and the LTEEC flux
+// [precision]: 0 FP64, 1 the mixed-precision form (FP32 flux arithmetic)
 struct LteKernels {
-  RHSLaunch rhs = nullptr;
-  RHSLaunch nodes = nullptr;  // temperature recovery per node (k_lte_nodes), before rhs
-  RHSLaunch ns_grad = nullptr;  // Navier-Stokes pass 1 of the tabulated gas (after nodes)
-  RHSLaunch ns = nullptr;       // Navier-Stokes pass 2 (after ns_grad)
+  RHSLaunch rhs[2] = {};
+  RHSLaunch nodes = nullptr;       // temperature recovery per node (k_lte_nodes, FP64), before rhs
+  RHSLaunch ns_grad[2] = {};       // Navier-Stokes pass 1 of the tabulated gas (after nodes)
+  RHSLaunch ns[2] = {};            // Navier-Stokes pass 2 (after ns_grad)
+  RHSLaunch pref = nullptr;        // mixed precision: the background pressure of a state
   int tpb = 0, epb = 0;
 };
 
@@ -142,13 +144,17 @@ struct OrdersRegistrar {
   }
 };
 
-template <int DIM, int P, bool VISC = false>
+template <typename R, int DIM, int P, bool VISC = false>
 void launch_rhs_lte(const RHSArgs &a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_rhs_lte<DIM, P, VISC>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+  hipLaunchKernelGGL((k_rhs_lte<R, DIM, P, VISC>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
 }
-template <int DIM, int P>
+template <typename R, int DIM, int P>
 void launch_ns_grad_lte(const RHSArgs &a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_ns_grad<double, DIM, P, 0, false, true>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+  hipLaunchKernelGGL((k_ns_grad<R, DIM, P, 0, false, true>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+}
+template <int DIM>
+void launch_pref_lte(const RHSArgs &a, int, hipStream_t s) {
+  hipLaunchKernelGGL((lte::k_pref_lte<DIM>), dim3(1), dim3(64), 0, s, a.U, a.Ntot, a.lte, const_cast<double *>(a.pref));
 }
 
 template <int DIM>
@@ -161,10 +167,14 @@ template <int DIM, int P>
 struct LteRegistrar {
   LteRegistrar() {
     LteKernels k;
-    k.rhs = &launch_rhs_lte<DIM, P>;
+    k.rhs[0] = &launch_rhs_lte<double, DIM, P>;
+    k.rhs[1] = &launch_rhs_lte<float, DIM, P>;
     k.nodes = &launch_lte_nodes<DIM>;
-    k.ns_grad = &launch_ns_grad_lte<DIM, P>;
-    k.ns = &launch_rhs_lte<DIM, P, true>;
+    k.ns_grad[0] = &launch_ns_grad_lte<double, DIM, P>;
+    k.ns_grad[1] = &launch_ns_grad_lte<float, DIM, P>;
+    k.ns[0] = &launch_rhs_lte<double, DIM, P, true>;
+    k.ns[1] = &launch_rhs_lte<float, DIM, P, true>;
+    k.pref = &launch_pref_lte<DIM>;
     k.tpb = Shape<DIM, P>::TPB;
     k.epb = Shape<DIM, P>::EPB;
     register_lte(DIM, P, k);

@@ -1191,7 +1191,7 @@ __device__ __forceinline__ void line_contract_mfma5(const float (*F)[64], const 
 template <typename R, int DIM, int P, int EPBX, bool BND, bool LTE = false>
 struct GradWaves {
   static constexpr bool fits = sizeof(R) == sizeof(double) && DIM == 3 && P == 3 && EPBX == 0 && !BND && !LTE;
-  static constexpr int value = fits ? 5 : (GradMfma<R, DIM, P, EPBX>::on && !BND ? 8 : 1);  // 64 VGPRs
+  static constexpr int value = fits ? 5 : (GradMfma<R, DIM, P, EPBX>::on && !BND && !LTE ? 8 : 1);  // 64 VGPRs
 };
 
 // The tabulated gas on the BR1 path: the primitive gradient (0, grad u, grad T)
@@ -1199,7 +1199,7 @@ struct GradWaves {
 // viscous flux with the node's tabulated viscosity and conductivity, grad T
 // taken from the energy slot (grad_temperature, :133-142)
 template <typename R, int DIM>
-__device__ __forceinline__ void grad_entropy_to_grad_prim(const lte::ViscNode<DIM> &t, const R *, const R *dE,
+__device__ __forceinline__ void grad_entropy_to_grad_prim(const lte::ViscNode<DIM, R> &t, const R *, const R *dE,
                                                           R *dP) {
   const R T = t.T;
   dP[0] = R(0);
@@ -1208,7 +1208,7 @@ __device__ __forceinline__ void grad_entropy_to_grad_prim(const lte::ViscNode<DI
   dP[DIM + 1] = T * T * dE[DIM + 1];
 }
 template <typename R, int DIM>
-__device__ __forceinline__ void viscous_flux(const GasC<R> &g, const lte::ViscNode<DIM> &t, const R (*dP)[DIM + 2],
+__device__ __forceinline__ void viscous_flux(const GasC<R> &g, const lte::ViscNode<DIM, R> &t, const R (*dP)[DIM + 2],
                                              R *out) {
   R gt[DIM];
 #pragma unroll
@@ -1216,12 +1216,12 @@ __device__ __forceinline__ void viscous_flux(const GasC<R> &g, const lte::ViscNo
   viscous_flux_core<R, DIM>(t.mu, t.kappa, g.mu_bulk, t.u, gt, dP, out);
 }
 
-// LTE: the tabulated gas (FP64, periodic meshes); its nodes' temperatures and
-// cells come from k_lte_nodes of the same state
+// LTE: the tabulated gas (periodic meshes); its nodes' temperatures and
+// cells come from k_lte_nodes of the same state (FP64)
 template <typename R, int DIM, int P, int EPBX = 0, bool BND = false, bool LTE = false>
 __global__ void __launch_bounds__((Shape<DIM, P, EPBX>::TPB))
 __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND, LTE>::value))) k_ns_grad(RHSArgs a) {
-  static_assert(!LTE || (sizeof(R) == sizeof(double) && !BND), "the tabulated gas: FP64, periodic");
+  static_assert(!LTE || !BND, "the tabulated gas: periodic meshes");
   using S = Shape<DIM, P, EPBX>;
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NV = VisComp<DIM>::N;
@@ -1316,7 +1316,7 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND, LTE>::value))
   for (int c = 0; c < NQ; ++c) q[c] = R(q64[c]);
   if (valid) {
     R Sv[NQ];
-    if constexpr (LTE) lte::entropy_state<DIM>(a.lte, q, Town, cown, Sv);
+    if constexpr (LTE) lte::entropy_state<DIM>(a.lte, q, R(Town), cown, Sv);
     else entropy_state<R, DIM>(g, q, Sv);
 #pragma unroll
     for (int c = 0; c < NQ; ++c) sS[el][c][node] = Sv[c];
@@ -1339,7 +1339,7 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND, LTE>::value))
         R qr[NQ], Sn[NQ];
 #pragma unroll
         for (int c = 0; c < NQ; ++c) qr[c] = R(qn[r][c]);
-        if constexpr (LTE) lte::entropy_state<DIM>(a.lte, qr, Tn[LTE ? r : 0], cn[LTE ? r : 0], Sn);
+        if constexpr (LTE) lte::entropy_state<DIM>(a.lte, qr, R(Tn[LTE ? r : 0]), cn[LTE ? r : 0], Sn);
         else entropy_state<R, DIM>(g, qr, Sn);
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {
@@ -1411,9 +1411,9 @@ __attribute__((amdgpu_waves_per_eu(GradWaves<R, DIM, P, EPBX, BND, LTE>::value))
         for (int c = 0; c < NQ; ++c) acc[dd][c] = dir == 0 ? dref[c] * ad : acc[dd][c] + dref[c] * ad;
       }
     }
-    using Thermo = std::conditional_t<LTE, lte::ViscNode<DIM>, NodeThermo<R, DIM>>;
+    using Thermo = std::conditional_t<LTE, lte::ViscNode<DIM, R>, NodeThermo<R, DIM>>;
     Thermo th;
-    if constexpr (LTE) th = lte::visc_node<DIM>(a.lte, q, Town, cown);
+    if constexpr (LTE) th = lte::visc_node<DIM>(a.lte, q, R(Town), cown);
     else th = node_thermo<R, DIM>(g, q);
 
     // gradient = volume part / J + w * jump * n from the face through the
@@ -1594,7 +1594,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
     for (int c = 0; c < NQ; ++c) s.q[c] = a.U[c * Ntot + ebase];
     derive(g, s);
     const bool tab = a.lte.nx > 0;
-    lte::Eval ev;
+    lte::Eval<double> ev;
     if (tab) ev = lte::state_eval<DIM>(a.lte, s.q);
     const double c_s = tab ? sqrt(ev.c2()) : sound_speed(g, s);
     const double invJ = 1.0 / a.J[e];
@@ -1784,7 +1784,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_measures(MeasArgs a) {
     const double rhoE = q[DIM + 1];
     double press, temper;
     if (a.lte.nx > 0) {
-      const lte::Eval st = lte::state_eval<DIM>(a.lte, q);
+      const lte::Eval<double> st = lte::state_eval<DIM>(a.lte, q);
       press = st.pressure();
       temper = st.T;
     } else {

@@ -10,31 +10,46 @@
 // viscous volume divergence with the own half of the central viscous face
 // flux, and the neighbours' n.Fv, come from k_ns_grad<..., LTE> as for the
 // ideal gas (DGSEMIntegrator.hpp:261-316, 486-622).
+//
+// R = float is the mixed-precision form: the staged states and the pair and
+// face fluxes (the Itoh-Abe means of the interpolant) in FP32; each node's
+// temperature (Newton, k_lte_nodes), pressure and sound speed in FP64 once
+// per node, the momentum flux's pressure taken less a constant background
+// pressure (formed in FP64, as for the ideal gas), and the stage
+// combination in FP64 from the FP64 state.
 #pragma once
 
 #include "rhs_kernels.hpp"
 
 namespace thx {
 
-template <int DIM, int P, bool VISC = false>
+template <typename R, int DIM, int P, bool VISC = false>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   using S = Shape<DIM, P>;
-  using SC = RhsScratch<double, DIM, P>;
+  using SC = RhsScratch<R, DIM, P>;
+  constexpr bool FP64 = sizeof(R) == sizeof(double);
   constexpr int Np = S::Np, ND = S::NDOF, NF = S::NFP, NQ = S::NEQ, EPB = S::EPB, TPB = S::TPB;
   constexpr int NFACE = 2 * DIM * NF;
   constexpr int NFR = (EPB * NFACE + TPB - 1) / TPB;
   constexpr int NPL = SC::NPL, NPAIR = SC::NPAIR;
   constexpr int NVD = DIM + 1;  // viscous terms: momentum and energy
 
-  __shared__ double sq[EPB][NQ][ND];
-  __shared__ double sT[EPB][ND];     // recovered temperature
-  __shared__ double sp[EPB][ND];     // pressure
-  __shared__ double sc[EPB][ND];     // sound speed
+  constexpr int SCR = FP64 ? SC::DOUBLES : SC::ROWS * NQ;  // flux rows, in R
+
+  __shared__ R sq[EPB][NQ][ND];
+  __shared__ R sT[EPB][ND];     // recovered temperature
+  __shared__ R sp[EPB][ND];     // pressure
+  __shared__ R sc[EPB][ND];     // sound speed
+  __shared__ R spd[FP64 ? 1 : EPB][FP64 ? 1 : ND];  // mixed: pressure less the background
   __shared__ int scell[EPB][ND];     // table cell j*nx + i
-  __shared__ double sraw[EPB][SC::DOUBLES];
-  __shared__ double sD2[Np * Np];
+  __shared__ R sraw[EPB][SCR];
+  __shared__ R sD2[Np * Np];
   __shared__ ElemMeta<DIM, EPB> sm;
-  auto sscr = [&](int pe, int row, int q) -> double & { return sraw[pe][row * NQ + q]; };
+  auto sscr = [&](int pe, int row, int q) -> R & { return sraw[pe][row * NQ + q]; };
+  auto spm = [&](int pe, int n) -> R {  // the momentum flux's pressure
+    if constexpr (FP64) return sp[pe][n];
+    else return spd[pe][n];
+  };
 
   // a stage of a variable step of length 0 (end time reached) does nothing
   if (a.mode != 0 && a.case_dt && a.case_dt[0] == 0.0) return;
@@ -49,8 +64,9 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   const bool valid = lane_ok && e < a.ne;
   const long long Ntot = a.Ntot;
   const long long Gtot = (long long)a.ne * NFACE;
-  const double *Vd = static_cast<const double *>(a.Vd);
-  const double *Gf = static_cast<const double *>(a.Gf);
+  const R *Vd = static_cast<const R *>(a.Vd);
+  const R *Gf = static_cast<const R *>(a.Gf);
+  const double pref = (FP64 || !a.pref) ? 0.0 : a.pref[0];
 
   // ---- loads (unconditional, clamped), as in k_rhs
   constexpr int NFT = EPB * NFACE;
@@ -74,17 +90,20 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   meta_load_normals<DIM, EPB, TPB>(a, ebase_blk, mr);
   const double Town = tv.node_T[ebase];
   const int cown = tv.node_cell[ebase];
-  if (tid < Np * Np) sD2[tid] = bD2;
+  if (tid < Np * Np) sD2[tid] = R(bD2);
 
-  // ---- stage: each node's temperature and cell, recovered once per node by k_lte_nodes
+  // ---- stage: each node's temperature and cell, recovered once per node by
+  //      k_lte_nodes; pressure and sound speed evaluated in FP64
   meta_store<DIM, EPB, TPB>(mr, sm);
   if (valid) {
-    const lte::Eval ev = lte::at_packed(tv, qown[0], Town, cown);
+    const lte::Eval<double> ev = lte::at_packed<double>(tv, qown[0], Town, cown);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) sq[el][q][node] = qown[q];
-    sT[el][node] = ev.T;
-    sp[el][node] = ev.pressure();
-    sc[el][node] = sqrt(ev.c2());
+    for (int q = 0; q < NQ; ++q) sq[el][q][node] = R(qown[q]);
+    sT[el][node] = R(ev.T);
+    const double pn = ev.pressure();
+    sp[el][node] = R(pn);
+    if constexpr (!FP64) spd[el][node] = R(pn - pref);
+    sc[el][node] = R(sqrt(ev.c2()));
     scell[el][node] = (int)(ev.cell.H - tv.planes);
   }
   lds_sync();
@@ -96,19 +115,19 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   const double Je = sm.J[elc];
 
   // a staged node's evaluation (cell from LDS, no search)
-  auto node_eval = [&](int pe, int n) -> lte::Eval {
+  auto node_eval = [&](int pe, int n) -> lte::Eval<R> {
     const int off = scell[pe][n];
     const int ci = off % tv.nx, cj = off / tv.nx;
-    return lte::at_cell(tv, sq[pe][0][n], sT[pe][n], ci, cj);
+    return lte::at_cell<R>(tv, sq[pe][0][n], sT[pe][n], ci, cj);
   };
 
   // ---- volume: every GLL-line pair once (DGSEMIntegrator.hpp:124-218)
   constexpr int NPT = EPB * NPAIR;
   constexpr int NRND = (DIM * NPT + TPB - 1) / TPB;
   static_assert(DIM < 3 || (2 * NPT) / TPB > (NPT - 1) / TPB, "slot reuse needs a round boundary");
-  double rate[NQ];
+  R rate[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) rate[q] = 0.0;
+  for (int q = 0; q < NQ; ++q) rate[q] = R(0);
 #pragma unroll 1
   for (int rr = 0; rr < NRND; ++rr) {
     const int w = rr * TPB + tid;
@@ -124,17 +143,18 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
         pair_nodes<Np>(r - line * NPL, pa, pb);
         const int base = face_node<DIM, Np>(dir, 0, line);
         const int na = base + pa * sdir, nb = base + pb * sdir;
-        double q1[NQ], q2[NQ], met[DIM];
+        R q1[NQ], q2[NQ], met[DIM];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           q1[q] = sq[pe][q][na];
           q2[q] = sq[pe][q][nb];
         }
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) met[d] = sm.adj[pe][dir * DIM + d];
-        const lte::Eval s1 = node_eval(pe, na), s2 = node_eval(pe, nb);
-        double F[NQ];
-        lte::lteec_flux<DIM, false>(tv, q1, q2, s1, s2, sp[pe][na], sp[pe][nb], 0.0, 0.0, met, F);
+        for (int d = 0; d < DIM; ++d) met[d] = R(sm.adj[pe][dir * DIM + d]);
+        const lte::Eval<R> s1 = node_eval(pe, na), s2 = node_eval(pe, nb);
+        R F[NQ];
+        lte::lteec_flux<DIM, false>(tv, q1, q2, s1, s2, sp[pe][na], sp[pe][nb], R(0), R(0), met, F, spm(pe, na),
+                                    spm(pe, nb));
 #pragma unroll
         for (int q = 0; q < NQ; ++q) sscr(pe, (dir & 1) * NPAIR + r, q) = F[q];
       }
@@ -150,7 +170,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
         for (int m = 0; m < Np; ++m) {
           if (m == mi) continue;
           const int pr = m < mi ? pair_index<Np>(m, mi) : pair_index<Np>(mi, m);
-          const double coef = sD2[mi * Np + m];
+          const R coef = sD2[mi * Np + m];
 #pragma unroll
           for (int q = 0; q < NQ; ++q) rate[q] += coef * sscr(el, (dir & 1) * NPAIR + line * NPL + pr, q);
         }
@@ -158,15 +178,15 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
       lds_sync();
     }
   }
-  double du[NQ];
+  R du[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) du[q] = (-1.0 / Je) * rate[q];
+  for (int q = 0; q < NQ; ++q) du[q] = R(-1.0 / Je) * rate[q];
 
   // the neighbours' face traces (state, temperature, cell) and the RK base
   // state, loaded once the pair fluxes no longer hold registers
   double qn[NFR][NQ], Tn[NFR];
   int cn[NFR];
-  double gn[NFR][VISC ? NVD : 1];
+  R gn[NFR][VISC ? NVD : 1];
 #pragma unroll
   for (int r = 0; r < NFR; ++r) {
     const int fi = min(tid + r * TPB, NFT - 1);
@@ -206,53 +226,58 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
         const int d = f >> 1, s = f & 1;
         const int on = face_node<DIM, Np>(d, s, t);
         const bool interior = fnb[r] >= 0;
-        double qo[NQ], qt[NQ];
+        R qo[NQ], qt[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           qo[q] = sq[pe][q][on];
-          qt[q] = interior ? qn[r][q] : qo[q];
+          qt[q] = interior ? R(qn[r][q]) : qo[q];
         }
-        const lte::Eval so = node_eval(pe, on);
-        const double po = sp[pe][on], co = sc[pe][on];
-        lte::Eval st = so;
-        double pt = po, ct = co;
+        const lte::Eval<R> so = node_eval(pe, on);
+        const R po = sp[pe][on], co = sc[pe][on], pdo = spm(pe, on);
+        lte::Eval<R> st = so;
+        R pt = po, ct = co, pdt = pdo;
         if (interior) {
-          st = lte::at_packed(tv, qt[0], Tn[r], cn[r]);
-          pt = st.pressure();
-          ct = sqrt(st.c2());
+          // the neighbour's trace: pressure and sound speed in FP64
+          const lte::Eval<double> st64 = lte::at_packed<double>(tv, qn[r][0], Tn[r], cn[r]);
+          if constexpr (FP64) st = st64;
+          else st = lte::at_packed<R>(tv, qt[0], R(Tn[r]), cn[r]);
+          const double pn = st64.pressure();
+          pt = R(pn);
+          pdt = FP64 ? pt : R(pn - pref);
+          ct = R(sqrt(st64.c2()));
         }
-        double n[DIM];
+        R n[DIM];
 #pragma unroll
-        for (int dd = 0; dd < DIM; ++dd) n[dd] = sm.nrm[pe][f][dd];
+        for (int dd = 0; dd < DIM; ++dd) n[dd] = R(sm.nrm[pe][f][dd]);
         // canonical order (the low-side element first) by selects: one
         // evaluation per lane whichever side of the element it is on
         const bool own_low = s != 0;
-        double ql[NQ], qh[NQ];
+        R ql[NQ], qh[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           ql[q] = own_low ? qo[q] : qt[q];
           qh[q] = own_low ? qt[q] : qo[q];
         }
-        const lte::Eval sl = own_low ? so : st, sh = own_low ? st : so;
-        double F[NQ];
+        const lte::Eval<R> sl = own_low ? so : st, sh = own_low ? st : so;
+        R F[NQ];
         lte::lteec_flux<DIM, true>(tv, ql, qh, sl, sh, own_low ? po : pt, own_low ? pt : po, own_low ? co : ct,
-                                   own_low ? ct : co, n, F);
+                                   own_low ? ct : co, n, F, own_low ? pdo : pdt, own_low ? pdt : pdo);
         if constexpr (VISC) {
           // F* - 0.5 n.(Fv_L + Fv_R); the own side's -0.5 n.Fv is in Vd
 #pragma unroll
-          for (int c = 0; c < NVD; ++c) F[1 + c] -= interior ? 0.5 * gn[r][c] : 0.0;
+          for (int c = 0; c < NVD; ++c) F[1 + c] -= interior ? R(0.5) * gn[r][c] : R(0);
         }
         const double wown = 1.0 / (w0 * sm.J[pe]);
-        const double cs = (interior && !s) ? wown : -wown;
+        const R cs = R((interior && !s) ? wown : -wown);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) sscr(pe, fl, q) = cs * F[q];
       }
     }
   }
   lds_sync();
-  double fsum[NQ];
+  R fsum[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) fsum[q] = 0.0;
+  for (int q = 0; q < NQ; ++q) fsum[q] = R(0);
   if constexpr (VISC) {
     // the viscous volume divergence with the own half of the face term
 #pragma unroll
@@ -266,15 +291,16 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
       const int row = (2 * d + (hi ? 1 : 0)) * NF + face_index<DIM, Np>(d, i, j, k);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        const double v = sscr(el, row, q);
-        fsum[q] += on ? v : 0.0;
+        const R v = sscr(el, row, q);
+        fsum[q] += on ? v : R(0);
       }
     }
     const double dt = a.case_dt ? a.case_dt[0] : a.dt;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const double r = du[q] + fsum[q];
-      const double qo = sq[el][q][node];
+      const double r = FP64 ? double(du[q] + fsum[q]) : double(du[q]) + double(fsum[q]);
+      // the stage combination from the FP64 state (its LDS copy is FP32 in the mixed form)
+      const double qo = FP64 ? double(sq[el][q][node]) : a.U[q * Ntot + ebase];
       double o;
       if (a.mode == 0) o = r;
       else if (a.mode == 1) o = qo + dt * r;
