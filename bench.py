@@ -226,6 +226,32 @@ def _lte_run(viscous, steps, mixed=False, total_steps=50):
     return ms, finite, k_ms, r_ms, U0.size, u
 
 
+def _lte_entropy(u):
+    """Entropy production (FP64 check) of the FP64 and the mixed-precision
+    LTE right-hand sides at the state u (24^3, p=3)."""
+    import dataclasses
+
+    import torch
+
+    from theseus_amd import _abi as A
+    from theseus_amd.cases import c4_lte_tgv
+    from theseus_amd.operator import MakeRHSOperator
+
+    rt, mesh, _, table = c4_lte_tgv(n=24, order=3)
+    out = {}
+    du = torch.empty_like(u)
+    for name, r in (("fp64", rt), ("mixed", dataclasses.replace(rt, precision=A.PRECISION_MIXED))):
+        op = MakeRHSOperator(r, mesh, lte_table=table)
+        op.Mult(u, du)
+        P, M = op.EntropyProduction(u, du)
+        out[name] = P
+        if name == "fp64":
+            out["magnitude"] = M
+        op.close()
+    out["mixed_minus_fp64_over_magnitude"] = (out["mixed"] - out["fp64"]) / out["magnitude"]
+    return out
+
+
 def lte_leg(log, steps=10):
     """LTE Taylor-Green vortex, 24^3 hexes, p=3, LTEEC flux, the synthetic
     256 x 256 Helmholtz table, FP64: DOF*RHS-evals/s over SSP-RK3 steps; the
@@ -243,9 +269,12 @@ def lte_leg(log, steps=10):
     a = umx.reshape(5, -1)
     b = u64.reshape(5, -1)
     mx_err = [float(torch.linalg.norm(a[q] - b[q]) / torch.linalg.norm(b[q])) for q in range(5)]
+    # the FP64 entropy-production check of both right-hand sides at the FP64 state
+    entropy = _lte_entropy(u64)
     del u64, umx, a, b
     log(f"lte mixed: {mx_ms/steps:.3f} ms/step ({mx_value/value:.2f}x), kernel {mk_ms*1e3:.1f} us, "
-        f"state err after 50 steps vs FP64 {mx_err}")
+        f"state err after 50 steps vs FP64 {mx_err}, entropy production fp64 {entropy['fp64']:.6e} "
+        f"mixed {entropy['mixed']:.6e} (magnitude {entropy['magnitude']:.3e})")
     ns_ms, ns_finite, p2_ms, p1_ms, _, _ = _lte_run(True, steps)
     ns_value = dof * 3 * steps / (ns_ms * 1e-3)
     log(f"lte navier-stokes: {ns_ms/steps:.3f} ms/step, {ns_value:.3e} DOF*RHS/s, pass 2 {p2_ms*1e3:.1f} us, "
@@ -257,6 +286,7 @@ def lte_leg(log, steps=10):
             "mixed": {"value": mx_value, "ms_per_step": mx_ms / steps, "speedup_vs_fp64": mx_value / value,
                       "kernel_mean_ms": mk_ms, "finite": mx_finite,
                       "state_rel_l2_vs_fp64_per_field": mx_err, "state_after_steps": 50,
+                      "entropy_production": entropy,
                       "arithmetic": "FP32 pair and face fluxes (Itoh-Abe means on an FP32 copy of the table); "
                                     "FP64 temperature recovery, nodal pressure and sound speed, stage combination"},
             "navier_stokes": {"value": ns_value, "ms_per_step": ns_ms / steps, "pass2_mean_ms": p2_ms,

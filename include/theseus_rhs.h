@@ -286,7 +286,9 @@ THESEUS_API int theseus_rhs_integral_measures_noalloc(theseus_rhs *h, const doub
  * checks a mixed-precision right-hand side in FP64.  No counterpart in the
  * reference.  u and dudt are device L-vectors in the same layout (file or
  * aligned frames alike: the sum does not depend on the node order inside an
- * element).  Ideal gas only (THESEUS_ERR_UNSUPPORTED for the tabulated gas).
+ * element).  With the tabulated gas S = ((g - |v|^2/2)/T, v/T, -1/T) at the
+ * temperature recovered from u (LTEHelmholtzGasEOS.hpp:144-157), the
+ * gradient of the entropy density rho H_T(rho, T).
  * Per-workgroup partials and a fixed-order second pass (deterministic);
  * synchronises `stream`. */
 typedef struct theseus_entropy_production {

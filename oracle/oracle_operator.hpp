@@ -992,10 +992,11 @@ struct Measures {
 
 // Entropy production of a right-hand side (the engine's FP64 check of a
 // mixed-precision RHS; no counterpart in the reference): sum over elements
-// and nodes of w_i J_e S(U_i) . dU_i and of its absolute values, S the ideal
-// gas's entropy variables (EOS.hpp:206-234), w the GLL weights on [0,1]^dim.
+// and nodes of w_i J_e S(U_i) . dU_i and of its absolute values, S the gas's
+// entropy variables (EOS.hpp:206-234; the tabulated gas's,
+// LTEHelmholtzGasEOS.hpp:144-157), w the GLL weights on [0,1]^dim.
 inline void EntropyProduction(const Case &c, const double *U, const double *dU, double out[2]) {
-  if (c.lte) throw std::runtime_error("entropy production is implemented for the ideal gas");
+  const lte::ViscousGas tab{c.lte, c.dim, c.gas.mu_bulk};
   const size_t N = c.ntot();
   const int Np = c.Np, dim = c.dim, neq = c.neq;
   double prod = 0.0, mag = 0.0;
@@ -1003,7 +1004,8 @@ inline void EntropyProduction(const Case &c, const double *U, const double *dU, 
   for (int e = 0; e < c.ne; ++e)
     for (int node = 0; node < c.ndof; ++node) {
       c.gather(U, e, node, q);
-      c.gas.entropy_state(q, S);
+      if (c.lte) tab.entropy_state(q, S);
+      else c.gas.entropy_state(q, S);
       double sd = 0.0;
       for (int k = 0; k < neq; ++k) sd += S[k] * dU[k * N + (size_t)e * c.ndof + node];
       const int i = node % Np, j = dim > 1 ? (node / Np) % Np : 0, k = dim > 2 ? node / (Np * Np) : 0;

@@ -210,7 +210,7 @@ class OracleCase:
         return dict(zip(self.MEASURES, out.tolist()))
 
     def entropy_production(self, U, dU):
-        """(sum w J S(U).dU, sum w J |S(U).dU|) with the ideal gas's entropy variables."""
+        """(sum w J S(U).dU, sum w J |S(U).dU|) with the case gas's entropy variables."""
         U, dU = arr(U), arr(dU)
         out = np.zeros(2)
         if self._lib.oracle_entropy_production(self._h, _p(U), _p(dU), _p(out)):

@@ -5,7 +5,9 @@ mathematical entropy eta(U) = -rho s/(gamma-1), s = ln p - gamma ln rho
 (EOS.hpp:206-234), so sum w J S(U).V equals d/de sum w J eta(U + eV), here
 by a central difference of eta written independently in numpy.  An
 entropy-stable scheme (Ranocha volume flux, Lax-Friedrichs faces, BR1 viscous
-terms) on a periodic mesh produces no entropy: the production is <= 0."""
+terms) on a periodic mesh produces no entropy: the production is <= 0.
+For the tabulated gas the entropy density is rho H_T(rho, T) = -rho s, with T
+recovered from the internal energy and s from the table's interpolant."""
 import numpy as np
 import pytest
 
@@ -89,4 +91,45 @@ def test_entropy_stable_scheme_dissipates(flow, dim):
     U = _state(mesh, 3, dim, rng).reshape(dim + 2, -1)
     U = (U * (1 + 0.01 * rng.uniform(-1, 1, U.shape[1]))).reshape(-1)
     P, M = c.entropy_production(U, c.rhs(U))
+    assert P < -1e-6 * M, (P, M)
+
+
+def test_tabulated_gas_entropy_production_is_the_derivative_of_its_total_entropy():
+    """The same chain rule for the tabulated gas (LTEHelmholtzGasEOS.hpp:144-157):
+    eta = -rho s(rho, T) with T recovered from e and s = -H_T of the
+    interpolant, node by node through the oracle's table (a 2^3 p=2 case of
+    the LTE Taylor-Green vortex on a 64x64 synthetic table); and the LTEEC
+    operator on that periodic box dissipates (production < 0)."""
+    from oracle.pyoracle import LteTable
+    from theseus_amd.cases import c4_lte_tgv
+    from theseus_amd.lte import synthetic_table
+
+    table = synthetic_table(64, 64)
+    rt, mesh, U, _ = c4_lte_tgv(n=2, order=2, table=table)
+    desc, keep = build_descriptor(rt, mesh)
+    c = OracleCase(desc)
+    c._keep = keep
+    t = LteTable(arrays=(table.rho, table.T, table.e, table.planes.reshape(-1), table.inv.reshape(-1)))
+    t.attach(c)
+    Wall = _weights(mesh, 2, 3)
+
+    def total(Ux):
+        q = Ux.reshape(5, -1)
+        tot = 0.0
+        for k in range(q.shape[1]):
+            rho = q[0, k]
+            e = (q[4, k] - 0.5 * (q[1:4, k] @ q[1:4, k]) / rho) / rho
+            T, st, _ = t.recover(rho, e)
+            assert st == 0
+            tot += Wall[k] * (-rho * t.eval(rho, T)["entropy"])
+        return tot
+
+    rng = np.random.default_rng(11)
+    V = rng.uniform(-1, 1, U.size) * np.repeat(np.abs(U.reshape(5, -1)).max(axis=1), U.size // 5) * 1e-3
+    P, M = c.entropy_production(U, V)
+    eps = 1e-4
+    fd = (total(U + eps * V) - total(U - eps * V)) / (2 * eps)
+    assert abs(P - fd) <= 1e-6 * M, (P, fd, M)
+    Up = (U.reshape(5, -1) * (1 + 1e-3 * rng.uniform(-1, 1, U.size // 5))).reshape(-1)
+    P, M = c.entropy_production(Up, c.rhs(Up))
     assert P < -1e-6 * M, (P, M)

@@ -239,13 +239,15 @@ __global__ void __launch_bounds__(256) k_measures_final(const double *partial, i
 
 // Entropy production (theseus_rhs_entropy_production): one lane per node,
 // FP64; per-workgroup sums of w J S.dU and w J |S.dU| in a fixed tree order.
+// With the tabulated gas (lte.nx > 0) S = ((g - |u|^2/2)/T, u/T, -1/T) at the
+// node's recovered temperature (LTEHelmholtzGasEOS.hpp:144-157).
 struct GllWeights {
   double w[MAXNP];  // GLL weights on [0,1]
 };
 template <int DIM>
 __global__ void __launch_bounds__(256) k_entropy_production(const double *U, const double *dU, long long Ntot,
                                                              int ndof, int Np, const double *J, GasC<double> gas,
-                                                             GllWeights gw, double *partial) {
+                                                             lte::View lv, GllWeights gw, double *partial) {
   __shared__ double red[256][2];
   const int tid = threadIdx.x;
   const long long t = (long long)blockIdx.x * 256 + tid;
@@ -254,7 +256,20 @@ __global__ void __launch_bounds__(256) k_entropy_production(const double *U, con
     double q[DIM + 2], S[DIM + 2];
 #pragma unroll
     for (int c = 0; c < DIM + 2; ++c) q[c] = U[c * Ntot + t];
-    entropy_state<double, DIM>(gas, q, S);
+    if (lv.nx > 0) {
+      const lte::Eval<double> ev = lte::state_eval<DIM>(lv, q);
+      const double beta = 1 / ev.T;
+      double m2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) m2 += q[1 + d] * q[1 + d];
+      const double kinetic = 0.5 * m2 / q[0] / q[0];
+      S[0] = beta * (ev.gibbs() - kinetic);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) S[1 + d] = beta * (q[1 + d] / q[0]);
+      S[DIM + 1] = -beta;
+    } else {
+      entropy_state<double, DIM>(gas, q, S);
+    }
     double sd = 0.0;
 #pragma unroll
     for (int c = 0; c < DIM + 2; ++c) sd += S[c] * dU[c * Ntot + t];
@@ -1142,7 +1157,6 @@ int theseus_rhs_integral_measures(theseus_rhs *h, const double *u, theseus_integ
 int theseus_rhs_entropy_production(theseus_rhs *h, const double *u, const double *dudt,
                                    theseus_entropy_production *out, void *stream) {
   if (!h || !u || !dudt || !out) return fail(THESEUS_ERR_INVALID_ARGUMENT, "null handle, vector or output");
-  if (h->lk) return fail(THESEUS_ERR_UNSUPPORTED, "entropy production is implemented for the ideal gas");
   const long long nbl = (h->Ntot + 255) / 256;
   if (nbl > INT_MAX) return fail(THESEUS_ERR_INVALID_ARGUMENT, "too many nodes");
   const int nb = (int)nbl;
@@ -1156,10 +1170,10 @@ int theseus_rhs_entropy_production(theseus_rhs *h, const double *u, const double
   std::memcpy(gw.w, h->basis.w, sizeof gw.w);
   if (h->d.dim == 2)
     hipLaunchKernelGGL(k_entropy_production<2>, dim3(nb), dim3(256), 0, s, u, dudt, h->Ntot, ndof, Np, h->d_J, h->gas,
-                       gw, h->d_ent);
+                       h->ltev, gw, h->d_ent);
   else
     hipLaunchKernelGGL(k_entropy_production<3>, dim3(nb), dim3(256), 0, s, u, dudt, h->Ntot, ndof, Np, h->d_J, h->gas,
-                       gw, h->d_ent);
+                       h->ltev, gw, h->d_ent);
   hipLaunchKernelGGL(k_sum_pairs_final, dim3(1), dim3(256), 0, s, h->d_ent, nb, h->d_ent + 2 * (size_t)nb);
   HIP_TRY(hipGetLastError());
   double r[2];

@@ -231,9 +231,10 @@ class RHSOperator:
 
     def EntropyProduction(self, u, dudt, stream=None) -> tuple[float, float]:
         """FP64 entropy production of the right-hand side dudt at u:
-        (sum w J S(u).dudt, sum w J |S(u).dudt|), S the entropy variables
-        (EOS.hpp:206-234); <= 0 for an entropy-stable scheme on a periodic
-        mesh, whatever precision computed dudt (theseus_rhs_entropy_production)."""
+        (sum w J S(u).dudt, sum w J |S(u).dudt|), S the gas's entropy
+        variables (EOS.hpp:206-234; LTEHelmholtzGasEOS.hpp:144-157 for the
+        tabulated gas); <= 0 for an entropy-stable scheme on a periodic mesh,
+        whatever precision computed dudt (theseus_rhs_entropy_production)."""
         self._check(u, "u")
         self._check(dudt, "dudt")
         r = A.EntropyProduction()
