@@ -193,6 +193,14 @@ typedef struct theseus_lte_table {
   const double *e_grid;   /* [ny] strictly increasing: the inverse table's energy axis */
   const double *planes;   /* [6][ny][nx]: H, H_rho, H_T, H_rhoT, mu, lambda */
   const double *inverse;  /* [ny][nx]: temperature guess at (rho_i, e_j) */
+  /* Viscosity of the Navier-Stokes form (LTETransport.hpp:22-43): 0 = the
+   * mu plane, bilinear (the reference's default build); 1 = Sutherland's law
+   * mu0 (T0 + Ts) (T/T0)^1.5 / (T + Ts) at the recovered temperature (its
+   * SUTHERLAND build; the reference takes mu0 from the runTime key "mu0",
+   * default 1.716e-5, and T0 = 273.15, Ts = 110.4, Physics.hpp:40-42).  The
+   * conductivity stays the lambda plane either way. */
+  int viscosity_model;
+  double mu0, T0, Ts;
 } theseus_lte_table;
 
 /* MakeRHSOperator for the LTEHelmholtzGas with the LTEEC flux

@@ -306,6 +306,15 @@ int oracle_lte_viscous_flux(void *h, int dim, double mu_bulk, const double *q, c
   }
 }
 
+// The table's viscosity by Sutherland's law (the reference's SUTHERLAND build).
+void oracle_lte_set_sutherland(void *h, double mu0, double T0, double Ts) {
+  lte::Table &t = *static_cast<lte::Table *>(h);
+  t.sutherland = true;
+  t.mu0 = mu0;
+  t.T0 = T0;
+  t.Ts = Ts;
+}
+
 // The case's gas becomes the table (kept alive by the caller) with the LTEEC flux.
 void oracle_case_set_lte(void *c, void *t) { static_cast<Case *>(c)->lte = static_cast<lte::Table *>(t); }
 

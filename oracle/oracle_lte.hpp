@@ -28,6 +28,10 @@ struct Table {
   std::vector<double> rho, T, e;   // grids (e: the inverse table's energy axis)
   std::vector<double> planes;      // 6 * nx * ny
   std::vector<double> inv;         // nx * ny: temperature guess at (rho_i, e_j)
+  // viscosity by Sutherland's law instead of the mu plane (the reference's
+  // SUTHERLAND build, LTETransport.hpp:22-43)
+  bool sutherland = false;
+  double mu0 = 0, T0 = 0, Ts = 0;
   int index(int prop, int i, int j) const { return prop * nx * ny + j * nx + i; }
 };
 
@@ -331,6 +335,14 @@ inline double transport(const Table &t, const Eval &s, int prop) {
          p[t.nx + 1] * s.xi * s.eta;
 }
 
+// LTETransport.hpp:22-43: the mu plane, or with the SUTHERLAND build
+// viscosity_from_temperature at the recovered temperature
+inline double viscosity(const Table &t, const Eval &s) {
+  if (!t.sutherland) return transport(t, s, 4);
+  const double relative_temperature = s.T / t.T0;
+  return t.mu0 * (t.T0 + t.Ts) * relative_temperature * std::sqrt(relative_temperature) / (s.T + t.Ts);
+}
+
 // The tabulated gas on the viscous path, LTEGasModel<LTEHelmholtzGasEOS,
 // LTETransport> (LTEGasModel.hpp:339): viscosity and conductivity from the
 // table (LTETransport.hpp:32-65, the build without SUTHERLAND), the constant
@@ -344,7 +356,7 @@ struct ViscousGas {
 
   Eval state(const double *q) const { return evaluate_state(*t, q, dim); }
   double velocity(const double *q, int d) const { return q[1 + d] / q[0]; }
-  double viscosity(const double *q) const { return transport(*t, state(q), 4); }
+  double viscosity(const double *q) const { return lte::viscosity(*t, state(q)); }
   double thermal_conductivity(const double *q) const { return transport(*t, state(q), 5); }
   double bulk_viscosity(const double *) const { return mu_bulk; }
   // LTEHelmholtzGasEOS.hpp:133-142: the energy slot of the primitive gradient is grad T

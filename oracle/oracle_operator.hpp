@@ -895,7 +895,7 @@ inline void EstimateStability(const Case &c, const double *U, double out[3]) {
         double shear, conductivity, gamma, cp;
         if (c.lte) {  // LTETransport.hpp:32-65; gamma = cp/cv of the table (LTEHelmholtzEOS.hpp:71-91)
           const lte::Eval s = lte::evaluate_state(*c.lte, q, dim);
-          shear = lte::transport(*c.lte, s, 4);
+          shear = lte::viscosity(*c.lte, s);
           conductivity = lte::transport(*c.lte, s, 5);
           cp = s.cp();
           gamma = cp / s.cv();

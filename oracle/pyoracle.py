@@ -290,6 +290,7 @@ def _lte_sigs(lib):
     lib.oracle_lte_gas_point.restype = C.c_int
     lib.oracle_lte_viscous_flux.argtypes = [vp, C.c_int, C.c_double] + [_dp] * 5
     lib.oracle_lte_viscous_flux.restype = C.c_int
+    lib.oracle_lte_set_sutherland.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     lib._lte_ready = True
     return lib
 
@@ -370,6 +371,10 @@ class LteTable:
         if self._lib.oracle_lte_viscous_flux(self._h, dim, mu_bulk, _p(q), _p(dx), _p(dy), _p(dz), _p(F)) != 0:
             raise RuntimeError(self._lib.oracle_last_error().decode())
         return F.reshape(5, 3)[: dim + 2, :dim]
+
+    def set_sutherland(self, mu0, T0, Ts):
+        """Viscosity by Sutherland's law instead of the mu plane."""
+        self._lib.oracle_lte_set_sutherland(self._h, float(mu0), float(T0), float(Ts))
 
     def attach(self, case: "OracleCase"):
         """Make the oracle case use this table with the LTEEC flux."""

@@ -206,6 +206,8 @@ def _lte_arrays(nx=4, ny=4):
     ("ideal_flux", A.ERR_INVALID_ARGUMENT),
     ("bad_grid", A.ERR_INVALID_ARGUMENT),
     ("ns_with_boundaries", A.ERR_UNSUPPORTED),
+    ("bad_viscosity_model", A.ERR_INVALID_ARGUMENT),
+    ("bad_sutherland", A.ERR_INVALID_ARGUMENT),
 ])
 def test_create_lte_rejects_invalid_inputs(mutate, code):
     """theseus_rhs_create_lte validates before any device work; Navier-Stokes
@@ -225,6 +227,10 @@ def test_create_lte_rejects_invalid_inputs(mutate, code):
     t = A.LteTable()
     t.nx, t.ny = 4, 4
     t.rho_grid, t.T_grid, t.e_grid, t.planes, t.inverse = (a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrays)
+    if mutate == "bad_viscosity_model":
+        t.viscosity_model = 2
+    if mutate == "bad_sutherland":
+        t.viscosity_model, t.mu0, t.T0, t.Ts = 1, 1.716e-5, 0.0, 110.4  # T0 must be positive
     h = C.c_void_p()
     assert lib.theseus_rhs_create_lte(C.byref(d), C.byref(t), C.byref(h)) == code
     assert not h.value and lib.theseus_last_error().decode()

@@ -31,7 +31,10 @@ def _oracle(oracle, rt, mesh, table):
     c = oracle.OracleCase(desc)
     c._keep = keep
     t = oracle.LteTable(arrays=(table.rho, table.T, table.e, table.planes.reshape(-1), table.inv.reshape(-1)))
+    if getattr(table, "sutherland", None) is not None:
+        t.set_sutherland(*table.sutherland)
     t.attach(c)
+    c._table = t
     return c
 
 
@@ -261,3 +264,43 @@ def test_lte_mixed_full_size_state_per_field(table):
     _say(f"  state after 50 steps per-field rel L2 {serr}")
     assert max(serr[0], serr[1], serr[2], serr[4]) <= 1e-5, serr
     assert serr[3] <= 5e-4, serr
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lte_navier_stokes_sutherland_viscosity(oracle, table, precision):
+    """The SUTHERLAND build of LTETransport (LTETransport.hpp:22-43): viscosity
+    mu0 (T0 + Ts) (T/T0)^1.5 / (T + Ts) at each node's recovered temperature,
+    T0 = 273.15, Ts = 110.4 (Physics.hpp:40-42), mu0 = 1.716e-2 (the
+    reference's default 1.716e-5 scaled by 1e3 so that the viscous terms
+    matter), the conductivity still the lambda plane (scaled alike): RHS
+    <= 1e-12 in FP64 (its viscous part <= 1e-10), per field <= 1e-4 but the
+    energy (<= 1e-2) in mixed precision."""
+    import dataclasses
+
+    from theseus_amd import _abi as A
+    from theseus_amd.cases import c4_lte_tgv
+
+    planes = table.planes.copy()
+    planes[5] *= 1e3
+    st = dataclasses.replace(table, planes=planes, sutherland=(1.716e-2, 273.15, 110.4))
+    _say(f"case lte navier-stokes sutherland {precision}")
+    rt, mesh, U, _ = c4_lte_tgv(n=2, order=3, table=st, viscous=True)
+    rt_eu = dataclasses.replace(rt, flow_model=A.FLOW_EULER)
+    if precision == "mixed":
+        rt = dataclasses.replace(rt, precision=A.PRECISION_MIXED)
+    ora = _oracle(oracle, dataclasses.replace(rt, precision=A.PRECISION_FP64), mesh, st)
+    ref = ora.rhs(U)
+    got = _gpu_rhs(rt, mesh, st, U)
+    if precision == "fp64":
+        err = rel_l2(got, ref)
+        assert err <= 1e-12, (err, per_field_rel(got, ref, 5))
+        ref_eu = _oracle(oracle, rt_eu, mesh, st).rhs(U)
+        vis_ref = ref - ref_eu
+        assert np.abs(vis_ref).max() > 1e-3 * np.abs(ref).max()
+        vis_got = got - _gpu_rhs(rt_eu, mesh, st, U)
+        verr = rel_l2(vis_got, vis_ref)
+        assert verr <= 1e-10, verr
+    else:
+        errs = per_field_rel(got, ref, 5)
+        _say(f"  RHS per-field rel L2 {errs}")
+        assert max(errs[:4]) <= 1e-4 and errs[4] <= 1e-2, errs

@@ -500,3 +500,17 @@ def test_lte_navier_stokes_free_stream_and_conservation(small_table):
     Wall = np.tile(W, mesh.num_elements) * np.repeat(mesh.elem_jac, 64)
     for k in range(5):
         assert abs(r[k] @ Wall) <= 1e-11 * (np.abs(r[k]) @ Wall + 1e-300), k
+
+
+def test_lte_sutherland_viscosity_matches_reference_known_answer():
+    """helmholtz_gas_tests.cpp:154-159, the SUTHERLAND build: with mu0 = T0 =
+    Ts = 1 (:106-109) the device viscosity at the recovered T is
+    2 T sqrt(T) / (T + 1); the conductivity stays the lambda plane 4 + 2r + T."""
+    t = polynomial_table()
+    t.set_sutherland(1.0, 1.0, 1.0)
+    for k in range(5):
+        r, T = 0.75 + 0.125 * k, 3 + 0.25 * k
+        q = np.array([r, 3 * r, -2 * r, r, r * (2 * r + r * r / 2 + T * T + 7)])
+        v = t.gas_point(3, q)
+        assert abs(v["mu"] - 2 * T * np.sqrt(T) / (T + 1)) <= 1e-10 * max(1.0, v["mu"]), (k, v["mu"])
+        assert abs(v["lambda"] - (4 + 2 * r + T)) <= 1e-10 * (4 + 2 * r + T), (k, v["lambda"])

@@ -118,6 +118,10 @@ class LteTable(C.Structure):
         ("e_grid", C.POINTER(C.c_double)),
         ("planes", C.POINTER(C.c_double)),
         ("inverse", C.POINTER(C.c_double)),
+        ("viscosity_model", C.c_int),
+        ("mu0", C.c_double),
+        ("T0", C.c_double),
+        ("Ts", C.c_double),
     ]
 
 

@@ -744,6 +744,12 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
       return fail(THESEUS_ERR_INVALID_ARGUMENT, "the tabulated LTE gas uses the LTEEC volume and surface fluxes");
     if (lt->nx < 2 || lt->ny < 2 || !lt->rho_grid || !lt->T_grid || !lt->e_grid || !lt->planes || !lt->inverse)
       return fail(THESEUS_ERR_INVALID_ARGUMENT, "LTE table needs nx, ny >= 2 and all arrays");
+    if (lt->viscosity_model != 0 && lt->viscosity_model != 1)
+      return fail(THESEUS_ERR_INVALID_ARGUMENT, "LTE viscosity_model must be 0 (the mu plane) or 1 (Sutherland)");
+    if (lt->viscosity_model == 1 &&
+        (!(lt->mu0 >= 0) || !(lt->T0 > 0) || !(lt->Ts >= 0) || !std::isfinite(lt->mu0) || !std::isfinite(lt->T0) ||
+         !std::isfinite(lt->Ts)))
+      return fail(THESEUS_ERR_INVALID_ARGUMENT, "Sutherland viscosity needs finite mu0 >= 0, T0 > 0, Ts >= 0");
     const double *grids[3] = {lt->rho_grid, lt->T_grid, lt->e_grid};
     const int counts[3] = {lt->nx, lt->ny, lt->ny};
     for (int g = 0; g < 3; ++g)
@@ -850,6 +856,10 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     }
     h->ltev.node_T = h->d_lteT;
     h->ltev.node_cell = h->d_ltecell;
+    h->ltev.sutherland = lt->viscosity_model == 1;
+    h->ltev.mu0 = lt->mu0;
+    h->ltev.T0 = lt->T0;
+    h->ltev.Ts = lt->Ts;
     if (h->pm) {
       // the mixed-precision form's FP32 copy of the planes
       std::vector<float> p32(lt->planes, lt->planes + 6 * nx * ny);

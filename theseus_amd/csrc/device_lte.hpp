@@ -37,7 +37,17 @@ struct View {
   // recovered temperature and the table cell i | j << 16 (-1 outside)
   double *node_T = nullptr;
   int *node_cell = nullptr;
+  // viscosity of the Navier-Stokes form: the mu plane, or Sutherland's law
+  // mu0 (T0 + Ts) (T/T0)^1.5 / (T + Ts) when sutherland is set
+  int sutherland = 0;
+  double mu0 = 0, T0 = 0, Ts = 0;
 };
+
+// LTETransport.hpp:22-29 (viscosity_from_temperature)
+__device__ __forceinline__ double sutherland_viscosity(const View &v, double T) {
+  const double rel = T / v.T0;
+  return v.mu0 * (v.T0 + v.Ts) * rel * sqrt(rel) / (T + v.Ts);
+}
 
 // bisection: largest i with grid[i] <= value (count-2 at the top end); -1 outside
 __device__ __forceinline__ int find_cell(const double *grid, int count, double value) {
@@ -466,7 +476,7 @@ __device__ __forceinline__ ViscNode<DIM, R> visc_node(const View &v, const R *q,
 #pragma unroll
   for (int d = 0; d < DIM; ++d) n.u[d] = q[1 + d] / q[0];
   n.T = T;
-  n.mu = s.template transport<4>();
+  n.mu = v.sutherland ? R(sutherland_viscosity(v, double(T))) : s.template transport<4>();
   n.kappa = s.template transport<5>();
   return n;
 }

@@ -1619,7 +1619,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
       const double rho = s.q[0];
       double shear = g.mu, kappa = g.mu * g.cp * g.PrInverse, gam = g.gamma, cp = g.cp;
       if (tab) {
-        shear = ev.transport<4>();
+        shear = a.lte.sutherland ? lte::sutherland_viscosity(a.lte, ev.T) : ev.transport<4>();
         kappa = ev.transport<5>();
         cp = ev.cp();
         gam = cp / ev.cv();

@@ -28,6 +28,9 @@ class HelmholtzTable:
     e: np.ndarray        # (ny,) inverse-table energy axis
     planes: np.ndarray   # (6, ny, nx)
     inv: np.ndarray      # (ny, nx) temperature guess at (rho_i, e_j)
+    # viscosity by Sutherland's law (mu0, T0, Ts) instead of the mu plane
+    # (LTETransport.hpp:22-43, the reference's SUTHERLAND build); None: the plane
+    sutherland: tuple | None = None
 
     @property
     def nx(self) -> int:

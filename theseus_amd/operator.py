@@ -290,12 +290,18 @@ class NSOperator(RHSOperator):
 
 
 def lte_descriptor(table):
-    """theseus_lte_table from a theseus_amd.lte.HelmholtzTable; returns (desc, keepalive)."""
+    """theseus_lte_table from a theseus_amd.lte.HelmholtzTable; returns (desc,
+    keepalive).  A table whose `sutherland` attribute is (mu0, T0, Ts) takes
+    its viscosity from Sutherland's law (the reference's SUTHERLAND build)."""
     arrays = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
               for x in (table.rho, table.T, table.e, table.planes, table.inv)]
     d = A.LteTable()
     d.nx, d.ny = int(table.rho.size), int(table.T.size)
     d.rho_grid, d.T_grid, d.e_grid, d.planes, d.inverse = (a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrays)
+    suth = getattr(table, "sutherland", None)
+    if suth is not None:
+        d.viscosity_model = 1
+        d.mu0, d.T0, d.Ts = (float(x) for x in suth)
     return d, arrays
 
 
