@@ -4,7 +4,9 @@ Mirrors the reference factory's string handling (SimFactory_impl.hpp:63-128):
 lower-cased ``numerical_flux`` / ``flow_model`` / ``gas_model`` with the same
 aliases and precedence, and the PhysicsConstants defaults
 (SimFactory_impl.hpp:133-137, Physics.hpp:40: gamma 1.4, Pr 0.72,
-R_gas 287.05, mu 0.02, mu_bulk 2/3) plus ``alpha_max`` (src/Simulation.cpp:482-488).
+R_gas 287.05, mu 0.02, mu_bulk 2/3, and mu0 1.716e-5, the Sutherland
+reference viscosity of the tabulated gas's SUTHERLAND build) plus
+``alpha_max`` (src/Simulation.cpp:482-488).
 
 Two keys have no counterpart in the reference and exist for pairings the
 reference cannot select (one ``numerical_flux`` string fixes both fluxes
@@ -84,6 +86,7 @@ class RunTime:
     R_gas: float = 287.05
     mu: float = 0.02
     mu_bulk: float = 2.0 / 3.0
+    mu0: float = 1.716e-5
     alpha_max: float = 0.5
     alpha_min: float = 0.001
     blending: bool = True
@@ -131,6 +134,7 @@ class RunTime:
             Pr=float(runtime.get("Pr", 0.72)),
             R_gas=float(runtime.get("R_gas", 287.05)),
             mu=float(runtime.get("mu", 0.02)),
+            mu0=float(runtime.get("mu0", 1.716e-5)),
             alpha_max=float(runtime.get("alpha_max", 0.5)),
             blending=bool(runtime.get("blending", (compile_time or {}).get("SUBCELL_FV_BLENDING", True))),
             precision=A.PRECISION_MIXED if prec == "mixed" else A.PRECISION_FP64,

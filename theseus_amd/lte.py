@@ -41,6 +41,20 @@ class HelmholtzTable:
         return self.T.size
 
 
+# Physics.hpp:41-42: the Sutherland constants of the reference's SUTHERLAND build
+SUTHERLAND_T0 = 273.15
+SUTHERLAND_TS = 110.4
+
+
+def with_sutherland(table: HelmholtzTable, mu0: float = 1.716e-5, T0: float = SUTHERLAND_T0,
+                    Ts: float = SUTHERLAND_TS) -> HelmholtzTable:
+    """The table with its viscosity from Sutherland's law (LTETransport.hpp:22-43,
+    the reference's SUTHERLAND build; mu0 is the runTime key "mu0")."""
+    import dataclasses
+
+    return dataclasses.replace(table, sutherland=(mu0, T0, Ts))
+
+
 def log_grid(n: int, lo: float, hi: float) -> np.ndarray:
     """LTETable.hpp:94-102"""
     step = np.log10(hi / lo) / (n - 1)
