@@ -67,6 +67,11 @@ class RHSOperator {
   RHSOperator() = default;
   /* MakeRHSOperator + Finalize (SimFactory.hpp:22-31, RHSOperator_impl.hpp:273-314) */
   explicit RHSOperator(const theseus_case_desc &desc) { check_status(theseus_rhs_create(&desc, &h_)); }
+  /* The same with the tabulated LTE gas (LTEGasModel.hpp:339: LTEEC fluxes,
+   * Euler or Navier-Stokes; theseus_rhs_create_lte). */
+  RHSOperator(const theseus_case_desc &desc, const theseus_lte_table &table) {
+    check_status(theseus_rhs_create_lte(&desc, &table, &h_));
+  }
   explicit operator bool() const { return h_ != nullptr; }
   ~RHSOperator() {
     if (h_) theseus_rhs_destroy(h_);
@@ -141,6 +146,14 @@ class RHSOperator {
     theseus_integral_measures m{};
     check_status(theseus_rhs_integral_measures_baseline(h_, &m));
     return m;
+  }
+
+  /* FP64 entropy production (sum w J S(u).dudt, sum w J |S(u).dudt|) of a
+   * right-hand side; no counterpart in the reference (theseus_rhs_entropy_production). */
+  theseus_entropy_production EntropyProduction(const double *u, const double *dudt, void *stream = nullptr) const {
+    theseus_entropy_production e{};
+    check_status(theseus_rhs_entropy_production(h_, u, dudt, &e, stream));
+    return e;
   }
 
   /* Blending coefficient per element (device array of num_elements doubles). */

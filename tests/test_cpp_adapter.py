@@ -51,11 +51,24 @@ int main() {
     std::printf("invalid_argument: %s\n", e.what());
     seen |= 4;
   }
-  // the measure and device-clock members compile against the C-ABI (no device work here)
+  // the tabulated gas: blending is rejected before any device work
+  try {
+    static double grid[2] = {1.0, 2.0}, planes[24] = {0}, inv[4] = {1.5, 1.5, 1.5, 1.5};
+    theseus_lte_table t{};
+    t.nx = 2; t.ny = 2; t.rho_grid = grid; t.T_grid = grid; t.e_grid = grid; t.planes = planes; t.inverse = inv;
+    theseus_case_desc d = make(3, 3, THESEUS_FLOW_NAVIER_STOKES, THESEUS_PRECISION_FP64);
+    d.volume_flux = THESEUS_VOLUME_LTEEC; d.surface_flux = THESEUS_SURFACE_LTEEC; d.blending = 1;
+    theseus::RHSOperator op(d, t);
+  } catch (const std::invalid_argument &e) {
+    std::printf("lte: %s\n", e.what());
+    seen |= 256;
+  }
+  // the measure, device-clock and entropy members compile against the C-ABI (no device work here)
   auto m1 = &theseus::RHSOperator::ComputeIntegralMeasures;
   auto m2 = &theseus::RHSOperator::GetIntegralMeasuresBaseline;
   auto m3 = &theseus::RHSOperator::StepCFL;
-  (void)m1; (void)m2; (void)m3;
+  auto m4 = &theseus::RHSOperator::EntropyProduction;
+  (void)m1; (void)m2; (void)m3; (void)m4;
   static_assert(sizeof(theseus_clock) == 64, "clock layout");
   static_assert(sizeof(theseus_integral_measures) == 72, "measures layout");
   theseus::StabilityEstimate s;
@@ -81,7 +94,7 @@ int main() {
   }
   std::printf("seen=%d code=%d,%d nbr=%d,%d,%d,%d|%d,%d,%d,%d\n", seen, code[0], code[1], nbr[0], nbr[1], nbr[2], nbr[3],
               nbr[4], nbr[5], nbr[6], nbr[7]);
-  return seen == 255 ? 0 : 1;
+  return seen == 511 ? 0 : 1;
 }
 """
 
@@ -98,3 +111,4 @@ def test_cpp_adapter_compiles_links_and_raises(tmp_path):
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.count("invalid_argument:") == 3
     assert "dim" in out.stdout and "order" in out.stdout
+    assert "lte: " in out.stdout and "blending" in out.stdout
