@@ -363,12 +363,27 @@ __device__ __forceinline__ Gradient<R> discrete_gradient(const View &v, const Ev
   return g;
 }
 
-// LTEHelmholtzEOS.hpp:386-397
+// LTEHelmholtzEOS.hpp:386-397.  In FP64 the two discrete gradients are
+// taken in separate passes of a loop that is not unrolled: written one after
+// the other they share their cell evaluations, which keeps both gradients'
+// table values live at once (about 590 registers, spilled to scratch); apart
+// they fit the register file (256 VGPRs + 238 AGPRs in the 3-D p = 3 RHS
+// kernel, no scratch), and the RHS kernel runs 1.3x faster although the
+// shared evaluations are repeated.  Same arithmetic, same results.
 template <typename R>
 __device__ __forceinline__ void thermodynamic_means(const View &v, const Eval<R> &l, const Eval<R> &r, R &rho_mean,
                                                     R &e_mean) {
-  const Gradient<R> p = discrete_gradient<Q::pressure>(v, l, r);
-  const Gradient<R> g = discrete_gradient<Q::gibbs>(v, l, r);
+  Gradient<R> p, g;
+  if constexpr (sizeof(R) == sizeof(double)) {
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+      if (k == 0) p = discrete_gradient<Q::pressure>(v, l, r);
+      else g = discrete_gradient<Q::gibbs>(v, l, r);
+    }
+  } else {
+    p = discrete_gradient<Q::pressure>(v, l, r);
+    g = discrete_gradient<Q::gibbs>(v, l, r);
+  }
   rho_mean = p.density / g.density;
   e_mean = g.beta - p.beta / rho_mean;
 }
