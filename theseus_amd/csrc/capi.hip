@@ -1046,6 +1046,12 @@ static void enqueue_stability(theseus_rhs *h, const double *u, hipStream_t s) {
   a.gas = h->gas;
   a.w0 = h->basis.w[0];
   a.lte = h->ltev;
+  if (h->lk) {
+    // the tabulated gas: each node's temperature and cell recovered once
+    // (k_lte_nodes), instead of a Newton solve per node and per face point
+    h->lk->nodes(h->args(u, nullptr), 0, s);
+    a.lte_nodes = 1;
+  }
   h->sk->stability(a, h->nblocks(), s);
   hipLaunchKernelGGL(k_rates_fold, dim3(1), dim3(64), 0, s, h->d_rates);
 }

@@ -1568,6 +1568,7 @@ struct StabArgs {
   GasC<double> gas;
   double w0;
   lte::View lte;  // tabulated LTE gas: sound speed from the table when lte.nx > 0
+  int lte_nodes;  // the state's temperatures and cells are in lte.node_T / node_cell (k_lte_nodes)
 };
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -1595,7 +1596,9 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
     derive(g, s);
     const bool tab = a.lte.nx > 0;
     lte::Eval<double> ev;
-    if (tab) ev = lte::state_eval<DIM>(a.lte, s.q);
+    if (tab)
+      ev = a.lte_nodes ? lte::at_packed<double>(a.lte, s.q[0], a.lte.node_T[ebase], a.lte.node_cell[ebase])
+                       : lte::state_eval<DIM>(a.lte, s.q);
     const double c_s = tab ? sqrt(ev.c2()) : sound_speed(g, s);
     const double invJ = 1.0 / a.J[e];
     const double *adj_e = a.adj + (size_t)e * DIM * DIM;
@@ -1663,7 +1666,16 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_stability(StabArgs a) 
           double vo = 0.0;
 #pragma unroll
           for (int dd = 0; dd < DIM; ++dd) vo += o.v[dd] * n[dd];
-          const double c_o = a.lte.nx > 0 ? sqrt(lte::state_eval<DIM>(a.lte, o.q).c2()) : sound_speed(g, o);
+          double c_o;
+          if (a.lte.nx > 0) {
+            const size_t nbase = (size_t)nb * ND + nn;
+            const lte::Eval<double> eo =
+                a.lte_nodes ? lte::at_packed<double>(a.lte, o.q[0], a.lte.node_T[nbase], a.lte.node_cell[nbase])
+                            : lte::state_eval<DIM>(a.lte, o.q);
+            c_o = sqrt(eo.c2());
+          } else {
+            c_o = sound_speed(g, o);
+          }
           ws = fmax(ws, fabs(vo) + c_o * nm);
           wmax = fmax(wmax, fabs(1.0 / (a.w0 * a.J[nb])));
         } else {
