@@ -232,11 +232,12 @@ def test_lte_mixed_rhs_and_steps_per_field(oracle, table, viscous_table, viscous
     assert all(serr[q] <= 5e-3 for q in zmom), serr
 
 
-def test_lte_mixed_full_size_state_per_field(table):
-    """The LTE Taylor-Green workload at full size (24^3, p=3): the mixed
-    state after 50 SSP-RK3 steps against the FP64 engine's (itself within
-    1e-10 of the oracle over steps): <= 1e-5 per field but the z-momentum
-    (zero at the start: <= 5e-4)."""
+@pytest.mark.parametrize("viscous", [False, True])
+def test_lte_mixed_full_size_state_per_field(table, viscous):
+    """The LTE Taylor-Green workload at full size (24^3, p=3), Euler and
+    Navier-Stokes: the mixed state after 50 SSP-RK3 steps against the FP64
+    engine's (itself inside the oracle's own rounding spread over steps):
+    <= 1e-5 per field but the z-momentum (zero at the start: <= 5e-4)."""
     import dataclasses
 
     import torch
@@ -245,8 +246,8 @@ def test_lte_mixed_full_size_state_per_field(table):
     from theseus_amd.cases import c4_lte_tgv
     from theseus_amd.operator import MakeRHSOperator
 
-    _say("case lte mixed full size 24^3 p=3, 50 steps")
-    rt, mesh, U, _ = c4_lte_tgv(n=24, order=3, table=table)
+    _say(f"case lte mixed full size 24^3 p=3, 50 steps, viscous={viscous}")
+    rt, mesh, U, _ = c4_lte_tgv(n=24, order=3, table=table, viscous=viscous)
     out = {}
     dt = None
     for name, r in (("fp64", rt), ("mixed", dataclasses.replace(rt, precision=A.PRECISION_MIXED))):
