@@ -203,13 +203,20 @@ typedef struct theseus_lte_table {
   double mu0, T0, Ts;
 } theseus_lte_table;
 
-/* MakeRHSOperator for the LTEHelmholtzGas with the LTEEC flux
- * (LTEGasModel.hpp:339, LTEECFlux.hpp:93-107): volume_flux =
- * THESEUS_VOLUME_LTEEC, surface_flux = THESEUS_SURFACE_LTEEC, no blending.
- * precision FP64, or MIXED: FP32 pair and face fluxes (the discrete-gradient
- * means of the interpolant on an FP32 copy of the planes), with the
- * temperature recovery, each node's pressure and sound speed, the stage
- * combination and the stability estimate in FP64.  Euler, or
+/* MakeRHSOperator for the LTEHelmholtzGas (LTEGasModel.hpp:339) with one of
+ * the flux pairs the reference's factory builds for it
+ * (SimFactory_impl.hpp:140-245), as (volume_flux, surface_flux):
+ * (THESEUS_VOLUME_LTEEC, THESEUS_SURFACE_LTEEC), LTEECFlux.hpp:93-107;
+ * (THESEUS_VOLUME_PIROZZOLI, THESEUS_SURFACE_PIROZZOLI);
+ * (THESEUS_VOLUME_CENTRAL, THESEUS_SURFACE_HLL) or
+ * (THESEUS_VOLUME_CENTRAL, THESEUS_SURFACE_LLF); any other pair is
+ * THESEUS_ERR_INVALID_ARGUMENT.  The last three take pressure, sound speed
+ * and the Roe dissipation's gamma = cp/cv from the table at each node's
+ * recovered temperature.  No blending.
+ * precision FP64, or MIXED: FP32 pair and face fluxes (LTEEC: the
+ * discrete-gradient means of the interpolant on an FP32 copy of the planes),
+ * with the temperature recovery, each node's pressure and sound speed, the
+ * stage combination and the stability estimate in FP64.  Euler, or
  * Navier-Stokes (flow_model THESEUS_FLOW_NAVIER_STOKES, periodic meshes):
  * BR1 on the gas's entropy variables ((g - |u|^2/2)/T, u/T, -1/T), viscosity
  * and conductivity bilinear from the table's mu and lambda planes

@@ -67,8 +67,9 @@ class RHSOperator {
   RHSOperator() = default;
   /* MakeRHSOperator + Finalize (SimFactory.hpp:22-31, RHSOperator_impl.hpp:273-314) */
   explicit RHSOperator(const theseus_case_desc &desc) { check_status(theseus_rhs_create(&desc, &h_)); }
-  /* The same with the tabulated LTE gas (LTEGasModel.hpp:339: LTEEC fluxes,
-   * Euler or Navier-Stokes; theseus_rhs_create_lte). */
+  /* The same with the tabulated LTE gas (LTEGasModel.hpp:339: the LTEEC,
+   * Pirozzoli, HLL or LLF flux pairs, Euler or Navier-Stokes;
+   * theseus_rhs_create_lte). */
   RHSOperator(const theseus_case_desc &desc, const theseus_lte_table &table) {
     check_status(theseus_rhs_create_lte(&desc, &table, &h_));
   }

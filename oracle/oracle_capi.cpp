@@ -267,12 +267,12 @@ int oracle_lte_flux(void *h, int face, int dim, const double *q1, const double *
   }
 }
 
-// The tabulated gas at one conservative state q (oracle_lte.hpp ViscousGas):
+// The tabulated gas at one conservative state q (oracle_lte.hpp TableGas):
 // out = T, p, mu, lambda, cp, cv, the entropy variables [dim+2], and the
 // primitive gradient [dim+2] that the entropy-variable gradient dS converts to.
 int oracle_lte_gas_point(void *h, int dim, double mu_bulk, const double *q, const double *dS, double *out) {
   try {
-    const lte::ViscousGas g{static_cast<lte::Table *>(h), dim, mu_bulk};
+    const lte::TableGas g{static_cast<lte::Table *>(h), dim, mu_bulk};
     const lte::Eval s = g.state(q);
     const int neq = dim + 2;
     out[0] = s.T;
@@ -294,7 +294,7 @@ int oracle_lte_gas_point(void *h, int dim, double mu_bulk, const double *q, cons
 int oracle_lte_viscous_flux(void *h, int dim, double mu_bulk, const double *q, const double *dx, const double *dy,
                             const double *dz, double *F) {
   try {
-    const lte::ViscousGas g{static_cast<lte::Table *>(h), dim, mu_bulk};
+    const lte::TableGas g{static_cast<lte::Table *>(h), dim, mu_bulk};
     double f[MAXEQ][MAXDIM] = {{0}};
     ComputeViscousFluxKernel(g, q, dx, dy, dz, f);
     for (int e = 0; e < MAXEQ; ++e)

@@ -349,12 +349,32 @@ inline double viscosity(const Table &t, const Eval &s) {
 // bulk-viscosity ratio, the entropy variables ((g - |u|^2/2)/T, u/T, -1/T)
 // and their gradient conversion (LTEHelmholtzGasEOS.hpp:133-171).  The same
 // members as the ideal gas, so the Navier-Stokes assembly takes either.
-struct ViscousGas {
+struct TableGas {
   const Table *t = nullptr;
   int dim = 3;
   double mu_bulk = 2.0 / 3.0;
 
   Eval state(const double *q) const { return evaluate_state(*t, q, dim); }
+  // The accessors the Pirozzoli, HLL and LLF fluxes take from the gas
+  // (LTEGasModel.hpp:160-267, LTEHelmholtzGasEOS.hpp:40-110): the state's
+  // own density, momentum and energy; pressure, sound speed and
+  // gamma = cp/cv of the interpolant at the recovered temperature.
+  int neq() const { return dim + 2; }
+  double density(const double *q) const { return q[0]; }
+  double momentum(const double *q, int d) const { return q[1 + d]; }
+  double energy(const double *q) const { return q[dim + 1]; }
+  double kinetic_energy_density(const double *q) const {
+    double m2 = 0;
+    for (int d = 0; d < dim; ++d) m2 += q[1 + d] * q[1 + d];
+    return 0.5 * m2 / q[0];
+  }
+  double specific_internal_energy(const double *q) const { return (energy(q) - kinetic_energy_density(q)) / q[0]; }
+  double pressure(const double *q) const { return state(q).pressure(); }
+  double sound_speed(const double *q) const { return std::sqrt(state(q).sound_speed_squared()); }
+  double gamma_at(const double *q) const {
+    const Eval s = state(q);
+    return s.cp() / s.cv();
+  }
   double velocity(const double *q, int d) const { return q[1 + d] / q[0]; }
   double viscosity(const double *q) const { return lte::viscosity(*t, state(q)); }
   double thermal_conductivity(const double *q) const { return transport(*t, state(q), 5); }

@@ -163,7 +163,7 @@ struct Case {
   int flow, vol_flux, face_flux, blending;
   double alpha_max, alpha_min, sharpness_fac, modalThreshold;
   Gas gas;
-  const lte::Table *lte = nullptr;  // tabulated LTE gas with the LTEEC flux (volume and face) when set
+  const lte::Table *lte = nullptr;  // tabulated LTE gas (its flux pairing in vol_flux, face_flux) when set
   Basis basis;
   std::vector<double> J, adj;  // per element
   std::vector<int> nbr;        // [ne][2*dim]
@@ -339,15 +339,29 @@ struct Case {
 };
 
 // Flux and sound speed of the case's gas: the ideal gas (ABI flux ids) or
-// the tabulated LTE gas with LTEEC (LTEECFlux.hpp:93-107).
+// the tabulated LTE gas with the pairings the reference's factory builds for
+// it (SimFactory_impl.hpp:140-245): LTEEC (LTEECFlux.hpp:93-107), Pirozzoli,
+// or the central volume flux with an HLL or LLF face, the last three on the
+// table's pressure, sound speed and cp/cv (lte::TableGas).
 inline void CaseVolumeFlux(const Case &c, const double *q1, const double *q2, const double *m1, const double *m2,
                            double *F) {
-  if (c.lte) lte::lteec_flux(*c.lte, false, c.dim, q1, q2, m1, m2, F);
-  else VolumeFlux(c.vol_flux, c.gas, q1, q2, m1, m2, F);
+  if (!c.lte) VolumeFlux(c.vol_flux, c.gas, q1, q2, m1, m2, F);
+  else if (c.vol_flux == THESEUS_VOLUME_LTEEC) lte::lteec_flux(*c.lte, false, c.dim, q1, q2, m1, m2, F);
+  else {
+    const lte::TableGas g{c.lte, c.dim, c.gas.mu_bulk};
+    if (c.vol_flux == THESEUS_VOLUME_PIROZZOLI) PirozzoliVolume(g, q1, q2, m1, m2, F);
+    else CentralVolume(g, q1, q2, m1, m2, F);
+  }
 }
 inline void CaseFaceFlux(const Case &c, const double *q1, const double *q2, const double *n, double *F) {
-  if (c.lte) lte::lteec_flux(*c.lte, true, c.dim, q1, q2, n, n, F);
-  else FaceFlux(c.face_flux, c.gas, q1, q2, n, F);
+  if (!c.lte) FaceFlux(c.face_flux, c.gas, q1, q2, n, F);
+  else if (c.face_flux == THESEUS_SURFACE_LTEEC) lte::lteec_flux(*c.lte, true, c.dim, q1, q2, n, n, F);
+  else {
+    const lte::TableGas g{c.lte, c.dim, c.gas.mu_bulk};
+    if (c.face_flux == THESEUS_SURFACE_PIROZZOLI) PirozzoliFace(g, q1, q2, n, F);
+    else if (c.face_flux == THESEUS_SURFACE_HLL) HLLFace(g, q1, q2, n, F);
+    else LLFFace(g, q1, q2, n, F);
+  }
 }
 inline double CaseSoundSpeed(const Case &c, const double *q) {
   if (c.lte) return std::sqrt(lte::evaluate_state(*c.lte, q, c.dim).sound_speed_squared());
@@ -856,7 +870,7 @@ inline void RHS(const Case &c, const double *U, double *dudt, std::vector<double
     if (c.lte) {
       for (int nb : c.nbr)
         if (nb < 0) throw std::runtime_error("the tabulated gas with Navier-Stokes needs a periodic mesh");
-      NSRHS(c, lte::ViscousGas{c.lte, c.dim, c.gas.mu_bulk}, U, dudt, ap);
+      NSRHS(c, lte::TableGas{c.lte, c.dim, c.gas.mu_bulk}, U, dudt, ap);
     } else {
       NSRHS(c, c.gas, U, dudt, ap);
     }
@@ -996,7 +1010,7 @@ struct Measures {
 // entropy variables (EOS.hpp:206-234; the tabulated gas's,
 // LTEHelmholtzGasEOS.hpp:144-157), w the GLL weights on [0,1]^dim.
 inline void EntropyProduction(const Case &c, const double *U, const double *dU, double out[2]) {
-  const lte::ViscousGas tab{c.lte, c.dim, c.gas.mu_bulk};
+  const lte::TableGas tab{c.lte, c.dim, c.gas.mu_bulk};
   const size_t N = c.ntot();
   const int Np = c.Np, dim = c.dim, neq = c.neq;
   double prod = 0.0, mag = 0.0;

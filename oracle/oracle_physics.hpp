@@ -77,6 +77,7 @@ struct Gas {
     const double p = pressure(q);
     return std::sqrt(gamma * p / rho);
   }
+  double gamma_at(const double *) const { return gamma; }  // uniform gamma
   double viscosity(const double *) const { return mu; }
   double bulk_viscosity(const double *) const { return mu_bulk; }
   double thermal_conductivity(const double *q) const { return viscosity(q) * cp * PrInverse; }  // Transport.hpp:92-98
@@ -129,14 +130,16 @@ struct NormalFluxState {
   const double *conserved;
   double inverse_density, pressure, total_enthalpy;
 };
-inline NormalFluxState MakeInviscidNormalFluxState(const Gas &g, const double *q) {
+template <class G>
+inline NormalFluxState MakeInviscidNormalFluxState(const G &g, const double *q) {
   const double inverse_density = 1.0 / g.density(q);
   const double pressure = g.pressure(q);
   return {q, inverse_density, pressure, (g.energy(q) + pressure) * inverse_density};
 }
 
 // NavierStokesFlux.hpp:295-332
-inline double ComputeInviscidNormalFluxEquation(const Gas &g, const NormalFluxState &s, const double *normal,
+template <class G>
+inline double ComputeInviscidNormalFluxEquation(const G &g, const NormalFluxState &s, const double *normal,
                                                 int equation) {
   double result = 0.0;
   for (int direction = 0; direction < g.dim; ++direction) {
@@ -157,7 +160,8 @@ inline double ComputeInviscidNormalFluxEquation(const Gas &g, const NormalFluxSt
 }
 
 // NavierStokesFlux.hpp:246-270
-inline double MaximumNormalWaveSpeed(const Gas &g, const double *q1, const double *q2, const double *n) {
+template <class G>
+inline double MaximumNormalWaveSpeed(const G &g, const double *q1, const double *q2, const double *n) {
   double vd1 = 0.0, vd2 = 0.0, nn = 0.0;
   for (int d = 0; d < g.dim; ++d) {
     vd1 += g.velocity(q1, d) * n[d];
@@ -169,7 +173,8 @@ inline double MaximumNormalWaveSpeed(const Gas &g, const double *q1, const doubl
 }
 
 // NavierStokesFlux.hpp:339-387
-inline void ComputeInviscidFluxKernel(const Gas &g, const double *q, double F[MAXEQ][MAXDIM]) {
+template <class G>
+inline void ComputeInviscidFluxKernel(const G &g, const double *q, double F[MAXEQ][MAXDIM]) {
   const int dim = g.dim;
   const double density = g.density(q);
   const double spec_vol = 1.0 / density;
@@ -188,7 +193,7 @@ inline void ComputeInviscidFluxKernel(const Gas &g, const double *q, double F[MA
 }
 
 // NavierStokesFlux.hpp:160-290 (Cartesian: axisymmetric=false); G is the
-// ideal gas or the tabulated one (oracle_lte.hpp ViscousGas)
+// ideal gas or the tabulated one (oracle_lte.hpp TableGas)
 template <class G>
 inline void ComputeViscousFluxKernel(const G &g, const double *q, const double *dx, const double *dy,
                                      const double *dz, double vf[MAXEQ][MAXDIM]) {
@@ -304,7 +309,8 @@ inline void RanochaVolume(const Gas &g, const double *q1, const double *q2, cons
 }
 
 // PirozzoliFlux.hpp:19-71
-inline void PirozzoliVolume(const Gas &g, const double *q1, const double *q2, const double *met1, const double *met2,
+template <class G>
+inline void PirozzoliVolume(const G &g, const double *q1, const double *q2, const double *met1, const double *met2,
                             double *F) {
   const int dim = g.dim;
   double met[3] = {0, 0, 0};
@@ -330,7 +336,8 @@ inline void PirozzoliVolume(const Gas &g, const double *q1, const double *q2, co
 }
 
 // HLLFlux.hpp:19-51 (identical to RoeFlux.hpp:18-45 and LaxFriedrichsFlux.hpp:20-52)
-inline void CentralVolume(const Gas &g, const double *q1, const double *q2, const double *met1, const double *met2,
+template <class G>
+inline void CentralVolume(const G &g, const double *q1, const double *q2, const double *met1, const double *met2,
                           double *F) {
   const int dim = g.dim, neq = g.neq();
   double met[3] = {0, 0, 0};
@@ -350,7 +357,8 @@ inline void CentralVolume(const Gas &g, const double *q1, const double *q2, cons
 // ---------------------------------------------------------------------------
 
 // DissipationOperator.hpp:57-172
-inline void RoeDissipation(const Gas &g, const double *S1, const double *S2, const double *nor, double *diss) {
+template <class G>
+inline void RoeDissipation(const G &g, const double *S1, const double *S2, const double *nor, double *diss) {
   const int dim = g.dim, neq = g.neq(), ener = dim + 1;
   double vn1 = 0.0, vn2 = 0.0, vn_roe = 0.0, nor_mag = 0.0;
   const double rho1 = g.density(S1), rho2 = g.density(S2);
@@ -379,7 +387,7 @@ inline void RoeDissipation(const Gas &g, const double *S1, const double *S2, con
   vn1 *= inv_nor_mag;
   vn2 *= inv_nor_mag;
   vn_roe *= inv_nor_mag;
-  const double c_roe = std::sqrt((g.gamma - 1) * (roe[ener] - ke));
+  const double c_roe = std::sqrt((g.gamma_at(S1) - 1) * (roe[ener] - ke));
   double eig[3] = {0., std::abs(vn_roe), 0.};
   const double c1 = g.sound_speed(S1), c2 = g.sound_speed(S2);
   {
@@ -512,7 +520,8 @@ inline void RanochaFace(const Gas &g, const double *s1, const double *s2, const 
 }
 
 // PirozzoliFlux.hpp:73-120
-inline void PirozzoliFace(const Gas &g, const double *s1, const double *s2, const double *nor, double *flux) {
+template <class G>
+inline void PirozzoliFace(const G &g, const double *s1, const double *s2, const double *nor, double *flux) {
   const int dim = g.dim;
   const double rho1 = g.density(s1), rho2 = g.density(s2);
   const double rho_mean = 0.5 * (rho1 + rho2);
@@ -594,7 +603,8 @@ inline void RoeFace(const Gas &g, const double *qL, const double *qR, const doub
 }
 
 // HLLFlux.hpp:53-116
-inline void HLLFace(const Gas &g, const double *s1, const double *s2, const double *nor, double *flux) {
+template <class G>
+inline void HLLFace(const G &g, const double *s1, const double *s2, const double *nor, double *flux) {
   const int dim = g.dim, neq = g.neq();
   const NormalFluxState left = MakeInviscidNormalFluxState(g, s1);
   const NormalFluxState right = MakeInviscidNormalFluxState(g, s2);
@@ -623,7 +633,8 @@ inline void HLLFace(const Gas &g, const double *s1, const double *s2, const doub
 }
 
 // LaxFriedrichsFlux.hpp:188-211
-inline void LLFFace(const Gas &g, const double *s1, const double *s2, const double *nor, double *flux) {
+template <class G>
+inline void LLFFace(const G &g, const double *s1, const double *s2, const double *nor, double *flux) {
   const int neq = g.neq();
   const NormalFluxState left = MakeInviscidNormalFluxState(g, s1);
   const NormalFluxState right = MakeInviscidNormalFluxState(g, s2);

@@ -142,16 +142,25 @@ def test_numerical_flux_errors():
 
 
 def test_lte_gas_model_selection():
-    """SimFactory_impl.hpp:75-95: the Helmholtz LTE gas is gas_model
-    "lte_helmholtz", or "lte" with the LTEEC flux; the legacy LTE gas with
-    other fluxes is not implemented; LTEEC needs the LTE gas."""
+    """SimFactory_impl.hpp:75-95,140-145: the Helmholtz LTE gas is gas_model
+    "lte_helmholtz", or "lte" with the LTEEC flux, and takes the LTEEC,
+    Pirozzoli, HLL and LLF fluxes; the legacy LTE gas ("lte" with another
+    flux) is not implemented; LTEEC needs the LTE gas."""
     rt = RunTime.from_json({"dim": 3, "order": 3, "gas_model": "lte", "numerical_flux": "lteec"},
                            {"SUBCELL_FV_BLENDING": False})
     assert (rt.volume_flux, rt.surface_flux, rt.blending) == (A.VOLUME_LTEEC, A.SURFACE_LTEEC, False)
     with pytest.raises(NotImplementedError):
         RunTime.from_json({"dim": 2, "order": 3, "gas_model": "lte", "numerical_flux": "lfr"})
-    with pytest.raises(NotImplementedError):
-        RunTime.from_json({"dim": 2, "order": 3, "gas_model": "lte_helmholtz", "numerical_flux": "hll"})
+    # SimFactory_impl.hpp:140-145: the Helmholtz gas also takes Pirozzoli, HLL and LLF, nothing else
+    for name, pair in (("Pirozzoli", (A.VOLUME_PIROZZOLI, A.SURFACE_PIROZZOLI)),
+                       ("HLL", (A.VOLUME_CENTRAL, A.SURFACE_HLL)), ("LLF", (A.VOLUME_CENTRAL, A.SURFACE_LLF))):
+        rt = RunTime.from_json({"dim": 3, "order": 3, "gas_model": "lte_helmholtz", "numerical_flux": name},
+                               {"SUBCELL_FV_BLENDING": False})
+        assert (rt.volume_flux, rt.surface_flux, rt.lte) == (*pair, True)
+    assert not RunTime.from_json({"dim": 2, "order": 3, "numerical_flux": "hll"}).lte
+    for name in ("Ranocha", "Roe", "Chandrashekar"):
+        with pytest.raises(ValueError, match="Helmholtz LTE supports"):
+            RunTime.from_json({"dim": 2, "order": 3, "gas_model": "lte_helmholtz", "numerical_flux": name})
     with pytest.raises(ValueError):
         RunTime.from_json({"dim": 2, "order": 3, "numerical_flux": "lteec"})
     assert RunTime.from_json({"dim": 2, "order": 3}, {"SUBCELL_FV_BLENDING": False}).blending is False
@@ -204,6 +213,8 @@ def _lte_arrays(nx=4, ny=4):
 @pytest.mark.parametrize("mutate,code", [
     ("blending", A.ERR_UNSUPPORTED),
     ("ideal_flux", A.ERR_INVALID_ARGUMENT),
+    ("central_roe", A.ERR_INVALID_ARGUMENT),
+    ("central_pirozzoli", A.ERR_INVALID_ARGUMENT),
     ("bad_grid", A.ERR_INVALID_ARGUMENT),
     ("ns_with_boundaries", A.ERR_UNSUPPORTED),
     ("bad_viscosity_model", A.ERR_INVALID_ARGUMENT),
@@ -217,6 +228,10 @@ def test_create_lte_rejects_invalid_inputs(mutate, code):
                  surface_flux=A.SURFACE_LTEEC, blending=mutate == "blending")
     if mutate == "ideal_flux":
         rt.volume_flux = A.VOLUME_RANOCHA
+    if mutate == "central_roe":  # not one of the gas's pairings (SimFactory_impl.hpp:140-245)
+        rt.volume_flux, rt.surface_flux = A.VOLUME_CENTRAL, A.SURFACE_ROE
+    if mutate == "central_pirozzoli":
+        rt.volume_flux, rt.surface_flux = A.VOLUME_CENTRAL, A.SURFACE_PIROZZOLI
     if mutate == "ns_with_boundaries":
         rt.flow_model = A.FLOW_NAVIER_STOKES
     mesh = box_mesh(3, (2, 2, 2), 0.0, 1.0, periodic=mutate != "ns_with_boundaries")

@@ -1,0 +1,194 @@
+"""GPU parity of the tabulated LTE gas with the Pirozzoli, HLL and LLF flux
+pairings that the reference's factory builds for it (SimFactory_impl.hpp:
+140-245) against the CPU FP64 oracle on the synthetic Helmholtz table: FP64
+RHS relative L2 <= 1e-12 (Euler and Navier-Stokes, every compiled shape),
+the state after SSP-RK3 steps <= 1e-10, transmissive boundaries, and the
+mixed-precision form per field as for LTEEC (density and momenta <= 1e-4,
+energy <= 1e-2; 20 steps <= 5e-4 per field, the 3-D z-momentum <= 5e-3)."""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from .gpu_helpers import per_field_rel, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FLUXES = ["pirozzoli", "hll", "llf"]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def table():
+    from theseus_amd.lte import synthetic_table
+
+    return synthetic_table()
+
+
+@pytest.fixture(scope="module")
+def viscous_table(table):
+    """mu and lambda planes scaled by 1e3, so the viscous terms matter."""
+    planes = table.planes.copy()
+    planes[4:6] *= 1e3
+    return dataclasses.replace(table, planes=planes)
+
+
+def _oracle(oracle, rt, mesh, table):
+    from theseus_amd.operator import build_descriptor
+
+    desc, keep = build_descriptor(rt, mesh)
+    c = oracle.OracleCase(desc)
+    c._keep = keep
+    t = oracle.LteTable(arrays=(table.rho, table.T, table.e, table.planes.reshape(-1), table.inv.reshape(-1)))
+    t.attach(c)
+    c._table = t
+    return c
+
+
+def _gpu_rhs(rt, mesh, table, U):
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+
+    op = MakeRHSOperator(rt, mesh, lte_table=table)
+    u = torch.from_numpy(U).cuda()
+    du = torch.empty_like(u)
+    op.Mult(u, du)
+    torch.cuda.synchronize()
+    got = du.cpu().numpy()
+    op.close()
+    return got
+
+
+def _gpu_steps(rt, mesh, table, U, dt, steps):
+    import torch
+
+    from theseus_amd.operator import MakeRHSOperator
+
+    op = MakeRHSOperator(rt, mesh, lte_table=table)
+    u = torch.from_numpy(U).cuda()
+    t = 0.0
+    for _ in range(steps):
+        t = op.Step(u, t, dt)
+    torch.cuda.synchronize()
+    res = u.cpu().numpy()
+    op.close()
+    return res
+
+
+SHAPES = [(3, 2, 3), (3, 3, 2), (3, 2, 4), (3, 5, 1), (2, 3, 3), (2, 4, 2), (2, 3, 4)]
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("dim,n,order", SHAPES)
+def test_lte_pairing_euler_rhs_parity(oracle, table, flux, dim, n, order):
+    from theseus_amd.cases import c4_lte_tgv
+
+    _say(f"case lte {flux} euler dim={dim} n={n} p={order}")
+    rt, mesh, U, _ = c4_lte_tgv(n=n, order=order, table=table, dim=dim, flux=flux)
+    ref = _oracle(oracle, rt, mesh, table).rhs(U)
+    got = _gpu_rhs(rt, mesh, table, U)
+    assert np.isfinite(got).all()
+    err = rel_l2(got, ref)
+    assert err <= 1e-12, (err, per_field_rel(got, ref, dim + 2))
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("dim,n,order", [(3, 2, 3), (3, 5, 1), (2, 3, 3), (2, 3, 4)])
+def test_lte_pairing_navier_stokes_rhs_parity(oracle, viscous_table, flux, dim, n, order):
+    """Full RHS <= 1e-12; the viscous part (Navier-Stokes minus Euler) <= 1e-10."""
+    from theseus_amd import _abi as A
+    from theseus_amd.cases import c4_lte_tgv
+
+    _say(f"case lte {flux} navier-stokes dim={dim} n={n} p={order}")
+    rt, mesh, U, _ = c4_lte_tgv(n=n, order=order, table=viscous_table, viscous=True, dim=dim, flux=flux)
+    ref = _oracle(oracle, rt, mesh, viscous_table).rhs(U)
+    got = _gpu_rhs(rt, mesh, viscous_table, U)
+    assert np.isfinite(got).all()
+    err = rel_l2(got, ref)
+    assert err <= 1e-12, (err, per_field_rel(got, ref, dim + 2))
+    rte = dataclasses.replace(rt, flow_model=A.FLOW_EULER)
+    vis_ref = ref - _oracle(oracle, rte, mesh, viscous_table).rhs(U)
+    vis_got = got - _gpu_rhs(rte, mesh, viscous_table, U)
+    assert np.abs(vis_ref).max() > 5e-4 * np.abs(ref).max()
+    verr = rel_l2(vis_got, vis_ref)
+    assert verr <= 1e-10, (verr, per_field_rel(vis_got, vis_ref, dim + 2))
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("viscous", [False, True])
+def test_lte_pairing_steps_parity(oracle, table, viscous_table, flux, viscous):
+    """10 SSP-RK3 steps at CFL 0.5 of the oracle's stability estimate, 3-D
+    3^3 hexes p=3: FP64 state <= 1e-10."""
+    from theseus_amd.cases import c4_lte_tgv
+
+    tab = viscous_table if viscous else table
+    _say(f"case lte {flux} steps viscous={viscous}")
+    rt, mesh, U, _ = c4_lte_tgv(n=3, order=3, table=tab, viscous=viscous, flux=flux)
+    ora = _oracle(oracle, rt, mesh, tab)
+    rates = ora.stability(U)
+    dt = 0.5 / (max(rates[0], rates[1]) + rates[2])
+    ref = U.copy()
+    for _ in range(10):
+        ref = ora.step(ref, dt)
+    res = _gpu_steps(rt, mesh, tab, U, dt, 10)
+    err = rel_l2(res, ref)
+    assert err <= 1e-10, (err, per_field_rel(res, ref, 5))
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lte_pairing_transmissive_boundaries(oracle, table, flux, dim):
+    """Supersonic-outflow boundaries on every side (bc_kernels.hpp:236-238:
+    F*(q, q, n)): FP64 RHS <= 1e-12."""
+    from theseus_amd import _abi as A
+    from theseus_amd.cases import lte_taylor_green
+    from theseus_amd.config import PAIRINGS, RunTime
+    from theseus_amd.mesh import box_mesh
+
+    _say(f"case lte {flux} transmissive dim={dim}")
+    mesh = box_mesh(dim, 3, 0.0, 2 * np.pi, periodic=False, bc=A.BC_SUPERSONIC_OUTFLOW)
+    U = lte_taylor_green(mesh, 3, table)
+    vol, surf = PAIRINGS[flux]
+    rt = RunTime(dim=dim, order=3, flow_model=A.FLOW_EULER, volume_flux=vol, surface_flux=surf, blending=False,
+                 lte=True)
+    ref = _oracle(oracle, rt, mesh, table).rhs(U)
+    got = _gpu_rhs(rt, mesh, table, U)
+    err = rel_l2(got, ref)
+    assert err <= 1e-12, (err, per_field_rel(got, ref, dim + 2))
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("viscous", [False, True])
+@pytest.mark.parametrize("dim,n", [(3, 3), (2, 4)])
+def test_lte_pairing_mixed_per_field(oracle, table, viscous_table, flux, viscous, dim, n):
+    from theseus_amd import _abi as A
+    from theseus_amd.cases import c4_lte_tgv
+
+    tab = viscous_table if viscous else table
+    _say(f"case lte {flux} mixed dim={dim} n={n} viscous={viscous}")
+    rt, mesh, U, _ = c4_lte_tgv(n=n, order=3, table=tab, viscous=viscous, dim=dim, flux=flux)
+    rtm = dataclasses.replace(rt, precision=A.PRECISION_MIXED)
+    ora = _oracle(oracle, rt, mesh, tab)
+    ref = ora.rhs(U)
+    got = _gpu_rhs(rtm, mesh, tab, U)
+    nq = dim + 2
+    errs = per_field_rel(got, ref, nq)
+    _say(f"  RHS per-field rel L2 {errs}")
+    assert max(errs[:nq - 1]) <= 1e-4 and errs[nq - 1] <= 1e-2, errs
+    rates = ora.stability(U)
+    dt = 0.5 / (max(rates[0], rates[1]) + rates[2])
+    ref_u = U.copy()
+    for _ in range(20):
+        ref_u = ora.step(ref_u, dt)
+    res = _gpu_steps(rtm, mesh, tab, U, dt, 20)
+    serr = per_field_rel(res, ref_u, nq)
+    _say(f"  state after 20 steps per-field rel L2 {serr}")
+    zmom = [3] if dim == 3 else []
+    assert max(e for q, e in enumerate(serr) if q not in zmom) <= 5e-4, serr
+    assert all(serr[q] <= 5e-3 for q in zmom), serr

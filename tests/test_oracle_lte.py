@@ -221,12 +221,12 @@ def test_lteec_is_entropy_conservative_on_the_synthetic_table(synth):
         assert abs(lhs - rhs) <= 1e-9 * scale, (k, lhs, rhs)
 
 
-def _lte_case(n, order, table):
+def _lte_case(n, order, table, flux="lteec"):
     from theseus_amd.cases import c4_lte_tgv
     from theseus_amd.operator import build_descriptor
     from oracle.pyoracle import OracleCase
 
-    rt, mesh, U, _ = c4_lte_tgv(n=n, order=order, table=table)
+    rt, mesh, U, _ = c4_lte_tgv(n=n, order=order, table=table, flux=flux)
     desc, keep = build_descriptor(rt, mesh)
     c = OracleCase(desc)
     c._keep = keep
@@ -255,12 +255,13 @@ def test_product_table_builder_matches_the_oracle_builder():
     np.testing.assert_allclose(t.inv.reshape(-1), inv, rtol=1e-12)
 
 
-def test_lte_operator_free_stream_and_conservation(small_table):
+@pytest.mark.parametrize("flux", ["lteec", "pirozzoli", "hll", "llf"])
+def test_lte_operator_free_stream_and_conservation(small_table, flux):
     """Uniform flow gives dU = 0 to round-off; on a periodic box the
-    w_i J_i-weighted sum of dU vanishes (LTEEC is conservative)."""
+    w_i J_i-weighted sum of dU vanishes (every pairing is conservative)."""
     from theseus_amd.mesh import gll_nodes
 
-    rt, mesh, U, c = _lte_case(2, 3, small_table)
+    rt, mesh, U, c = _lte_case(2, 3, small_table, flux)
     N = mesh.num_elements * 64
     q = U.reshape(5, -1)[:, 1].copy()
     Uc = np.repeat(q, N)
@@ -514,3 +515,42 @@ def test_lte_sutherland_viscosity_matches_reference_known_answer():
         v = t.gas_point(3, q)
         assert abs(v["mu"] - 2 * T * np.sqrt(T) / (T + 1)) <= 1e-10 * max(1.0, v["mu"]), (k, v["mu"])
         assert abs(v["lambda"] - (4 + 2 * r + T)) <= 1e-10 * (4 + 2 * r + T), (k, v["lambda"])
+
+
+@pytest.mark.parametrize("fam", ["pirozzoli", "hll", "llf"])
+@pytest.mark.parametrize("flow", ["euler", "ns"])
+def test_lte_pairings_equal_the_ideal_gas_on_an_ideal_gas_table(fam, flow):
+    """The Pirozzoli, HLL and LLF pairings of the tabulated gas
+    (SimFactory_impl.hpp:140-245) take pressure, sound speed and the Roe
+    dissipation's gamma = cp/cv from the table at the recovered temperature
+    (LTEGasModel.hpp:205-267).  On a table of the calorically perfect gas they
+    reduce to the ideal gas's fluxes, which the reference's HLL cavity golden
+    and the flux known answers pin; left over is the interpolant's O(h^2)
+    c_v in the sound speed and gamma.  A wrong accessor (p, c or gamma) is an
+    O(1) error."""
+    from oracle.pyoracle import OracleCase
+    from theseus_amd import _abi as A
+    from theseus_amd.config import PAIRINGS, RunTime
+    from theseus_amd.mesh import box_mesh
+    from theseus_amd.operator import build_descriptor
+
+    R, gamma, mu, Pr = 1.0, 1.4, 0.05, 0.71
+    mesh = box_mesh(3, 2, 0.0, 2 * np.pi, periodic=True)
+    U, rho, T = _tgv_state(mesh, 3, R)
+    table = ideal_gas_table(rho.min() * 0.95, rho.max() * 1.05, T.min() * 0.95, T.max() * 1.05, R, gamma, mu, Pr)
+    vol, surf = PAIRINGS[fam]
+    fm = A.FLOW_NAVIER_STOKES if flow == "ns" else A.FLOW_EULER
+    out = []
+    for lte in (False, True):
+        rt = RunTime(dim=3, order=3, flow_model=fm, volume_flux=vol, surface_flux=surf, gamma=gamma, Pr=Pr,
+                     R_gas=R, mu=mu, blending=False)
+        desc, keep = build_descriptor(rt, mesh)
+        c = OracleCase(desc)
+        if lte:
+            table.attach(c)
+        out.append(c.rhs(U))
+    ri, rt_ = out
+    scale = np.abs(ri).max()
+    assert scale > 1.0
+    err = np.abs(rt_ - ri).max() / scale
+    assert err <= 1e-9, err  # 6e-11 here

@@ -252,7 +252,7 @@ def load_case_config(cfg: dict, mesh_path, lte_table=None, bc_types=None) -> Cas
     ic = cfg["runTime"]["conditions"]["initial_conditions"]
     U0 = initial_condition(ic["function"], ic.get("params", {}), mesh, rt.order)
     table = lte_table
-    if table is None and rt.volume_flux == cases.A.VOLUME_LTEEC:
+    if table is None and (rt.lte or rt.volume_flux == cases.A.VOLUME_LTEEC):
         from .lte import synthetic_table
 
         table = synthetic_table(int(rtj.get("N_rho", 101)), int(rtj.get("N_T", 101)),

@@ -226,16 +226,21 @@ def lte_taylor_green(mesh: BoxMesh, order: int, table, rho0: float = 0.1, T0: fl
     return _pack(rho, mom, E)
 
 
-def c4_lte_tgv(n: int = 24, order: int = 3, table=None, viscous: bool = False, dim: int = 3):
+def c4_lte_tgv(n: int = 24, order: int = 3, table=None, viscous: bool = False, dim: int = 3, flux: str = "lteec"):
     """3-D LTE Taylor-Green vortex, periodic [0,2pi]^3, LTEEC flux, the
     synthetic dissociating-gas table (256 x 256, log rho in [-4,1], log T in
     [2.3,4.2]), rho0 = 0.1, T0 = 3500 K, amplitude Mach 0.3.  viscous: the
     Navier-Stokes form (BR1, viscosity and conductivity from the table's
-    mu and lambda planes); dim = 2: the planar vortex on [0,2pi]^2."""
+    mu and lambda planes); dim = 2: the planar vortex on [0,2pi]^2; flux:
+    another pairing of the tabulated gas ("pirozzoli", "hll", "llf")."""
+    from .config import LTE_FAMILIES, PAIRINGS
     from .lte import synthetic_table
 
+    if flux not in LTE_FAMILIES:
+        raise ValueError(f"the tabulated gas takes the fluxes {LTE_FAMILIES}, not {flux!r}")
     table = table if table is not None else synthetic_table()
     mesh = box_mesh(dim, n, 0.0, 2 * np.pi, periodic=True)
+    vol, surf = PAIRINGS[flux]
     rt = RunTime(dim=dim, order=order, flow_model=A.FLOW_NAVIER_STOKES if viscous else A.FLOW_EULER,
-                 volume_flux=A.VOLUME_LTEEC, surface_flux=A.SURFACE_LTEEC, blending=False)
+                 volume_flux=vol, surface_flux=surf, blending=False, lte=True)
     return rt, mesh, lte_taylor_green(mesh, order, table), table

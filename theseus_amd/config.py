@@ -30,6 +30,8 @@ PAIRINGS = {
     "roe": (A.VOLUME_CENTRAL, A.SURFACE_ROE),
     "lteec": (A.VOLUME_LTEEC, A.SURFACE_LTEEC),
 }
+# flux families the tabulated (Helmholtz) LTE gas is built with
+LTE_FAMILIES = ("lteec", "pirozzoli", "hll", "llf")
 VOLUME_NAMES = {
     "central": A.VOLUME_CENTRAL,
     "chandrashekar": A.VOLUME_CHANDRASHEKAR,
@@ -95,6 +97,7 @@ class RunTime:
     dt: float | None = None
     variable_dt: bool = True
     max_dt_growth: float = 1.2
+    lte: bool = False  # the tabulated (Helmholtz) LTE gas: the operator needs its table
 
     @classmethod
     def from_json(cls, runtime: dict, compile_time: dict | None = None) -> "RunTime":
@@ -107,10 +110,10 @@ class RunTime:
         helmholtz = gas == "lte_helmholtz" or (gas == "lte" and fam == "lteec")
         if gas not in ("", "cpg", "ideal", "ideal_gas") and not helmholtz:
             raise NotImplementedError(f"gas_model {gas!r} with numerical_flux {runtime.get('numerical_flux')!r} is "
-                                      "not implemented (calorically perfect gas, or the Helmholtz LTE gas with "
-                                      "the LTEEC flux)")
-        if helmholtz and fam != "lteec":
-            raise NotImplementedError("the Helmholtz LTE gas is implemented with the LTEEC flux only")
+                                      "not implemented (calorically perfect gas, or the Helmholtz LTE gas)")
+        # SimFactory_impl.hpp:140-145: the Helmholtz gas takes LTEEC, Pirozzoli, HLL or LLF
+        if helmholtz and fam not in LTE_FAMILIES:
+            raise ValueError("Helmholtz LTE supports LTEECFlux, Pirozzoli, HLL and LLF")
         if fam == "lteec" and not helmholtz:
             raise ValueError("the LTEEC flux needs gas_model 'lte' or 'lte_helmholtz'")
         vol, surf = PAIRINGS[fam]
@@ -142,4 +145,5 @@ class RunTime:
             dt=float(runtime["dt"]) if "dt" in runtime else None,
             variable_dt=bool(runtime.get("variable_dt", "dt" not in runtime)),
             max_dt_growth=float(runtime.get("max_dt_growth", 1.2)),
+            lte=helmholtz,
         )

@@ -61,7 +61,31 @@ std::map<std::tuple<int, int>, LteKernels> &lte_table() {
   static std::map<std::tuple<int, int>, LteKernels> t;
   return t;
 }
-void register_lte(int dim, int p, LteKernels k) { lte_table()[{dim, p}] = k; }
+// merges: the LTEEC file registers the shared kernels, the others one pairing each
+void register_lte(int dim, int p, LteKernels k) {
+  LteKernels &e = lte_table()[{dim, p}];
+  for (int fx = 0; fx < LTE_NFLUX; ++fx)
+    for (int pm = 0; pm < 2; ++pm) {
+      if (k.rhs[fx][pm]) e.rhs[fx][pm] = k.rhs[fx][pm];
+      if (k.ns[fx][pm]) e.ns[fx][pm] = k.ns[fx][pm];
+    }
+  if (k.nodes) {
+    e.nodes = k.nodes;
+    e.ns_grad[0] = k.ns_grad[0];
+    e.ns_grad[1] = k.ns_grad[1];
+    e.pref = k.pref;
+    e.tpb = k.tpb;
+    e.epb = k.epb;
+  }
+}
+// The pairings of SimFactory_impl.hpp:140-245 (LteFlux), -1 for any other
+int lte_flux_of(int vol, int surf) {
+  if (vol == THESEUS_VOLUME_LTEEC && surf == THESEUS_SURFACE_LTEEC) return LTE_FLUX_LTEEC;
+  if (vol == THESEUS_VOLUME_PIROZZOLI && surf == THESEUS_SURFACE_PIROZZOLI) return LTE_FLUX_PIROZZOLI;
+  if (vol == THESEUS_VOLUME_CENTRAL && surf == THESEUS_SURFACE_HLL) return LTE_FLUX_HLL;
+  if (vol == THESEUS_VOLUME_CENTRAL && surf == THESEUS_SURFACE_LLF) return LTE_FLUX_LLF;
+  return -1;
+}
 const LteKernels *find_lte(int dim, int p) {
   auto it = lte_table().find({dim, p});
   return it == lte_table().end() ? nullptr : &it->second;
@@ -441,6 +465,7 @@ struct theseus_rhs {
   const FluxKernels *fk = nullptr;
   const ShapeKernels *sk = nullptr;
   const LteKernels *lk = nullptr;  // tabulated LTE gas
+  int lfx = 0;                     // its flux pairing (LteFlux)
   lte::View ltev;                  // its device arrays
   double *d_lte = nullptr;
   float *d_lte32 = nullptr;   // mixed precision: the planes in FP32
@@ -549,10 +574,10 @@ static int enqueue_rhs(theseus_rhs *h, RHSArgs a, hipStream_t s) {
     if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
       h->lk->nodes(a, nb, s);
       timed(h, 1, s, [&] { h->lk->ns_grad[h->pm](a, nb, s); });
-      timed(h, 0, s, [&] { h->lk->ns[h->pm](a, nb, s); });
+      timed(h, 0, s, [&] { h->lk->ns[h->lfx][h->pm](a, nb, s); });
     } else {
       timed(h, 1, s, [&] { h->lk->nodes(a, nb, s); });
-      timed(h, 0, s, [&] { h->lk->rhs[h->pm](a, nb, s); });
+      timed(h, 0, s, [&] { h->lk->rhs[h->lfx][h->pm](a, nb, s); });
     }
   } else if (h->d.flow_model == THESEUS_FLOW_NAVIER_STOKES) {
     timed(h, 1, s, [&] { h->sk->ns_grad[h->pm][h->has_boundary](a, nb, s); });
@@ -738,10 +763,14 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
   if (d.flow_model == THESEUS_FLOW_NAVIER_STOKES && (!(d.Pr > 0) || !(d.mu >= 0)))
     return fail(THESEUS_ERR_INVALID_ARGUMENT, "Navier-Stokes needs Pr > 0 and mu >= 0");
   const LteKernels *lk = nullptr;
+  int lfx = -1;
   if (lt) {
     if (d.blending) return fail(THESEUS_ERR_UNSUPPORTED, "the tabulated LTE gas is implemented without blending");
-    if (d.volume_flux != THESEUS_VOLUME_LTEEC || d.surface_flux != THESEUS_SURFACE_LTEEC)
-      return fail(THESEUS_ERR_INVALID_ARGUMENT, "the tabulated LTE gas uses the LTEEC volume and surface fluxes");
+    lfx = lte_flux_of(d.volume_flux, d.surface_flux);
+    if (lfx < 0)
+      return fail(THESEUS_ERR_INVALID_ARGUMENT,
+                  "the tabulated LTE gas takes the LTEEC, Pirozzoli, central/HLL or central/LLF "
+                  "volume/surface flux pairs");
     if (lt->nx < 2 || lt->ny < 2 || !lt->rho_grid || !lt->T_grid || !lt->e_grid || !lt->planes || !lt->inverse)
       return fail(THESEUS_ERR_INVALID_ARGUMENT, "LTE table needs nx, ny >= 2 and all arrays");
     if (lt->viscosity_model != 0 && lt->viscosity_model != 1)
@@ -759,7 +788,8 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     if (!(lt->rho_grid[0] > 0) || !(lt->T_grid[0] > 0))
       return fail(THESEUS_ERR_INVALID_ARGUMENT, "LTE table density and temperature must be positive");
     lk = find_lte(d.dim, d.order);
-    if (!lk) return fail(THESEUS_ERR_UNSUPPORTED, "no LTE kernel compiled for this dim/order");
+    if (!lk || !lk->nodes || !lk->rhs[lfx][0] || !lk->ns[lfx][0])
+      return fail(THESEUS_ERR_UNSUPPORTED, "no LTE kernel compiled for this dim/order and flux");
   }
   const FluxKernels *fk = lt ? nullptr : find_flux(d.dim, d.order, d.volume_flux, d.surface_flux);
   const ShapeKernels *sk = find_shape(d.dim, d.order);
@@ -796,6 +826,7 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
   h->fk = fk;
   h->sk = sk;
   h->lk = lk;
+  h->lfx = lfx;
   h->Np = d.order + 1;
   h->ndof = dim == 2 ? h->Np * h->Np : h->Np * h->Np * h->Np;
   h->nfp = dim == 2 ? h->Np : h->Np * h->Np;

@@ -353,9 +353,13 @@ __device__ __forceinline__ void scalar_dissipation(const GasC<R> &g, const Pt<R,
 }
 
 // DissipationOperator.hpp:57-172 (Harten-Hyman entropy fix)
-template <typename R, int DIM>
-__device__ __forceinline__ void roe_dissipation(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b,
-                                                const R *nor, R *diss) {
+// The gas enters through gm() = gamma of the first state and cs(s, side) =
+// the sound speed of state s (side 0 for a, 1 for b): the ideal gas's
+// uniform gamma and sqrt(gamma p/rho), or the tabulated gas's cp/cv and
+// table sound speed.  Callables keep the ideal gas's evaluation order.
+template <typename R, int DIM, typename GM, typename CS>
+__device__ __forceinline__ void roe_dissipation_core(const Pt<R, DIM> &a, const Pt<R, DIM> &b, GM gm, CS cs,
+                                                     const R *nor, R *diss) {
   const R rho1 = a.q[0], rho2 = b.q[0];
   const R H1 = (a.q[DIM + 1] + a.p) * a.ir;
   const R H2 = (b.q[DIM + 1] + b.p) * b.ir;
@@ -382,8 +386,8 @@ __device__ __forceinline__ void roe_dissipation(const GasC<R> &g, const Pt<R, DI
   vn1 *= inv_nor_mag;
   vn2 *= inv_nor_mag;
   vn_roe *= inv_nor_mag;
-  const R c_roe = dsqrt((g.gamma - R(1)) * (roe_H - ke));
-  const R c1 = sound_speed(g, a), c2 = sound_speed(g, b);
+  const R c_roe = dsqrt((gm() - R(1)) * (roe_H - ke));
+  const R c1 = cs(a, 0), c2 = cs(b, 1);
   R eig0, eig2;
   {
     const R er = vn_roe - c_roe;
@@ -422,6 +426,13 @@ __device__ __forceinline__ void roe_dissipation(const GasC<R> &g, const Pt<R, DI
   diss[DIM + 1] = de;
 #pragma unroll
   for (int e = 0; e < DIM + 2; e++) diss[e] *= nor_mag;
+}
+
+template <typename R, int DIM>
+__device__ __forceinline__ void roe_dissipation(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b,
+                                                const R *nor, R *diss) {
+  roe_dissipation_core(
+      a, b, [&] { return g.gamma; }, [&](const Pt<R, DIM> &s, int) { return sound_speed(g, s); }, nor, diss);
 }
 
 // ChandrashekarFlux.hpp:91-155 (EC + scalar dissipation)
@@ -467,14 +478,20 @@ __device__ __forceinline__ void face_ranocha(const GasC<R> &g, const Pt<R, DIM> 
 }
 
 // PirozzoliFlux.hpp:73-120 (KEP + Roe dissipation)
+template <typename R, int DIM, typename GM, typename CS>
+__device__ __forceinline__ void face_pirozzoli_core(const Pt<R, DIM> &a, const Pt<R, DIM> &b, GM gm, CS cs,
+                                                    const R *nor, R *F) {
+  R ec[DIM + 2], diss[DIM + 2];
+  vol_pirozzoli<R, DIM>(GasC<R>{}, a, b, nor, ec);
+  roe_dissipation_core(a, b, gm, cs, nor, diss);
+#pragma unroll
+  for (int e = 0; e < DIM + 2; ++e) F[e] = ec[e] - diss[e];
+}
 template <typename R, int DIM>
 __device__ __forceinline__ void face_pirozzoli(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b,
                                                const R *nor, R *F) {
-  R ec[DIM + 2], diss[DIM + 2];
-  vol_pirozzoli<R, DIM>(g, a, b, nor, ec);
-  roe_dissipation(g, a, b, nor, diss);
-#pragma unroll
-  for (int e = 0; e < DIM + 2; ++e) F[e] = ec[e] - diss[e];
+  face_pirozzoli_core(
+      a, b, [&] { return g.gamma; }, [&](const Pt<R, DIM> &s, int) { return sound_speed(g, s); }, nor, F);
 }
 
 // RoeFlux.hpp:283-390 (no entropy fix)
@@ -539,9 +556,9 @@ __device__ __forceinline__ void face_roe(const GasC<R> &g, const Pt<R, DIM> &L, 
 }
 
 // HLLFlux.hpp:53-116
-template <typename R, int DIM>
-__device__ __forceinline__ void face_hll(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b, const R *nor,
-                                         R *F) {
+// cs(s, side): the sound speed of state s (side 0 for a, 1 for b)
+template <typename R, int DIM, typename CS>
+__device__ __forceinline__ void face_hll_core(const Pt<R, DIM> &a, const Pt<R, DIM> &b, CS cs, const R *nor, R *F) {
   R un1 = R(0), un2 = R(0), nor_mag2 = R(0);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
@@ -550,8 +567,8 @@ __device__ __forceinline__ void face_hll(const GasC<R> &g, const Pt<R, DIM> &a, 
     nor_mag2 += nor[d] * nor[d];
   }
   const R nor_mag = dsqrt(nor_mag2);
-  const R c1 = sound_speed(g, a) * nor_mag;
-  const R c2 = sound_speed(g, b) * nor_mag;
+  const R c1 = cs(a, 0) * nor_mag;
+  const R c2 = cs(b, 1) * nor_mag;
   const R sL1 = un1 - c1, sL2 = un2 - c2, sR1 = un1 + c1, sR2 = un2 + c2;
   const R sL = (sL1 < sL2) ? sL1 : sL2;
   const R sR = (sR1 > sR2) ? sR1 : sR2;
@@ -565,11 +582,15 @@ __device__ __forceinline__ void face_hll(const GasC<R> &g, const Pt<R, DIM> &a, 
     else F[e] = (sR * f1[e] - sL * f2[e] + sL * sR * (b.q[e] - a.q[e])) / (sR - sL);
   }
 }
+template <typename R, int DIM>
+__device__ __forceinline__ void face_hll(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b, const R *nor,
+                                         R *F) {
+  face_hll_core(a, b, [&](const Pt<R, DIM> &s, int) { return sound_speed(g, s); }, nor, F);
+}
 
 // LaxFriedrichsFlux.hpp:188-211 with MaximumNormalWaveSpeed (NavierStokesFlux.hpp:246-270)
-template <typename R, int DIM>
-__device__ __forceinline__ void face_llf(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b, const R *nor,
-                                         R *F) {
+template <typename R, int DIM, typename CS>
+__device__ __forceinline__ void face_llf_core(const Pt<R, DIM> &a, const Pt<R, DIM> &b, CS cs, const R *nor, R *F) {
   R vd1 = R(0), vd2 = R(0), nn = R(0);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
@@ -578,12 +599,17 @@ __device__ __forceinline__ void face_llf(const GasC<R> &g, const Pt<R, DIM> &a, 
     nn += nor[d] * nor[d];
   }
   const R nrm = dsqrt(nn);
-  const R lambda_max = dmax(dabs(vd1) + sound_speed(g, a) * nrm, dabs(vd2) + sound_speed(g, b) * nrm);
+  const R lambda_max = dmax(dabs(vd1) + cs(a, 0) * nrm, dabs(vd2) + cs(b, 1) * nrm);
   R f1[DIM + 2], f2[DIM + 2];
   normal_flux(a, nor, f1);
   normal_flux(b, nor, f2);
 #pragma unroll
   for (int e = 0; e < DIM + 2; ++e) F[e] = R(0.5) * (f1[e] + f2[e]) - R(0.5) * lambda_max * (b.q[e] - a.q[e]);
+}
+template <typename R, int DIM>
+__device__ __forceinline__ void face_llf(const GasC<R> &g, const Pt<R, DIM> &a, const Pt<R, DIM> &b, const R *nor,
+                                         R *F) {
+  face_llf_core(a, b, [&](const Pt<R, DIM> &s, int) { return sound_speed(g, s); }, nor, F);
 }
 
 // face_llf(a, b) evaluated from (own, oth) without ordering the operands:

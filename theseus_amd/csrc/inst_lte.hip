@@ -1,4 +1,5 @@
-// Kernel instantiations: Euler with the tabulated LTE gas and the LTEEC flux.
+// Kernel instantiations: the tabulated LTE gas with the LTEEC flux, and the
+// kernels every pairing shares (temperature recovery, BR1 gradient, p_ref).
 #include "registry.hpp"
 
 namespace thx {

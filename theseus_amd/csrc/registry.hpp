@@ -47,13 +47,14 @@ struct OrdersKernels {
   OrdersLaunch euler[2][2] = {};
 };
 
-// Euler with the tabulated LTE gas and the LTEEC flux
-// [precision]: 0 FP64, 1 the mixed-precision form (FP32 flux arithmetic)
+// The tabulated LTE gas.  [flux]: the pairing (LteFlux, rhs_lte.hpp);
+// [precision]: 0 FP64, 1 the mixed-precision form (FP32 flux arithmetic).
+// Each instantiation file registers the fields it sets (the rest null).
 struct LteKernels {
-  RHSLaunch rhs[2] = {};
+  RHSLaunch rhs[4][2] = {};        // Euler
   RHSLaunch nodes = nullptr;       // temperature recovery per node (k_lte_nodes, FP64), before rhs
   RHSLaunch ns_grad[2] = {};       // Navier-Stokes pass 1 of the tabulated gas (after nodes)
-  RHSLaunch ns[2] = {};            // Navier-Stokes pass 2 (after ns_grad)
+  RHSLaunch ns[4][2] = {};         // Navier-Stokes pass 2 (after ns_grad)
   RHSLaunch pref = nullptr;        // mixed precision: the background pressure of a state
   int tpb = 0, epb = 0;
 };
@@ -144,9 +145,9 @@ struct OrdersRegistrar {
   }
 };
 
-template <typename R, int DIM, int P, bool VISC = false>
+template <typename R, int DIM, int P, bool VISC = false, int FX = LTE_FLUX_LTEEC>
 void launch_rhs_lte(const RHSArgs &a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_rhs_lte<R, DIM, P, VISC>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
+  hipLaunchKernelGGL((k_rhs_lte<R, DIM, P, VISC, FX>), dim3(nb), dim3(Shape<DIM, P>::TPB), 0, s, a);
 }
 template <typename R, int DIM, int P>
 void launch_ns_grad_lte(const RHSArgs &a, int nb, hipStream_t s) {
@@ -167,16 +168,29 @@ template <int DIM, int P>
 struct LteRegistrar {
   LteRegistrar() {
     LteKernels k;
-    k.rhs[0] = &launch_rhs_lte<double, DIM, P>;
-    k.rhs[1] = &launch_rhs_lte<float, DIM, P>;
+    k.rhs[LTE_FLUX_LTEEC][0] = &launch_rhs_lte<double, DIM, P>;
+    k.rhs[LTE_FLUX_LTEEC][1] = &launch_rhs_lte<float, DIM, P>;
     k.nodes = &launch_lte_nodes<DIM>;
     k.ns_grad[0] = &launch_ns_grad_lte<double, DIM, P>;
     k.ns_grad[1] = &launch_ns_grad_lte<float, DIM, P>;
-    k.ns[0] = &launch_rhs_lte<double, DIM, P, true>;
-    k.ns[1] = &launch_rhs_lte<float, DIM, P, true>;
+    k.ns[LTE_FLUX_LTEEC][0] = &launch_rhs_lte<double, DIM, P, true>;
+    k.ns[LTE_FLUX_LTEEC][1] = &launch_rhs_lte<float, DIM, P, true>;
     k.pref = &launch_pref_lte<DIM>;
     k.tpb = Shape<DIM, P>::TPB;
     k.epb = Shape<DIM, P>::EPB;
+    register_lte(DIM, P, k);
+  }
+};
+
+// The Pirozzoli, HLL and LLF pairings of the tabulated gas
+template <int DIM, int P, int FX>
+struct LteFluxRegistrar {
+  LteFluxRegistrar() {
+    LteKernels k;
+    k.rhs[FX][0] = &launch_rhs_lte<double, DIM, P, false, FX>;
+    k.rhs[FX][1] = &launch_rhs_lte<float, DIM, P, false, FX>;
+    k.ns[FX][0] = &launch_rhs_lte<double, DIM, P, true, FX>;
+    k.ns[FX][1] = &launch_rhs_lte<float, DIM, P, true, FX>;
     register_lte(DIM, P, k);
   }
 };

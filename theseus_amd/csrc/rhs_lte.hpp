@@ -1,6 +1,11 @@
-// Euler right-hand side of the tabulated LTE gas with the LTEEC flux
-// (LTEECFlux.hpp:93-107; EulerOperator_impl.hpp:28-421 with the
-// LTEHelmholtzGas of LTEGasModel.hpp:339), FP64.  Same element-per-
+// Euler right-hand side of the tabulated LTE gas (EulerOperator_impl.hpp:
+// 28-421 with the LTEHelmholtzGas of LTEGasModel.hpp:339), FP64, with one of
+// the flux pairings the reference's factory builds for that gas
+// (SimFactory_impl.hpp:140-245; FX, LteFlux): LTEEC (LTEECFlux.hpp:93-107),
+// Pirozzoli (PirozzoliFlux.hpp:19-120), or the central volume flux with an
+// HLL (HLLFlux.hpp:19-116) or LLF (LaxFriedrichsFlux.hpp:20-77) face; the
+// last three take the gas's pressure, sound speed and cp/cv at each node
+// from the table at its recovered temperature.  Same element-per-
 // workgroup structure as k_rhs: the element's temperatures are recovered
 // once per node per stage by k_lte_nodes (inverse-table guess + Newton)
 // and staged in LDS with the table cell, so the pair fluxes only evaluate
@@ -23,7 +28,10 @@
 
 namespace thx {
 
-template <typename R, int DIM, int P, bool VISC = false>
+// Flux pairings of the tabulated gas (SimFactory_impl.hpp:140-245)
+enum LteFlux { LTE_FLUX_LTEEC = 0, LTE_FLUX_PIROZZOLI = 1, LTE_FLUX_HLL = 2, LTE_FLUX_LLF = 3, LTE_NFLUX = 4 };
+
+template <typename R, int DIM, int P, bool VISC = false, int FX = LTE_FLUX_LTEEC>
 __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   using S = Shape<DIM, P>;
   using SC = RhsScratch<R, DIM, P>;
@@ -42,6 +50,8 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   __shared__ R sc[EPB][ND];     // sound speed
   __shared__ R spd[FP64 ? 1 : EPB][FP64 ? 1 : ND];  // mixed: pressure less the background
   __shared__ int scell[EPB][ND];     // table cell j*nx + i
+  constexpr bool GAM = FX == LTE_FLUX_PIROZZOLI;  // the Roe dissipation's gamma = cp/cv
+  __shared__ R sg[GAM ? EPB : 1][GAM ? ND : 1];
   __shared__ R sraw[EPB][SCR];
   __shared__ R sD2[Np * Np];
   __shared__ ElemMeta<DIM, EPB> sm;
@@ -104,6 +114,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
     sp[el][node] = R(pn);
     if constexpr (!FP64) spd[el][node] = R(pn - pref);
     sc[el][node] = R(sqrt(ev.c2()));
+    if constexpr (GAM) sg[el][node] = R(ev.cp() / ev.cv());
     scell[el][node] = (int)(ev.cell.H - tv.planes);
   }
   lds_sync();
@@ -119,6 +130,19 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
     const int off = scell[pe][n];
     const int ci = off % tv.nx, cj = off / tv.nx;
     return lte::at_cell<R>(tv, sq[pe][0][n], sT[pe][n], ci, cj);
+  };
+  // the ideal-gas flux point of a state with the table's pressure p (pd:
+  // less the background in the mixed form)
+  auto make_pt = [&](const R *q, R p, R pd) {
+    Pt<R, DIM> s;
+#pragma unroll
+    for (int e2 = 0; e2 < NQ; ++e2) s.q[e2] = q[e2];
+    s.ir = drcp(q[0]);
+    s.p = p;
+    s.pd = pd;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) s.v[d] = q[1 + d] * s.ir;
+    return s;
   };
 
   // ---- volume: every GLL-line pair once (DGSEMIntegrator.hpp:124-218)
@@ -151,10 +175,16 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
         }
 #pragma unroll
         for (int d = 0; d < DIM; ++d) met[d] = R(sm.adj[pe][dir * DIM + d]);
-        const lte::Eval<R> s1 = node_eval(pe, na), s2 = node_eval(pe, nb);
         R F[NQ];
-        lte::lteec_flux<DIM, false>(tv, q1, q2, s1, s2, sp[pe][na], sp[pe][nb], R(0), R(0), met, F, spm(pe, na),
-                                    spm(pe, nb));
+        if constexpr (FX == LTE_FLUX_LTEEC) {
+          const lte::Eval<R> s1 = node_eval(pe, na), s2 = node_eval(pe, nb);
+          lte::lteec_flux<DIM, false>(tv, q1, q2, s1, s2, sp[pe][na], sp[pe][nb], R(0), R(0), met, F, spm(pe, na),
+                                      spm(pe, nb));
+        } else {
+          const Pt<R, DIM> A = make_pt(q1, sp[pe][na], spm(pe, na)), B = make_pt(q2, sp[pe][nb], spm(pe, nb));
+          if constexpr (FX == LTE_FLUX_PIROZZOLI) vol_pirozzoli<R, DIM>(GasC<R>{}, A, B, met, F);
+          else vol_central<R, DIM>(GasC<R>{}, A, B, met, F);
+        }
 #pragma unroll
         for (int q = 0; q < NQ; ++q) sscr(pe, (dir & 1) * NPAIR + r, q) = F[q];
       }
@@ -213,7 +243,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
     for (int q = 0; q < NQ; ++q) xb[q] = Xp[q * Ntot + ebase];
   }
 
-  // ---- faces (EulerOperator_impl.hpp:211-392): LTEEC with Lax dissipation
+  // ---- faces (EulerOperator_impl.hpp:211-392)
   {
     const double w0 = a.basis.w[0];
 #pragma unroll
@@ -232,19 +262,25 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
           qo[q] = sq[pe][q][on];
           qt[q] = interior ? R(qn[r][q]) : qo[q];
         }
-        const lte::Eval<R> so = node_eval(pe, on);
+        constexpr bool EC = FX == LTE_FLUX_LTEEC;
+        const lte::Eval<R> so = EC ? node_eval(pe, on) : lte::Eval<R>{};
         const R po = sp[pe][on], co = sc[pe][on], pdo = spm(pe, on);
+        R go = R(0);
+        if constexpr (GAM) go = sg[pe][on];
         lte::Eval<R> st = so;
-        R pt = po, ct = co, pdt = pdo;
+        R pt = po, ct = co, pdt = pdo, gt = go;
         if (interior) {
           // the neighbour's trace: pressure and sound speed in FP64
           const lte::Eval<double> st64 = lte::at_packed<double>(tv, qn[r][0], Tn[r], cn[r]);
-          if constexpr (FP64) st = st64;
-          else st = lte::at_packed<R>(tv, qt[0], R(Tn[r]), cn[r]);
+          if constexpr (EC) {
+            if constexpr (FP64) st = st64;
+            else st = lte::at_packed<R>(tv, qt[0], R(Tn[r]), cn[r]);
+          }
           const double pn = st64.pressure();
           pt = R(pn);
           pdt = FP64 ? pt : R(pn - pref);
           ct = R(sqrt(st64.c2()));
+          if constexpr (GAM) gt = R(st64.cp() / st64.cv());
         }
         R n[DIM];
 #pragma unroll
@@ -258,10 +294,23 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
           ql[q] = own_low ? qo[q] : qt[q];
           qh[q] = own_low ? qt[q] : qo[q];
         }
-        const lte::Eval<R> sl = own_low ? so : st, sh = own_low ? st : so;
         R F[NQ];
-        lte::lteec_flux<DIM, true>(tv, ql, qh, sl, sh, own_low ? po : pt, own_low ? pt : po, own_low ? co : ct,
-                                   own_low ? ct : co, n, F, own_low ? pdo : pdt, own_low ? pdt : pdo);
+        if constexpr (EC) {
+          const lte::Eval<R> sl = own_low ? so : st, sh = own_low ? st : so;
+          lte::lteec_flux<DIM, true>(tv, ql, qh, sl, sh, own_low ? po : pt, own_low ? pt : po, own_low ? co : ct,
+                                     own_low ? ct : co, n, F, own_low ? pdo : pdt, own_low ? pdt : pdo);
+        } else {
+          const Pt<R, DIM> L = make_pt(ql, own_low ? po : pt, own_low ? pdo : pdt);
+          const Pt<R, DIM> H = make_pt(qh, own_low ? pt : po, own_low ? pdt : pdo);
+          const R cl = own_low ? co : ct, ch = own_low ? ct : co;
+          auto cs = [&](const Pt<R, DIM> &, int side) { return side ? ch : cl; };
+          if constexpr (FX == LTE_FLUX_HLL) face_hll_core(L, H, cs, n, F);
+          else if constexpr (FX == LTE_FLUX_LLF) face_llf_core(L, H, cs, n, F);
+          else {
+            const R gl = own_low ? go : gt;  // gamma of the first state (DissipationOperator.hpp:109)
+            face_pirozzoli_core(L, H, [&] { return gl; }, cs, n, F);
+          }
+        }
         if constexpr (VISC) {
           // F* - 0.5 n.(Fv_L + Fv_R); the own side's -0.5 n.Fv is in Vd
 #pragma unroll

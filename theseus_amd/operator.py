@@ -308,8 +308,11 @@ def lte_descriptor(table):
 def MakeRHSOperator(runtime: dict | RunTime, mesh: BoxMesh, lte_table=None, elem_orient=None) -> RHSOperator:
     """SimFactory_impl.hpp:63-488: pick the operator from the runTime strings;
     with `lte_table` (a theseus_amd.lte.HelmholtzTable) the gas is the
-    tabulated LTE gas with the LTEEC flux (LTEGasModel.hpp:339);
-    `elem_orient` gives the elements' storage frames (theseus_case_desc)."""
+    tabulated LTE gas (LTEGasModel.hpp:339) with the runtime's LTEEC,
+    Pirozzoli, HLL or LLF flux pair; `elem_orient` gives the elements'
+    storage frames (theseus_case_desc)."""
     rt = runtime if isinstance(runtime, RunTime) else RunTime.from_json(runtime)
+    if rt.lte and lte_table is None:
+        raise ValueError("the tabulated LTE gas needs its table (lte_table)")
     cls = NSOperator if rt.flow_model == A.FLOW_NAVIER_STOKES else EulerOperator
     return cls(rt, mesh, lte_table, elem_orient)
