@@ -184,11 +184,12 @@ def serving_leg(num_cases, log, fp64=False):
     return out
 
 
-def _lte_run(viscous, steps, mixed=False, total_steps=50):
-    """SSP-RK3 steps of the LTE Taylor-Green vortex (24^3, p=3): elapsed ms
-    of `steps` timed steps, finiteness, the per-kernel means (slot 0: the RHS
-    pass; slot 1: the temperature recovery for Euler, the gradient pass for
-    Navier-Stokes), and the state after `total_steps` steps (device tensor)."""
+def _lte_run(viscous, steps, mixed=False, total_steps=50, flux="lteec"):
+    """SSP-RK3 steps of the LTE Taylor-Green vortex (24^3, p=3) with the flux
+    pair `flux`: elapsed ms of `steps` timed steps, finiteness, the
+    per-kernel means (slot 0: the RHS pass; slot 1: the temperature recovery
+    for Euler, the gradient pass for Navier-Stokes), and the state after
+    `total_steps` steps (device tensor)."""
     import dataclasses
 
     import torch
@@ -197,7 +198,7 @@ def _lte_run(viscous, steps, mixed=False, total_steps=50):
     from theseus_amd.cases import c4_lte_tgv
     from theseus_amd.operator import MakeRHSOperator
 
-    rt, mesh, U0, table = c4_lte_tgv(n=24, order=3, viscous=viscous)
+    rt, mesh, U0, table = c4_lte_tgv(n=24, order=3, viscous=viscous, flux=flux)
     if mixed:
         rt = dataclasses.replace(rt, precision=A.PRECISION_MIXED)
     op = MakeRHSOperator(rt, mesh, lte_table=table)
@@ -279,6 +280,15 @@ def lte_leg(log, steps=10):
     ns_value = dof * 3 * steps / (ns_ms * 1e-3)
     log(f"lte navier-stokes: {ns_ms/steps:.3f} ms/step, {ns_value:.3e} DOF*RHS/s, pass 2 {p2_ms*1e3:.1f} us, "
         f"pass 1 {p1_ms*1e3:.1f} us")
+    # the other flux pairs of the tabulated gas, Euler, FP64 and mixed
+    pairs = {}
+    for fx in ("pirozzoli", "hll", "llf"):
+        f_ms, f_fin, fk_ms, _, _, _ = _lte_run(False, steps, flux=fx)
+        m_ms, m_fin, mk2_ms, _, _, _ = _lte_run(False, steps, mixed=True, flux=fx)
+        pairs[fx] = {"value": dof * 3 * steps / (f_ms * 1e-3), "ms_per_step": f_ms / steps, "kernel_mean_ms": fk_ms,
+                     "mixed_ms_per_step": m_ms / steps, "mixed_kernel_mean_ms": mk2_ms, "finite": f_fin and m_fin}
+        log(f"lte {fx}: {f_ms/steps:.3f} ms/step, kernel {fk_ms*1e3:.1f} us; mixed {m_ms/steps:.3f} ms/step, "
+            f"kernel {mk2_ms*1e3:.1f} us")
     return {"value": value, "unit": "DOF*RHS-evals/s", "ms_per_step": ms / steps, "steps": steps,
             "kernel_mean_ms": k_ms, "recovery_mean_ms": r_ms, "finite": finite, "dof": dof,
             "workload": "3-D LTE Taylor-Green vortex, 24^3 hexes, p=3, LTEEC, synthetic 256x256 Helmholtz table, "
@@ -291,7 +301,9 @@ def lte_leg(log, steps=10):
                                     "FP64 temperature recovery, nodal pressure and sound speed, stage combination"},
             "navier_stokes": {"value": ns_value, "ms_per_step": ns_ms / steps, "pass2_mean_ms": p2_ms,
                               "pass1_mean_ms": p1_ms, "finite": ns_finite,
-                              "workload": "the same, Navier-Stokes BR1 with the table's mu and lambda planes"}}
+                              "workload": "the same, Navier-Stokes BR1 with the table's mu and lambda planes"},
+            "flux_pairs": {"workload": "the same (Euler) with the Pirozzoli, central/HLL and central/LLF pairs",
+                           **pairs}}
 
 
 # Whole-RHS bytes per DOF (node x equation) of the workload table's
