@@ -471,6 +471,7 @@ struct theseus_rhs {
   float *d_lte32 = nullptr;   // mixed precision: the planes in FP32
   double *d_lteT = nullptr;   // per node: recovered temperature of the stage state
   int *d_ltecell = nullptr;   // per node: its table cell
+  double *d_ltepcg = nullptr; // per node, pairs other than LTEEC: pressure, sound speed (, gamma)
   BasisArgs basis;
   GasC<double> gas;
   GasC<float> gas32;
@@ -887,6 +888,16 @@ static int create_impl(const theseus_case_desc *desc, const theseus_lte_table *l
     }
     h->ltev.node_T = h->d_lteT;
     h->ltev.node_cell = h->d_ltecell;
+    if (lfx != LTE_FLUX_LTEEC) {
+      const int nf = lfx == LTE_FLUX_PIROZZOLI ? 3 : 2;
+      if ((rc = upload<double>(&h->d_ltepcg, nullptr, (size_t)nf * h->Ntot))) {
+        theseus_rhs_destroy(h);
+        return rc;
+      }
+      h->ltev.node_p = h->d_ltepcg;
+      h->ltev.node_c = h->d_ltepcg + h->Ntot;
+      if (nf == 3) h->ltev.node_g = h->d_ltepcg + 2 * h->Ntot;
+    }
     h->ltev.sutherland = lt->viscosity_model == 1;
     h->ltev.mu0 = lt->mu0;
     h->ltev.T0 = lt->T0;
@@ -993,6 +1004,7 @@ int theseus_rhs_destroy(theseus_rhs *h) {
   hipFree(h->d_lte32);
   hipFree(h->d_lteT);
   hipFree(h->d_ltecell);
+  hipFree(h->d_ltepcg);
   hipFree(h->d_stage[0]);
   hipFree(h->d_stage[1]);
   hipFree(h->d_rates);

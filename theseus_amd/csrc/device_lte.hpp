@@ -37,6 +37,9 @@ struct View {
   // recovered temperature and the table cell i | j << 16 (-1 outside)
   double *node_T = nullptr;
   int *node_cell = nullptr;
+  // the Pirozzoli, HLL and LLF pairs: each node's pressure and sound speed
+  // (and, for Pirozzoli, gamma = cp/cv), evaluated by k_lte_nodes; null for LTEEC
+  double *node_p = nullptr, *node_c = nullptr, *node_g = nullptr;
   // viscosity of the Navier-Stokes form: the mu plane, or Sutherland's law
   // mu0 (T0 + Ts) (T/T0)^1.5 / (T + Ts) when sutherland is set
   int sutherland = 0;
@@ -436,8 +439,11 @@ __device__ __forceinline__ void lteec_flux(const View &v, const R *q1, const R *
 
 // Temperature and table cell of every node of a state (one lane per node):
 // the recovery of state_eval, done once per node and stage, for the RHS
-// kernel's staged nodes and its neighbours' face points.
-template <int DIM>
+// kernel's staged nodes and its neighbours' face points.  PCG: also each
+// node's pressure, sound speed (and gamma when node_g is set) for the pairs
+// other than LTEEC (a variant of its own, so that LTEEC's recovery keeps its
+// 70 registers).
+template <int DIM, bool PCG = false>
 __global__ void __launch_bounds__(256) k_lte_nodes(const double *U, long long Ntot, View v) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Ntot) return;
@@ -452,8 +458,15 @@ __global__ void __launch_bounds__(256) k_lte_nodes(const double *U, long long Nt
   const double T = temperature_from_energy(v, rho, e);
   const int i = find_cell(v.rho, v.nx, rho);
   const int j = find_cell(v.T, v.ny, T);
+  const int cell = (i < 0 || j < 0) ? -1 : (i | (j << 16));
   v.node_T[t] = T;
-  v.node_cell[t] = (i < 0 || j < 0) ? -1 : (i | (j << 16));
+  v.node_cell[t] = cell;
+  if constexpr (PCG) {
+    const Eval<double> ev = at_packed<double>(v, rho, T, cell);
+    v.node_p[t] = ev.pressure();
+    v.node_c[t] = sqrt(ev.c2());
+    if (v.node_g) v.node_g[t] = ev.cp() / ev.cv();
+  }
 }
 
 // ---- the viscous path (LTEGasModel<LTEHelmholtzGasEOS, LTETransport>,

@@ -51,6 +51,7 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   __shared__ R spd[FP64 ? 1 : EPB][FP64 ? 1 : ND];  // mixed: pressure less the background
   __shared__ int scell[EPB][ND];     // table cell j*nx + i
   constexpr bool GAM = FX == LTE_FLUX_PIROZZOLI;  // the Roe dissipation's gamma = cp/cv
+  constexpr bool EC0 = FX == LTE_FLUX_LTEEC;
   __shared__ R sg[GAM ? EPB : 1][GAM ? ND : 1];
   __shared__ R sraw[EPB][SCR];
   __shared__ R sD2[Np * Np];
@@ -103,19 +104,27 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
   if (tid < Np * Np) sD2[tid] = R(bD2);
 
   // ---- stage: each node's temperature and cell, recovered once per node by
-  //      k_lte_nodes; pressure and sound speed evaluated in FP64
+  //      k_lte_nodes; pressure and sound speed evaluated in FP64 (the
+  //      other pairs: evaluated by k_lte_nodes too, read here)
   meta_store<DIM, EPB, TPB>(mr, sm);
   if (valid) {
-    const lte::Eval<double> ev = lte::at_packed<double>(tv, qown[0], Town, cown);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) sq[el][q][node] = R(qown[q]);
-    sT[el][node] = R(ev.T);
-    const double pn = ev.pressure();
-    sp[el][node] = R(pn);
-    if constexpr (!FP64) spd[el][node] = R(pn - pref);
-    sc[el][node] = R(sqrt(ev.c2()));
-    if constexpr (GAM) sg[el][node] = R(ev.cp() / ev.cv());
-    scell[el][node] = (int)(ev.cell.H - tv.planes);
+    if constexpr (FX == LTE_FLUX_LTEEC) {
+      const lte::Eval<double> ev = lte::at_packed<double>(tv, qown[0], Town, cown);
+      sT[el][node] = R(ev.T);
+      const double pn = ev.pressure();
+      sp[el][node] = R(pn);
+      if constexpr (!FP64) spd[el][node] = R(pn - pref);
+      sc[el][node] = R(sqrt(ev.c2()));
+      scell[el][node] = (int)(ev.cell.H - tv.planes);
+    } else {
+      const double pn = tv.node_p[ebase];
+      sp[el][node] = R(pn);
+      if constexpr (!FP64) spd[el][node] = R(pn - pref);
+      sc[el][node] = R(tv.node_c[ebase]);
+      if constexpr (GAM) sg[el][node] = R(tv.node_g[ebase]);
+    }
   }
   lds_sync();
 
@@ -214,7 +223,8 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
 
   // the neighbours' face traces (state, temperature, cell) and the RK base
   // state, loaded once the pair fluxes no longer hold registers
-  double qn[NFR][NQ], Tn[NFR];
+  double qn[NFR][NQ], Tn[NFR];  // Tn: the other pairs' pressure
+  double pcn[EC0 ? 1 : NFR], gcn[GAM ? NFR : 1];
   int cn[NFR];
   R gn[NFR][VISC ? NVD : 1];
 #pragma unroll
@@ -227,8 +237,15 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
     const size_t nbase = (size_t)max(fnb[r], 0) * ND + face_node<DIM, Np>(d, 1 - s, t);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) qn[r][q] = a.U[q * Ntot + nbase];
-    Tn[r] = tv.node_T[nbase];
-    cn[r] = tv.node_cell[nbase];
+    if constexpr (FX == LTE_FLUX_LTEEC) {
+      Tn[r] = tv.node_T[nbase];
+      cn[r] = tv.node_cell[nbase];
+    } else {  // the neighbour's pressure, sound speed (and gamma) from k_lte_nodes
+      Tn[r] = tv.node_p[nbase];
+      cn[r] = 0;
+      pcn[r] = tv.node_c[nbase];
+      if constexpr (GAM) gcn[r] = tv.node_g[nbase];
+    }
     if constexpr (VISC) {
       // the neighbour's n.Fv at this point (the own side's half is in Vd)
       const size_t gb_n = (size_t)max(fnb[r], 0) * NFACE + (f ^ 1) * NF + t;
@@ -271,16 +288,21 @@ __global__ void __launch_bounds__((Shape<DIM, P>::TPB)) k_rhs_lte(RHSArgs a) {
         R pt = po, ct = co, pdt = pdo, gt = go;
         if (interior) {
           // the neighbour's trace: pressure and sound speed in FP64
-          const lte::Eval<double> st64 = lte::at_packed<double>(tv, qn[r][0], Tn[r], cn[r]);
           if constexpr (EC) {
+            const lte::Eval<double> st64 = lte::at_packed<double>(tv, qn[r][0], Tn[r], cn[r]);
             if constexpr (FP64) st = st64;
             else st = lte::at_packed<R>(tv, qt[0], R(Tn[r]), cn[r]);
+            const double pn = st64.pressure();
+            pt = R(pn);
+            pdt = FP64 ? pt : R(pn - pref);
+            ct = R(sqrt(st64.c2()));
+          } else {
+            const double pn = Tn[r];
+            pt = R(pn);
+            pdt = FP64 ? pt : R(pn - pref);
+            ct = R(pcn[r]);
+            if constexpr (GAM) gt = R(gcn[r]);
           }
-          const double pn = st64.pressure();
-          pt = R(pn);
-          pdt = FP64 ? pt : R(pn - pref);
-          ct = R(sqrt(st64.c2()));
-          if constexpr (GAM) gt = R(st64.cp() / st64.cv());
         }
         R n[DIM];
 #pragma unroll
