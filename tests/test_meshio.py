@@ -255,6 +255,31 @@ def test_loaded_tgv2d_ns_case_equals_the_uniform_box(oracle):
     assert np.linalg.norm(r_loaded - r_box) <= 1e-7 * np.linalg.norm(r_box)
 
 
+@pytest.mark.parametrize("flux,pair", [("HLL", ("CENTRAL", "HLL")), ("LLF", ("CENTRAL", "LLF")),
+                                       ("Pirozzoli", ("PIROZZOLI", "PIROZZOLI")), ("LTEEC", ("LTEEC", "LTEEC"))])
+def test_loaded_case_with_the_helmholtz_gas_builds_its_table(flux, pair):
+    """gas_model "lte_helmholtz" with each flux the reference's factory takes
+    for it (SimFactory_impl.hpp:140-245): the runtime selects the pair and the
+    tabulated gas, and the loader builds the table from the case's N_rho, N_T
+    and bounds (SimFactory_impl.hpp:146-180)."""
+    import json
+
+    from theseus_amd.casefile import load_case_config
+
+    cfg = json.loads((TGV2D / "config.json").read_text())
+    cfg["compileTime"] = {"SUBCELL_FV_BLENDING": False}
+    cfg["runTime"].update(gas_model="lte_helmholtz", numerical_flux=flux, N_rho=12, N_T=16, rho_min=0.5,
+                          rho_max=2.0, T_min=200.0, T_max=2000.0, rho_dist="log", T_dist="log")
+    cf = load_case_config(cfg, TGV2D / "TaylorGreenVortex2D.msh")
+    rt = cf.runtime
+    assert rt.lte and not rt.blending
+    assert (rt.volume_flux, rt.surface_flux) == (getattr(A, "VOLUME_" + pair[0]), getattr(A, "SURFACE_" + pair[1]))
+    t = cf.lte_table
+    assert t is not None and t.rho.size == 12 and t.T.size == 16
+    np.testing.assert_allclose([t.rho[0], t.rho[-1], t.T[0], t.T[-1]], [0.5, 2.0, 200.0, 2000.0], rtol=1e-12)
+    assert np.allclose(np.diff(np.log(t.T)), np.log(10.0) / 15)  # T_dist "log"
+
+
 def test_loaded_kelvin_helmholtz_case():
     from theseus_amd.casefile import load_case
 
