@@ -288,7 +288,7 @@ def lte_leg(log, steps=10):
         pairs[fx] = {"value": dof * 3 * steps / (f_ms * 1e-3), "ms_per_step": f_ms / steps, "kernel_mean_ms": fk_ms,
                      "recovery_mean_ms": fr_ms, "mixed_ms_per_step": m_ms / steps, "mixed_kernel_mean_ms": mk2_ms,
                      "finite": f_fin and m_fin}
-        log(f"lte {fx}: {f_ms/steps:.3f} ms/step, kernel {fk_ms*1e3:.1f} us, recovery with p, c "
+        log(f"lte {fx}: {f_ms/steps:.3f} ms/step, kernel {fk_ms*1e3:.1f} us, recovery + p, c "
             f"{fr_ms*1e3:.1f} us; mixed {m_ms/steps:.3f} ms/step, kernel {mk2_ms*1e3:.1f} us")
     return {"value": value, "unit": "DOF*RHS-evals/s", "ms_per_step": ms / steps, "steps": steps,
             "kernel_mean_ms": k_ms, "recovery_mean_ms": r_ms, "finite": finite, "dof": dof,

@@ -439,11 +439,8 @@ __device__ __forceinline__ void lteec_flux(const View &v, const R *q1, const R *
 
 // Temperature and table cell of every node of a state (one lane per node):
 // the recovery of state_eval, done once per node and stage, for the RHS
-// kernel's staged nodes and its neighbours' face points.  PCG: also each
-// node's pressure, sound speed (and gamma when node_g is set) for the pairs
-// other than LTEEC (a variant of its own, so that LTEEC's recovery keeps its
-// 70 registers).
-template <int DIM, bool PCG = false>
+// kernel's staged nodes and its neighbours' face points.
+template <int DIM>
 __global__ void __launch_bounds__(256) k_lte_nodes(const double *U, long long Ntot, View v) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Ntot) return;
@@ -458,15 +455,23 @@ __global__ void __launch_bounds__(256) k_lte_nodes(const double *U, long long Nt
   const double T = temperature_from_energy(v, rho, e);
   const int i = find_cell(v.rho, v.nx, rho);
   const int j = find_cell(v.T, v.ny, T);
-  const int cell = (i < 0 || j < 0) ? -1 : (i | (j << 16));
   v.node_T[t] = T;
-  v.node_cell[t] = cell;
-  if constexpr (PCG) {
-    const Eval<double> ev = at_packed<double>(v, rho, T, cell);
-    v.node_p[t] = ev.pressure();
-    v.node_c[t] = sqrt(ev.c2());
-    if (v.node_g) v.node_g[t] = ev.cp() / ev.cv();
-  }
+  v.node_cell[t] = (i < 0 || j < 0) ? -1 : (i | (j << 16));
+}
+
+// After k_lte_nodes, for the pairs other than LTEEC: each node's pressure,
+// sound speed and (when node_g is set) gamma = cp/cv at its recovered
+// temperature.  A pass of its own: the evaluation needs about 160 registers,
+// which inside the recovery would hold the Newton iterations to 3 waves per
+// SIMD instead of 7.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_lte_node_pcg(const double *U, long long Ntot, View v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Ntot) return;
+  const Eval<double> ev = at_packed<double>(v, U[t], v.node_T[t], v.node_cell[t]);
+  v.node_p[t] = ev.pressure();
+  v.node_c[t] = sqrt(ev.c2());
+  if (v.node_g) v.node_g[t] = ev.cp() / ev.cv();
 }
 
 // ---- the viscous path (LTEGasModel<LTEHelmholtzGasEOS, LTETransport>,

@@ -161,8 +161,8 @@ void launch_pref_lte(const RHSArgs &a, int, hipStream_t s) {
 template <int DIM>
 void launch_lte_nodes(const RHSArgs &a, int, hipStream_t s) {
   const dim3 grid((unsigned)((a.Ntot + 255) / 256));
-  if (a.lte.node_p) hipLaunchKernelGGL((lte::k_lte_nodes<DIM, true>), grid, dim3(256), 0, s, a.U, a.Ntot, a.lte);
-  else hipLaunchKernelGGL((lte::k_lte_nodes<DIM>), grid, dim3(256), 0, s, a.U, a.Ntot, a.lte);
+  hipLaunchKernelGGL((lte::k_lte_nodes<DIM>), grid, dim3(256), 0, s, a.U, a.Ntot, a.lte);
+  if (a.lte.node_p) hipLaunchKernelGGL((lte::k_lte_node_pcg<DIM>), grid, dim3(256), 0, s, a.U, a.Ntot, a.lte);
 }
 
 template <int DIM, int P>
