@@ -7,7 +7,7 @@ config's ``runTime`` keys (src/Simulation.cpp:105-313, through
 (include/StateInit.hpp; nodal interpolation at the GLL points, which is what
 ProjectCoefficient does for a DG GLL space, src/Simulation.cpp:1076-1109).
 
-Initial conditions restated: IsentropicVortexIC, TaylorGreenVortexIC,
+Initial conditions restated: IsentropicVortexIC, LTEVortexIC, TaylorGreenVortexIC,
 TaylorGreenVortex2DIC, CompressibleTaylorGreenVortexIC,
 KelvinHelmholtzInstabilityIC, LidDrivenCavityIC, ForwardFacingStepIC.  Boundaries:
 ``conditions.boundary_conditions`` by Gmsh physical name, with the
@@ -33,6 +33,7 @@ from .meshio import AffineMesh, gmsh_physical_names, hilbert_order, read_mesh, r
 INITIAL_CONDITIONS = {
     "LidDrivenCavityIC": (2, cases.lid_driven_cavity),
     "IsentropicVortexIC": (5, cases.reference_isentropic_vortex),
+    "LTEVortexIC": (5, cases.lte_vortex),
     "TaylorGreenVortexIC": (2, cases.taylor_green),
     "TaylorGreenVortex2DIC": (2, cases.taylor_green_2d),
     "CompressibleTaylorGreenVortexIC": (1, cases.compressible_taylor_green),

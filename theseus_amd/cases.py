@@ -138,6 +138,32 @@ def reference_isentropic_vortex(mesh: BoxMesh, order: int, radius=0.2, Minf=0.5,
     return _pack(den, [den * velX, den * velY], den * energy)
 
 
+def lte_vortex(mesh: BoxMesh, order: int, radius=0.2, vel_inf=10.0, beta=1.0, rho_inf=0.1,
+               temp_inf=275.0) -> np.ndarray:
+    """LTEVortexIC (StateInit.hpp:15-69): a vortex centred at the origin whose
+    field is shaped with the calorically perfect gas's relations (gamma 1.4,
+    R 287.05, whatever the case's gas): Gaussian temperature dip
+    T = T_inf - (vel_inf beta)^2/(2 cp) exp(-r^2/radius^2), clamped below at
+    0.2 T_inf, isentropic density and pressure from it, rhoE = p/(gamma-1) +
+    rho |u|^2/2."""
+    x01, _ = gll_nodes(order)
+    X = mesh.node_coordinates(x01)
+    x, y = X[0], X[1]
+    gamma, R_gas = 1.4, 287.05
+    gm1 = gamma - 1.0
+    cp = gamma * R_gas / gm1
+    pres_inf = rho_inf * R_gas * temp_inf
+    r2rad = (x * x + y * y) / (radius * radius)
+    velX = vel_inf * (1.0 - beta * y / radius * np.exp(-0.5 * r2rad))
+    velY = vel_inf * (beta * x / radius * np.exp(-0.5 * r2rad))
+    temp = temp_inf - 0.5 * (vel_inf * beta) ** 2 / cp * np.exp(-r2rad)
+    temp = np.maximum(temp, 0.2 * temp_inf)
+    den = rho_inf * (temp / temp_inf) ** (1.0 / gm1)
+    pres = pres_inf * (temp / temp_inf) ** (gamma / gm1)
+    rhoE = pres / gm1 + 0.5 * den * (velX * velX + velY * velY)
+    return _pack(den, [den * velX, den * velY], rhoE)
+
+
 def four_quadrant_riemann(mesh: BoxMesh, order: int, gamma: float = 1.4, split=(0.5, 0.5)) -> np.ndarray:
     """2-D four-quadrant Riemann problem (configuration with quadrant states
     NE (1.5,0,0,1.5), NW (0.5323,1.206,0,0.3), SW (0.138,1.206,1.206,0.029),
